@@ -1,0 +1,243 @@
+/*
+ * lightne.h -- C-ABI of the MI355X LightNE embedding pipeline.
+ *
+ * The reference exposes its three stages as in-process C++ templates; each
+ * entry point below replaces one of them and keeps its argument meaning:
+ *
+ *   lightne_sparsify / lightne_sparsify_counts
+ *       path_embed::generate_trunc_log_matrix_v2  (LightNE/PathEmbed.h:81-84)
+ *   lightne_rsvd
+ *       mkl_redsvd::MKLRedSVD<FP>::MKLRedSVD + run (LightNE/mklredsvd/mklredsvd.h:8-26,
+ *                                                   mklredsvd.cc:54-81,93-403)
+ *   lightne_u_sigma_root
+ *       path_embed::compute_u_sigma_root          (LightNE/PathEmbed.h:371-408)
+ *   lightne_propagate / lightne_chebyshev_expansion
+ *       spectral_propagation::chebyshev_expansion (LightNE/SpectralPropagation.h:45-229)
+ *   lightne_embed_dense
+ *       spectral_propagation::get_embedding_dense (LightNE/SpectralPropagation.h:16-43)
+ *   lightne_netsmf
+ *       path_embed::NetSMF                        (LightNE/PathEmbed.h:494-563)
+ *   lightne_graph_read_adj
+ *       gbbs_io::read_unweighted_symmetric_graph  (ligra/graph_io.h:185-252)
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - every call returns an int status (LIGHTNE_OK = 0); nothing calls exit()
+ *     (the reference exits with status 0 on a full table, par_table.h:104-107,
+ *     and on LAPACK failure, mklredsvd/util.cc:59-62);
+ *   - array arguments are DEVICE pointers unless the name ends in _host;
+ *   - outputs whose size is data dependent (the sampled table, the CSR) are
+ *     returned as library-owned handles with _info/_copy/_free accessors;
+ *   - one context per HIP stream; calls on one context are serialised by the
+ *     caller; contexts are independent;
+ *   - propagation does not modify its input (the reference's in-place
+ *     `a -= conv`, SpectralPropagation.h:201-208, is not observable because the
+ *     stage-2 embedding is saved before stage 3 runs, LightNE.cc:108-118).
+ */
+#ifndef LIGHTNE_H
+#define LIGHTNE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define LIGHTNE_OK 0
+#define LIGHTNE_ERR_BAD_ARG 1
+#define LIGHTNE_ERR_OOM 2
+#define LIGHTNE_ERR_TABLE_FULL 3
+#define LIGHTNE_ERR_HIP 4
+#define LIGHTNE_ERR_BAD_GRAPH 5
+#define LIGHTNE_ERR_NUMERIC 6
+#define LIGHTNE_ERR_IO 7
+#define LIGHTNE_ERR_NO_DEVICE 8
+
+/* panel precision */
+#define LIGHTNE_FP32 0
+#define LIGHTNE_BF16 1
+
+typedef struct lightne_ctx lightne_ctx;
+typedef struct lightne_table lightne_table;
+typedef struct lightne_csr lightne_csr;
+
+/* ---- context --------------------------------------------------------------
+ * No reference counterpart (the reference is a single process-wide
+ * computation); `stream` is a hipStream_t (NULL = the default stream). */
+int lightne_ctx_create(int device, void* stream, lightne_ctx** out);
+int lightne_ctx_destroy(lightne_ctx* ctx);
+int lightne_ctx_synchronize(lightne_ctx* ctx);
+/* Text of the last error on this context ("" if none). */
+const char* lightne_last_error(const lightne_ctx* ctx);
+/* Text of the last error raised outside any context (create, graph IO). */
+const char* lightne_global_error(void);
+/* Library version string. */
+const char* lightne_version(void);
+
+/* ---- graph ----------------------------------------------------------------
+ * Symmetric CSR in device memory; neighbours in input (file) order, which the
+ * sampler's i-th-neighbour draw depends on (ligra/vertex.h:296-300).
+ * offsets[n] == m == number of directed entries (GA.m). */
+typedef struct {
+  int64_t n;
+  int64_t m;
+  const int64_t* offsets; /* n+1 */
+  const uint32_t* edges;  /* m   */
+} lightne_graph;
+
+/* ---- stage 1: sparsifier (PathEmbed.h:81-369) ------------------------------ */
+typedef struct {
+  uint64_t walk_len;        /* -walklen (T)                                  */
+  uint64_t negative;        /* -negative (b)                                 */
+  int32_t sample;           /* -sample: 0 = plain, 1 = eff.-resistance       */
+  int32_t upper;            /* -upper: store the upper triangle only         */
+  float sample_ratio;       /* -sample_ratio (kept as float, LightNE.cc:42)  */
+  float mem_ratio;          /* -mem_ratio                                    */
+  uint64_t tablesz;         /* -tablesz                                      */
+  uint64_t seed;            /* pbbs::random state (reference: time(0))      */
+  uint64_t max_table_bytes; /* cap on the hash table (0 = no cap)            */
+} lightne_sparsify_params;
+
+typedef struct {
+  /* host scalars, PathEmbed.h:89,99,108-111 */
+  double logn, M, one_over_lambda, fractional_bits;
+  uint64_t exp_rv_rounded, table_size;
+  /* sampler */
+  uint64_t table_capacity; /* slots allocated on the device                */
+  uint64_t attempts;       /* per-edge loop trips, sum of exp_rv           */
+  uint64_t samples;        /* emitted samples                              */
+  uint64_t walk_steps;     /* total random-walk steps                      */
+  uint64_t distinct_keys;  /* U                                            */
+  /* trunc-log / CSR, PathEmbed.h:175-311 */
+  double M_exact, factor;
+  uint64_t kept_upper;     /* keys with log(adj) > 0                       */
+  uint64_t nnz;            /* CSR entries                                  */
+  /* device time in seconds (HIP events on the context stream) */
+  double t_sample, t_table, t_csr;
+} lightne_sparsify_stats;
+
+/* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */
+int lightne_sample_scalars(int64_t n, int64_t m, const lightne_sparsify_params* p, lightne_sparsify_stats* out);
+
+/* Sampler + table only: the key->(count, weight) multiset before trunc-log.
+ * Keys are (min(x,z) << 32) + max(x,z) (PathEmbed.h:122-126); the handle
+ * returns them sorted ascending. */
+int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                            lightne_table** out, lightne_sparsify_stats* stats);
+int lightne_table_info(const lightne_table* t, uint64_t* nkeys);
+/* keys: u64[nkeys], counts: u32[nkeys], weights: f64[nkeys] (may be NULL);
+ * destination pointers may be host or device. */
+int lightne_table_copy(lightne_ctx* ctx, const lightne_table* t, uint64_t* keys, uint32_t* counts, double* weights);
+int lightne_table_free(lightne_table* t);
+
+/* Full stage 1: sampler, table, trunc-log, CSR with rows sorted by column.
+ * (generate_trunc_log_matrix_v2 with rows_start != NULL, PathEmbed.h:224-316.) */
+int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                     lightne_csr** out, lightne_sparsify_stats* stats);
+
+/* CSR handle: int64 row_ptr[n+1], int32 col[nnz], float val[nnz] on the device. */
+int lightne_csr_create(lightne_ctx* ctx, int64_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                       const float* val, lightne_csr** out);
+int lightne_csr_info(const lightne_csr* c, int64_t* n, int64_t* nnz);
+int lightne_csr_device_ptrs(const lightne_csr* c, const int64_t** row_ptr, const int32_t** col, const float** val);
+int lightne_csr_copy(lightne_ctx* ctx, const lightne_csr* c, int64_t* row_ptr, int32_t* col, float* val);
+int lightne_csr_free(lightne_csr* c);
+
+/* ---- stage 2: randomized SVD (mklredsvd.cc:93-403) -------------------------- */
+typedef struct {
+  int64_t rank;             /* -rank: sketch width k (d + oversampling)       */
+  int32_t power_iters;      /* q >= 1; 1 = the reference schedule             */
+  int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 SpMM panels        */
+  int32_t random_project_only; /* -random_project_only (mklredsvd.cc:254-258) */
+  int32_t upper;            /* A holds only the upper triangle (symmetric)    */
+  uint64_t seed;            /* Gaussian sketch seed (reference: VSL seed 777) */
+} lightne_rsvd_params;
+
+typedef struct {
+  double t_sketch, t_spmm, t_qr, t_gemm, t_small_svd, t_total; /* seconds */
+  int32_t spmm_calls, qr_calls;
+} lightne_rsvd_stats;
+
+/* U_out: n x rank row-major (device).  S_host: rank singular values,
+ * descending (NULL allowed when random_project_only, which returns Y = A*Omega
+ * in U_out as the reference returns it in matU with S = NULL). */
+int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* A, const lightne_rsvd_params* p, float* U_out,
+                 float* S_host, lightne_rsvd_stats* stats);
+
+/* emb[:, j] = sqrt(S_j) * U[:, j] for j < dim (U has row stride rank), then the
+ * row L2 normalisation (norm > 1e-5) when normalize != 0.  S_host NULL means
+ * S = 1 (PathEmbed.h:371-408). */
+int lightne_u_sigma_root(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank,
+                         int64_t dim, int32_t normalize, float* emb_out);
+
+/* ---- stage 3: spectral propagation (SpectralPropagation.h:45-229) ----------- */
+typedef struct {
+  int64_t order;            /* -order K                                      */
+  float mu;                 /* reference hard-codes 0.2 (LightNE.cc:114)     */
+  float theta;              /* reference hard-codes 0.5 (LightNE.cc:114)     */
+  int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 iterates          */
+} lightne_prop_params;
+
+typedef struct {
+  double t_cheb, t_aggregate, t_svd, t_total;
+  int32_t spmm_calls;
+} lightne_prop_stats;
+
+/* y = a - sum_{i<K} c_i T_i a and z = (A + I) y (SpectralPropagation.h:94-221).
+ * a: n x dim (device, not modified).  y_out / z_out may be NULL. */
+int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                      const lightne_prop_params* p, float* y_out, float* z_out, lightne_prop_stats* stats);
+
+/* get_embedding_dense: thin SVD of mat (n x rank), emb = rownorm(U[:, :dim] sqrt(S)).
+ * S_host (rank values, descending) may be NULL. */
+int lightne_embed_dense(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim,
+                        float* emb_out, float* S_host);
+
+/* chebyshev_expansion(a, GA, rank=dim, dim, order, mu, s): propagation, then
+ * get_embedding_dense.  emb_out: n x dim. */
+int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                                const lightne_prop_params* p, float* emb_out, lightne_prop_stats* stats);
+
+/* Adjoint of lightne_propagate's y (z = (A+I)y when with_aggregate != 0):
+ * given grad_out = dL/dy (or dL/dz), returns dL/da (n x dim, device) and
+ * dL/dmu, dL/dtheta (host doubles).  The forward Chebyshev iterates are
+ * recomputed from checkpoints taken every `checkpoint_every` steps.
+ * No reference counterpart: the reference has no gradients. */
+int lightne_propagate_backward(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                               const lightne_prop_params* p, int32_t with_aggregate, const float* grad_out,
+                               float* grad_a, double* grad_mu_host, double* grad_theta_host,
+                               int32_t checkpoint_every);
+
+/* ---- whole NetSMF stage 1 + 2 (PathEmbed.h:494-563) -------------------------- */
+typedef struct {
+  double t_sparsify, t_svd, t_total;
+  lightne_sparsify_stats sparsify;
+  lightne_rsvd_stats rsvd;
+} lightne_netsmf_stats;
+
+int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* sp,
+                   const lightne_rsvd_params* rp, int64_t dim, int32_t normalize, float* emb_out,
+                   float* S_host, lightne_netsmf_stats* stats);
+
+/* ---- host graph utilities (no device work) ---------------------------------- */
+/* Ligra AdjacencyGraph text reader/writer (ligra/graph_io.h:185-231).
+ * *offsets_host has n+1 entries; free both with lightne_host_free. */
+int lightne_graph_read_adj(const char* path, int64_t* n, int64_t* m, int64_t** offsets_host,
+                           uint32_t** edges_host);
+int lightne_graph_write_adj(const char* path, int64_t n, const int64_t* offsets_host, const uint32_t* edges_host);
+/* Seeded synthetic graphs: undirected, no self loops, no duplicates, each
+ * neighbour list sorted.  Chung-Lu with power-law expected degrees
+ * w_i ~ (i + i0)^(-1/(beta-1)) scaled to mean `mean_deg` and capped at
+ * `max_deg`; R-MAT with quadrant probabilities (a, b, c, 1-a-b-c). */
+int lightne_gen_chung_lu(int64_t n, double beta, double mean_deg, double max_deg, uint64_t seed, int64_t* m,
+                         int64_t** offsets_host, uint32_t** edges_host);
+int lightne_gen_rmat(int32_t scale, double edge_factor, double a, double b, double c, uint64_t seed,
+                     int64_t* n, int64_t* m, int64_t** offsets_host, uint32_t** edges_host);
+void lightne_host_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGHTNE_H */

@@ -1,0 +1,419 @@
+"""Host-side mirror of LightNE's stage API over the liblightne_amd C-ABI.
+
+Names and argument meaning follow the reference:
+
+* ``generate_trunc_log_matrix``  -- path_embed::generate_trunc_log_matrix_v2
+  (LightNE/PathEmbed.h:81-84)
+* ``MKLRedSVD``                  -- mkl_redsvd::MKLRedSVD<FP> (LightNE/mklredsvd/mklredsvd.h:8-26)
+* ``compute_u_sigma_root``       -- path_embed::compute_u_sigma_root (LightNE/PathEmbed.h:371-408)
+* ``chebyshev_expansion``        -- spectral_propagation::chebyshev_expansion
+  (LightNE/SpectralPropagation.h:45-229)
+* ``get_embedding_dense``        -- spectral_propagation::get_embedding_dense
+  (LightNE/SpectralPropagation.h:16-43)
+* ``NetSMF``                     -- path_embed::NetSMF (LightNE/PathEmbed.h:494-563)
+* ``read_unweighted_symmetric_graph`` -- gbbs_io::read_unweighted_symmetric_graph
+  (ligra/graph_io.h:233-252)
+
+Device buffers are torch tensors on ``cuda``; torch only provides memory and
+the stream, every computation runs in the HIP library.  Errors raise
+``LightNEError`` (the reference calls exit(0) instead, par_table.h:104-107).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BF16, FP32, Graph, LightNEError, NetsmfStats, PropParams, PropStats, RsvdParams, RsvdStats,
+                   SparsifyParams, SparsifyStats, check, load)
+
+
+def _torch():
+    import torch  # imported lazily: host-only helpers work without torch
+
+    return torch
+
+
+def _stats_dict(s) -> dict:
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        out[name] = _stats_dict(v) if isinstance(v, C.Structure) else v
+    return out
+
+
+# ---------------------------------------------------------------------------
+# host graphs
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class HostGraph:
+    """Symmetric CSR on the host: offsets int64[n+1], edges uint32[m] (input order)."""
+
+    offsets: np.ndarray
+    edges: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.offsets.shape[0] - 1)
+
+    @property
+    def m(self) -> int:
+        return int(self.edges.shape[0])
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.offsets)
+
+    def to_device(self, device: str = "cuda") -> "DeviceGraph":
+        torch = _torch()
+        off = torch.from_numpy(self.offsets.astype(np.int64)).to(device)
+        # torch has no uint32 kernels we need; int32 storage carries the same bits
+        edg = torch.from_numpy(self.edges.astype(np.uint32).view(np.int32)).to(device)
+        return DeviceGraph(off, edg, self.n, self.m)
+
+
+@dataclasses.dataclass
+class DeviceGraph:
+    offsets: "object"  # torch.int64 [n+1]
+    edges: "object"  # torch.int32 [m], bits of uint32
+    n: int
+    m: int
+
+    def desc(self) -> Graph:
+        return Graph(self.n, self.m, self.offsets.data_ptr(), self.edges.data_ptr() if self.m else None)
+
+
+def _take_host_graph(n: int, m: int, off_p: C.c_void_p, edg_p: C.c_void_p) -> HostGraph:
+    lib = load()
+    try:
+        off = np.ctypeslib.as_array(C.cast(off_p, C.POINTER(C.c_int64)), shape=(n + 1,)).copy()
+        edg = (np.ctypeslib.as_array(C.cast(edg_p, C.POINTER(C.c_uint32)), shape=(m,)).copy()
+               if m else np.zeros(0, np.uint32))
+    finally:
+        lib.lightne_host_free(off_p)
+        lib.lightne_host_free(edg_p)
+    return HostGraph(off, edg)
+
+
+def read_unweighted_symmetric_graph(path: str) -> HostGraph:
+    """Read a Ligra AdjacencyGraph text file (ligra/graph_io.h:185-252)."""
+    lib = load()
+    n, m = C.c_int64(), C.c_int64()
+    op, ep = C.c_void_p(), C.c_void_p()
+    check(lib.lightne_graph_read_adj(path.encode(), C.byref(n), C.byref(m), C.byref(op), C.byref(ep)))
+    return _take_host_graph(n.value, m.value, op, ep)
+
+
+def write_adjacency_graph(path: str, g: HostGraph) -> None:
+    lib = load()
+    off = np.ascontiguousarray(g.offsets, dtype=np.int64)
+    edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
+    check(lib.lightne_graph_write_adj(path.encode(), g.n, off.ctypes.data, edg.ctypes.data if g.m else None))
+
+
+def chung_lu(n: int, beta: float, mean_deg: float, max_deg: float, seed: int) -> HostGraph:
+    lib = load()
+    m = C.c_int64()
+    op, ep = C.c_void_p(), C.c_void_p()
+    check(lib.lightne_gen_chung_lu(n, beta, mean_deg, max_deg, seed, C.byref(m), C.byref(op), C.byref(ep)))
+    return _take_host_graph(n, m.value, op, ep)
+
+
+def rmat(scale: int, edge_factor: float, a: float, b: float, c: float, seed: int) -> HostGraph:
+    lib = load()
+    n, m = C.c_int64(), C.c_int64()
+    op, ep = C.c_void_p(), C.c_void_p()
+    check(lib.lightne_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(n), C.byref(m), C.byref(op), C.byref(ep)))
+    return _take_host_graph(n.value, m.value, op, ep)
+
+
+# ---------------------------------------------------------------------------
+# context
+# ---------------------------------------------------------------------------
+class Context:
+    """One library context bound to a device and torch's current stream on it."""
+
+    def __init__(self, device: int = 0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise LightNEError(_lib.ERR_NO_DEVICE, "no GPU visible to torch")
+        self.lib = load()
+        self.device = device
+        with torch.cuda.device(device):
+            self.stream = torch.cuda.current_stream(device)
+        self.ptr = C.c_void_p()
+        check(self.lib.lightne_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(self.ptr)))
+
+    def check(self, rc: int) -> None:
+        check(rc, self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.lightne_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX: dict = {}
+
+
+def context(device: int = 0) -> Context:
+    c = _CTX.get(device)
+    if c is None:
+        c = Context(device)
+        _CTX[device] = c
+    return c
+
+
+def _dev(t) -> int:
+    return t.data_ptr()
+
+
+# ---------------------------------------------------------------------------
+# stage 1
+# ---------------------------------------------------------------------------
+def sample_scalars(n: int, m: int, walk_len: int = 10, sample_ratio: float = 1.0, mem_ratio: float = 1.0,
+                   table_size: int = 0) -> dict:
+    """Host evaluation of PathEmbed.h:89,99,108-117 (logn, M, 1/lambda, ...)."""
+    lib = load()
+    p = SparsifyParams(walk_len=walk_len, negative=1, sample=0, upper=0, sample_ratio=sample_ratio,
+                       mem_ratio=mem_ratio, tablesz=table_size, seed=0, max_table_bytes=0)
+    st = SparsifyStats()
+    check(lib.lightne_sample_scalars(n, m, C.byref(p), C.byref(st)))
+    return _stats_dict(st)
+
+
+def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
+                     max_table_bytes) -> SparsifyParams:
+    return SparsifyParams(walk_len=walk_len, negative=negative, sample=int(bool(sample)), upper=int(bool(upper)),
+                          sample_ratio=sample_ratio, mem_ratio=mem_ratio, tablesz=table_size,
+                          seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes)
+
+
+@dataclasses.dataclass
+class SparseTable:
+    keys: np.ndarray  # uint64, sorted: (min(x,z) << 32) + max(x,z)
+    counts: np.ndarray  # uint32 samples per key
+    weights: np.ndarray  # float64 summed weight per key
+    stats: dict
+
+
+def sparsify_counts(graph: DeviceGraph, seed: int, walk_len: int = 10, sample: bool = False,
+                    negative: int = 1, sample_ratio: float = 1.0, mem_ratio: float = 1.0, table_size: int = 0,
+                    max_table_bytes: int = 0, ctx: Optional[Context] = None) -> SparseTable:
+    """Key -> (count, weight) multiset of the sampler (PathEmbed.h:128-188)."""
+    ctx = ctx or context()
+    p = _sparsify_params(seed, walk_len, False, sample, table_size, negative, sample_ratio, mem_ratio,
+                         max_table_bytes)
+    st = SparsifyStats()
+    tab = C.c_void_p()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_sparsify_counts(ctx.ptr, C.byref(g), C.byref(p), C.byref(tab), C.byref(st)))
+    try:
+        nk = C.c_uint64()
+        check(ctx.lib.lightne_table_info(tab, C.byref(nk)))
+        keys = np.empty(nk.value, np.uint64)
+        cnt = np.empty(nk.value, np.uint32)
+        wgh = np.empty(nk.value, np.float64)
+        ctx.check(ctx.lib.lightne_table_copy(ctx.ptr, tab, keys.ctypes.data, cnt.ctypes.data, wgh.ctypes.data))
+    finally:
+        ctx.lib.lightne_table_free(tab)
+    return SparseTable(keys, cnt, wgh, _stats_dict(st))
+
+
+class CSR:
+    """Device CSR owned by the library (int64 row_ptr, int32 col, float val)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.handle = handle
+        n, nnz = C.c_int64(), C.c_int64()
+        check(ctx.lib.lightne_csr_info(handle, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    @classmethod
+    def from_arrays(cls, row_ptr, col, val, ctx: Optional[Context] = None) -> "CSR":
+        """Import a CSR (numpy host arrays or torch device tensors)."""
+        ctx = ctx or context()
+        keep = []
+
+        def ptr(a, dt):
+            if isinstance(a, np.ndarray):
+                a = np.ascontiguousarray(a, dtype=dt)
+                keep.append(a)
+                return a.ctypes.data
+            return a.data_ptr()
+
+        n = int(len(row_ptr) - 1)
+        nnz = int(len(col))
+        h = C.c_void_p()
+        ctx.check(ctx.lib.lightne_csr_create(ctx.ptr, n, nnz, ptr(row_ptr, np.int64), ptr(col, np.int32),
+                                             ptr(val, np.float32), C.byref(h)))
+        return cls(ctx, h)
+
+    def to_numpy(self):
+        rp = np.empty(self.n + 1, np.int64)
+        col = np.empty(self.nnz, np.int32)
+        val = np.empty(self.nnz, np.float32)
+        self.ctx.check(self.ctx.lib.lightne_csr_copy(self.ctx.ptr, self.handle, rp.ctypes.data, col.ctypes.data,
+                                                     val.ctypes.data))
+        return rp, col, val
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx.lib.lightne_csr_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def generate_trunc_log_matrix(graph: DeviceGraph, seed: int, walks_per_edge: int = 100, walk_len: int = 10,
+                              upper: bool = False, sample: bool = False, table_size: int = 0, negative: int = 1,
+                              sample_ratio: float = 1.0, mem_ratio: float = 1.0, max_table_bytes: int = 0,
+                              ctx: Optional[Context] = None):
+    """Stage 1 (PathEmbed.h:81-316): returns (CSR, stats).  ``walks_per_edge`` is
+    accepted and unused, as in the reference (PathEmbed.h:107-108)."""
+    del walks_per_edge
+    ctx = ctx or context()
+    p = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
+                         max_table_bytes)
+    st = SparsifyStats()
+    h = C.c_void_p()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_sparsify(ctx.ptr, C.byref(g), C.byref(p), C.byref(h), C.byref(st)))
+    return CSR(ctx, h), _stats_dict(st)
+
+
+# ---------------------------------------------------------------------------
+# stage 2
+# ---------------------------------------------------------------------------
+class MKLRedSVD:
+    """Randomized SVD of a symmetric sparse matrix (mklredsvd.h:8-26 / .cc:93-403).
+
+    After ``run()``: ``matU`` (torch n x rank, device) and ``S`` (numpy rank),
+    ``S`` is None with ``random_project_only`` as in the reference."""
+
+    def __init__(self, csr: CSR, rank: int, power_iters: int = 1, precision: int = FP32,
+                 random_project_only: bool = False, upper: bool = False, seed: int = 777,
+                 ctx: Optional[Context] = None):
+        self.csr = csr
+        self.ctx = ctx or csr.ctx
+        self.params = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
+                                 random_project_only=int(bool(random_project_only)), upper=int(bool(upper)),
+                                 seed=seed)
+        self.matU = None
+        self.S = None
+        self.stats = None
+
+    def run(self):
+        torch = _torch()
+        n, k = self.csr.n, self.params.rank
+        U = torch.empty((n, k), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        S = np.zeros(k, np.float32)
+        st = RsvdStats()
+        self.ctx.check(self.ctx.lib.lightne_rsvd(self.ctx.ptr, self.csr.handle, C.byref(self.params), _dev(U),
+                                                 S.ctypes.data, C.byref(st)))
+        self.matU = U
+        self.S = None if self.params.random_project_only else S
+        self.stats = _stats_dict(st)
+        return self
+
+
+def compute_u_sigma_root(U, S, n: int, rank: int, dim: int, normalize: bool, ctx: Optional[Context] = None):
+    """emb[:, j] = sqrt(S_j) U[:, j], optional row normalisation (PathEmbed.h:371-408)."""
+    torch = _torch()
+    ctx = ctx or context()
+    emb = torch.empty((n, dim), dtype=torch.float32, device=U.device)
+    s = None if S is None else np.ascontiguousarray(S, dtype=np.float32)
+    ctx.check(ctx.lib.lightne_u_sigma_root(ctx.ptr, _dev(U), s.ctypes.data if s is not None else None, n, rank, dim,
+                                           int(bool(normalize)), _dev(emb)))
+    return emb
+
+
+def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, upper: bool = False,
+           sample: bool = False, rank: int = 256, dim: int = 128, analyze: bool = False, table_size: int = 0,
+           negative: int = 1, normalize: bool = False, sample_ratio: float = 1.0, mem_ratio: float = 1.0,
+           random_project_only: bool = False, seed: int = 0, svd_seed: int = 777, power_iters: int = 1,
+           precision: int = FP32, max_table_bytes: int = 0, ctx: Optional[Context] = None):
+    """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats)."""
+    del walks_per_edge, analyze
+    torch = _torch()
+    ctx = ctx or context()
+    sp = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
+                          max_table_bytes)
+    rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
+                    random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed)
+    emb = torch.empty((graph.n, dim), dtype=torch.float32, device=f"cuda:{ctx.device}")
+    S = np.zeros(rank, np.float32)
+    st = NetsmfStats()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_netsmf(ctx.ptr, C.byref(g), C.byref(sp), C.byref(rp), dim, int(bool(normalize)),
+                                     _dev(emb), S.ctypes.data, C.byref(st)))
+    return emb, S, _stats_dict(st)
+
+
+# ---------------------------------------------------------------------------
+# stage 3
+# ---------------------------------------------------------------------------
+def propagate(a, graph: DeviceGraph, order: int = 10, mu: float = 0.2, theta: float = 0.5,
+              precision: int = FP32, aggregate: bool = True, ctx: Optional[Context] = None):
+    """y = a - sum_i c_i T_i a and z = (A + I) y (SpectralPropagation.h:94-221).
+    ``a`` (n x dim float32 on the device) is not modified.  Returns (y, z, stats)."""
+    torch = _torch()
+    ctx = ctx or context()
+    a = a.contiguous()
+    n, dim = a.shape
+    y = torch.empty_like(a)
+    z = torch.empty_like(a) if aggregate else None
+    p = PropParams(order=order, mu=mu, theta=theta, precision=precision)
+    st = PropStats()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_propagate(ctx.ptr, C.byref(g), _dev(a), dim, C.byref(p), _dev(y),
+                                        _dev(z) if z is not None else None, C.byref(st)))
+    return y, z, _stats_dict(st)
+
+
+def get_embedding_dense(mat, n: int, rank: int, dim: int, ctx: Optional[Context] = None):
+    """Thin SVD of mat (n x rank), emb = rownorm(U[:, :dim] sqrt(S)) (SpectralPropagation.h:16-43).
+    Returns (emb, S)."""
+    torch = _torch()
+    ctx = ctx or context()
+    mat = mat.contiguous()
+    emb = torch.empty((n, dim), dtype=torch.float32, device=mat.device)
+    S = np.zeros(rank, np.float32)
+    ctx.check(ctx.lib.lightne_embed_dense(ctx.ptr, _dev(mat), n, rank, dim, _dev(emb), S.ctypes.data))
+    return emb, S
+
+
+def chebyshev_expansion(a, graph: DeviceGraph, rank: int, dim: int, order: int = 10, mu: float = 0.2,
+                        s: float = 0.5, precision: int = FP32, ctx: Optional[Context] = None):
+    """Stage 3 (SpectralPropagation.h:45-229), non-mutating.  Returns (emb, stats)."""
+    torch = _torch()
+    if rank != dim:
+        raise LightNEError(_lib.ERR_BAD_ARG, "chebyshev_expansion: the runner passes rank == dim (LightNE.cc:114)")
+    ctx = ctx or context()
+    a = a.contiguous()
+    n = a.shape[0]
+    emb = torch.empty((n, dim), dtype=torch.float32, device=a.device)
+    p = PropParams(order=order, mu=mu, theta=s, precision=precision)
+    st = PropStats()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_chebyshev_expansion(ctx.ptr, C.byref(g), _dev(a), dim, C.byref(p), _dev(emb),
+                                                  C.byref(st)))
+    return emb, _stats_dict(st)
+
+
+def bessel_i(k: int, x: float) -> float:
+    """Modified Bessel I_k(x) as used for the Chebyshev weights."""
+    return load().lightne_bessel_i(k, x)

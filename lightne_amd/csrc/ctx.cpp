@@ -1,0 +1,144 @@
+// Context, error reporting and scratch management for liblightne_amd.
+#include "lightne_internal.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+}  // namespace
+
+int ctx_fail(lightne_ctx* ctx, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  std::lock_guard<std::mutex> g(g_err_mu);
+  g_err = buf;
+  return code;
+}
+
+int global_fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  std::lock_guard<std::mutex> g(g_err_mu);
+  g_err = buf;
+  return code;
+}
+
+void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes) {
+  auto& b = ctx->bufs[name];
+  if (b.bytes >= bytes && b.p) return b.p;
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes ? bytes : 256;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    ctx_fail(ctx, LIGHTNE_ERR_OOM, "out of device memory allocating %zu bytes for %s", want, name);
+    return nullptr;
+  }
+  b.bytes = want;
+  return b.p;
+}
+
+void ctx_release(lightne_ctx* ctx, const char* name) {
+  auto it = ctx->bufs.find(name);
+  if (it == ctx->bufs.end()) return;
+  if (it->second.p) (void)hipFree(it->second.p);
+  ctx->bufs.erase(it);
+}
+
+int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what) {
+  if (m.p) {
+    (void)hipFree(m.p);
+    m.p = nullptr;
+  }
+  if (hipMalloc(&m.p, bytes ? bytes : 256) != hipSuccess) {
+    (void)hipGetLastError();
+    m.p = nullptr;
+    return ctx_fail(ctx, LIGHTNE_ERR_OOM, "out of device memory allocating %zu bytes for %s", bytes, what);
+  }
+  return LIGHTNE_OK;
+}
+
+double ev_seconds(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0;
+  }
+  return ms * 1e-3;
+}
+
+extern "C" {
+
+const char* lightne_version(void) { return "lightne_amd 0.1 (gfx950)"; }
+
+int lightne_ctx_create(int device, void* stream, lightne_ctx** out) {
+  if (!out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return global_fail(LIGHTNE_ERR_NO_DEVICE, "lightne_ctx_create: no HIP device visible");
+  }
+  if (device < 0 || device >= ndev)
+    return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_ctx_create: device %d out of range [0,%d)", device, ndev);
+  if (hipSetDevice(device) != hipSuccess)
+    return global_fail(LIGHTNE_ERR_HIP, "lightne_ctx_create: hipSetDevice(%d) failed", device);
+  auto* ctx = new lightne_ctx();
+  ctx->device = device;
+  ctx->stream = (hipStream_t)stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return global_fail(LIGHTNE_ERR_HIP, "lightne_ctx_create: hipEventCreate failed");
+    }
+  }
+  *out = ctx;
+  return LIGHTNE_OK;
+}
+
+int lightne_ctx_destroy(lightne_ctx* ctx) {
+  if (!ctx) return LIGHTNE_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+  return LIGHTNE_OK;
+}
+
+int lightne_ctx_synchronize(lightne_ctx* ctx) {
+  if (!ctx) return global_fail(LIGHTNE_ERR_BAD_ARG, "ctx is NULL");
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+const char* lightne_last_error(const lightne_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+const char* lightne_global_error(void) {
+  static thread_local std::string copy;
+  std::lock_guard<std::mutex> g(g_err_mu);
+  copy = g_err;
+  return copy.c_str();
+}
+
+void lightne_host_free(void* p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,371 @@
+// Dense panel kernels for the randomized SVD and the tall SVD of stage 3.
+//
+//  * k_gaussian      : counter-based Gaussian sketch (Philox4x32-10 +
+//                      Box-Muller in double), replacing MKL VSL SFMT19937 +
+//                      Box-Muller2 (LightNE/mklredsvd/mklredsvd.cc:141,229-231,
+//                      mklredsvd/util.cc:21-38), whose stream cannot be
+//                      reproduced.  Element e of stream t is fully determined by
+//                      (seed, t, e), so CPU and GPU draw the same matrix.
+//  * k_gram_f64      : G = X^T Y in FP64 on v_mfma_f64_16x16x4f64 (the Gram of
+//                      CholeskyQR2 and C = Z^T B of mklredsvd.cc:327-345).
+//  * k_panel_small   : Out = X W for a tall panel X and a small W on
+//                      v_mfma_f32_16x16x4f32 (Z = B P, U = Z U_C,
+//                      mklredsvd.cc:294-312,379-397, and Q = Y R^-1), with an
+//                      optional row L2-normalisation epilogue
+//                      (compute_u_sigma_root, PathEmbed.h:390-406).
+#include "lightne_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * p >= count) return;
+  uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+  const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * log(u1));
+  const double t = 6.283185307179586476925286766559 * u2;
+  out[2 * p] = (float)(r * cos(t));
+  if (2 * p + 1 < count) out[2 * p + 1] = (float)(r * sin(t));
+}
+
+// ---------------------------------------------------------------------------
+// Gram: rows are staged through LDS in slabs of GR_ROWS; every wave owns a
+// fixed subset of the 16x16 output tiles and walks the slab 4 rows at a time.
+// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the accumulator holds D[4*(l/16)+v][l%16] in register v.
+// ---------------------------------------------------------------------------
+constexpr int GR_ROWS = 32;
+constexpr int GR_THREADS = 256;
+constexpr int GR_MAX_PAIRS_PER_WAVE = 24;
+
+__global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict__ X, const float* __restrict__ Y,
+                                                         int64_t n, int k1, int k2, int npairs,
+                                                         const int2* __restrict__ pairs, int64_t rows_per_block,
+                                                         double* __restrict__ part) {
+  extern __shared__ float s_mem[];
+  float* sX = s_mem;
+  float* sY = (X == Y) ? s_mem : s_mem + GR_ROWS * k1;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = GR_THREADS / 64;
+  // pairs of this wave: wid, wid + nw, ...
+  doublex4 acc[GR_MAX_PAIRS_PER_WAVE];
+#pragma unroll
+  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) acc[q] = doublex4{0.0, 0.0, 0.0, 0.0};
+  const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r_end = min(n, r_begin + rows_per_block);
+  const int li = lane & 15, lk = lane >> 4;
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += GR_ROWS) {
+    __syncthreads();
+    const int nrow = (int)min<int64_t>(GR_ROWS, r_end - r0);
+    for (int e = threadIdx.x; e < GR_ROWS * k1; e += GR_THREADS) {
+      const int rr = e / k1;
+      sX[e] = (rr < nrow) ? X[(r0 + rr) * k1 + (e - rr * k1)] : 0.f;
+    }
+    if (X != Y) {
+      for (int e = threadIdx.x; e < GR_ROWS * k2; e += GR_THREADS) {
+        const int rr = e / k2;
+        sY[e] = (rr < nrow) ? Y[(r0 + rr) * k2 + (e - rr * k2)] : 0.f;
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < GR_ROWS; s += 4) {
+      const float* xr = sX + (s + lk) * k1 + li;
+      const float* yr = sY + (s + lk) * k2 + li;
+#pragma unroll
+      for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
+        const int pi = wid + q * nw;
+        if (pi < npairs) {
+          const int2 pr = pairs[pi];
+          const double av = (double)xr[pr.x * 16];
+          const double bv = (double)yr[pr.y * 16];
+          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+        }
+      }
+    }
+  }
+  double* out = part + (int64_t)blockIdx.x * k1 * k2;
+#pragma unroll
+  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
+    const int pi = wid + q * nw;
+    if (pi < npairs) {
+      const int2 pr = pairs[pi];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[(int64_t)(pr.x * 16 + 4 * lk + v) * k2 + pr.y * 16 + li] = acc[q][v];
+    }
+  }
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int nparts, int64_t len, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0.0;
+  for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * len + i];
+  out[i] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Out(n x k2) = X(n x k1) W(k1 x k2).  One wave per 16-row slab, all k2
+// columns.  The contraction index is permuted so that lane l reads one float4
+// of its row per 4 MFMA steps: step (t, q) uses k = 16 t + 4 (l/16) + q for A
+// and the same k for B.
+// ---------------------------------------------------------------------------
+constexpr int PS_THREADS = 512;
+constexpr int PS_MAX_T2 = 16;  // k2 <= 256
+
+template <int T2>
+__global__ __launch_bounds__(PS_THREADS) void k_panel_small(const float* X, int64_t ldx, int64_t n,
+                                                            int k1, const float* __restrict__ W, int ldW,
+                                                            float* Out, int64_t ldo, int normalize) {
+  extern __shared__ float sW[];  // k1 x (k2 + 4): columns [0, k2) of this block of W
+  const int k2 = T2 * 16;
+  const int ldw = k2 + 4;
+  for (int e = threadIdx.x; e < k1 * k2; e += PS_THREADS) {
+    const int r = e / k2;
+    sW[r * ldw + (e - r * k2)] = W[(int64_t)r * ldW + (e - r * k2)];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t nslab = (n + 15) / 16;
+  const int64_t wave0 = ((int64_t)blockIdx.x * PS_THREADS + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * PS_THREADS) >> 6;
+  for (int64_t slab = wave0; slab < nslab; slab += nwaves) {
+    const int64_t row = slab * 16 + li;
+    const bool rv = row < n;
+    floatx4 acc[T2];
+#pragma unroll
+    for (int j = 0; j < T2; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < k1 / 16; ++t) {
+      float4 xa = rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float xs[4] = {xa.x, xa.y, xa.z, xa.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float* wr = sW + (16 * t + 4 * lk + q) * ldw + li;
+#pragma unroll
+        for (int j = 0; j < T2; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[q], wr[16 * j], acc[j], 0, 0, 0);
+      }
+    }
+    // acc[j][v] = Out[slab*16 + 4*lk + v][16 j + li]
+    if (normalize) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        float ss = 0.f;
+#pragma unroll
+        for (int j = 0; j < T2; ++j) ss += acc[j][v] * acc[j][v];
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) ss += __shfl_xor(ss, d, 64);
+        const float nrm = sqrtf(ss);
+        if (nrm > 1e-5f) {
+          const float inv = 1.0f / nrm;
+#pragma unroll
+          for (int j = 0; j < T2; ++j) acc[j][v] *= inv;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t orow = slab * 16 + 4 * lk + v;
+      if (orow < n) {
+#pragma unroll
+        for (int j = 0; j < T2; ++j) Out[orow * ldo + 16 * j + li] = acc[j][v];
+      }
+    }
+  }
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ in, __hip_bfloat16* __restrict__ out, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t u = __float_as_uint(in[i]);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  const unsigned short b = (unsigned short)(u >> 16);
+  out[i] = *reinterpret_cast<const __hip_bfloat16*>(&b);
+}
+
+// emb[i, j] = sqrt(S_j) U[i, j] (j < dim, U row stride rank), optional row
+// normalisation; one wave per row.
+__global__ void k_u_sigma_root(const float* __restrict__ U, const float* __restrict__ sq, int64_t n, int64_t rank,
+                               int64_t dim, int normalize, float* __restrict__ emb) {
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  float ss = 0.f;
+  for (int64_t j = lane; j < dim; j += 64) {
+    const float v = sq[j] * U[row * rank + j];
+    ss += v * v;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) ss += __shfl_xor(ss, d, 64);
+  const float nrm = sqrtf(ss);
+  const float inv = (normalize && nrm > 1e-5f) ? 1.0f / nrm : 1.0f;
+  for (int64_t j = lane; j < dim; j += 64) emb[row * dim + j] = sq[j] * U[row * rank + j] * inv;
+}
+
+}  // namespace
+
+void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* out) {
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int64_t p = 0; 2 * p < count; ++p) {
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+    const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    const double t = two_pi * u2;
+    out[2 * p] = (float)(r * std::cos(t));
+    if (2 * p + 1 < count) out[2 * p + 1] = (float)(r * std::sin(t));
+  }
+}
+
+int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out) {
+  if (count <= 0) return LIGHTNE_OK;
+  const int64_t pairs = (count + 1) / 2;
+  hipLaunchKernelGGL(k_gaussian, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, ctx->stream, seed, stream,
+                     count, out);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}
+
+int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+             double* G_host) {
+  if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 256)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: widths must be multiples of 16 in [16, 256]");
+  if (symmetric && (X != Y || k1 != k2)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: symmetric needs X == Y");
+  const int T1 = (int)(k1 / 16), T2 = (int)(k2 / 16);
+  std::vector<int2> pairs;
+  for (int a = 0; a < T1; ++a)
+    for (int b = 0; b < T2; ++b)
+      if (!symmetric || a <= b) pairs.push_back(make_int2(a, b));
+  const int npairs_all = (int)pairs.size();
+  const int per_launch = GR_MAX_PAIRS_PER_WAVE * (GR_THREADS / 64);
+  int nblk = std::max(1, ctx->num_cus * 2);
+  int64_t rpb = (n + nblk - 1) / nblk;
+  rpb = ((rpb + GR_ROWS - 1) / GR_ROWS) * GR_ROWS;
+  if (rpb == 0) rpb = GR_ROWS;
+  nblk = (int)std::max<int64_t>(1, (n + rpb - 1) / rpb);
+  const int64_t len = k1 * k2;
+  auto* dpairs = (int2*)ctx_buf(ctx, "gram_pairs", pairs.size() * sizeof(int2));
+  auto* part = (double*)ctx_buf(ctx, "gram_part", (size_t)nblk * len * sizeof(double));
+  auto* gsum = (double*)ctx_buf(ctx, "gram_sum", len * sizeof(double));
+  if (!dpairs || !part || !gsum) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+  const size_t lds = (size_t)GR_ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(float);
+  if (n > 0) {
+    // each launch owns a disjoint set of output tiles
+    for (int p0 = 0; p0 < npairs_all; p0 += per_launch) {
+      const int np = std::min(per_launch, npairs_all - p0);
+      hipLaunchKernelGGL(k_gram_f64, dim3(nblk), dim3(GR_THREADS), lds, ctx->stream, X, Y, n, (int)k1, (int)k2, np,
+                         (const int2*)(dpairs + p0), rpb, part);
+      LN_LAUNCH(ctx);
+    }
+  } else {
+    LN_HIP(ctx, hipMemsetAsync(part, 0, len * sizeof(double), ctx->stream));
+    nblk = 1;
+  }
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, ctx->stream, part, nblk, len,
+                     gsum);
+  LN_LAUNCH(ctx);
+  LN_HIP(ctx, hipMemcpyAsync(G_host, gsum, len * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (symmetric)
+    for (int64_t i = 0; i < k1; ++i)
+      for (int64_t j = 0; j < i; ++j) G_host[i * k2 + j] = G_host[j * k2 + i];
+  return LIGHTNE_OK;
+}
+
+int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,
+                      int64_t k2, float* Out, int normalize_rows, float*) {
+  if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 16 * PS_MAX_T2 || ldx < k1 || ldx % 4)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: bad shape k1=%lld k2=%lld ldx=%lld",
+                    (long long)k1, (long long)k2, (long long)ldx);
+  if (Out == X && (k1 != k2 || ldx != k1))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: in-place needs a square W");
+  // column blocks of W small enough for LDS (<= 160 KB per block)
+  int64_t kb = k2;
+  while ((size_t)k1 * (kb + 4) * sizeof(float) > 150 * 1024) kb -= 16;
+  if (kb < k2 && normalize_rows)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: row normalisation needs k2 in one block");
+  std::vector<float> wf(k1 * k2);
+  for (int64_t i = 0; i < k1 * k2; ++i) wf[i] = (float)W_host[i];
+  auto* dW = (float*)ctx_buf(ctx, "small_w", wf.size() * sizeof(float));
+  if (!dW) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemcpyAsync(dW, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  if (n <= 0) return LIGHTNE_OK;
+  float* dst = Out;
+  if (kb < k2 && Out == X) {  // later column blocks must still read the old X
+    dst = (float*)ctx_buf(ctx, "ps_tmp", (size_t)n * k2 * sizeof(float));
+    if (!dst) return LIGHTNE_ERR_OOM;
+  }
+  const int64_t nslab = (n + 15) / 16;
+  const int64_t waves_per_block = PS_THREADS / 64;
+  const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((nslab + waves_per_block - 1) / waves_per_block,
+                                                              (int64_t)ctx->num_cus * 4));
+  for (int64_t j0 = 0; j0 < k2; j0 += kb) {
+    const int64_t kk = std::min(kb, k2 - j0);
+    const size_t lds = (size_t)k1 * (kk + 4) * sizeof(float);
+    const int T2 = (int)(kk / 16);
+#define PS_CASE(t)                                                                                               \
+  case t:                                                                                                        \
+    hipLaunchKernelGGL(k_panel_small<t>, dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X, ldx, n,     \
+                       (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows);        \
+    break;
+    switch (T2) {
+      PS_CASE(1) PS_CASE(2) PS_CASE(3) PS_CASE(4) PS_CASE(5) PS_CASE(6) PS_CASE(7) PS_CASE(8)
+      PS_CASE(9) PS_CASE(10) PS_CASE(11) PS_CASE(12) PS_CASE(13) PS_CASE(14) PS_CASE(15) PS_CASE(16)
+      default:
+        return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: k2 too large");
+    }
+#undef PS_CASE
+    LN_LAUNCH(ctx);
+  }
+  if (dst != Out)
+    LN_HIP(ctx, hipMemcpyAsync(Out, dst, (size_t)n * k2 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int f32_to_bf16(lightne_ctx* ctx, const float* in, __hip_bfloat16* out, int64_t count) {
+  if (count <= 0) return LIGHTNE_OK;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}
+
+int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
+                     int normalize, float* emb) {
+  std::vector<float> sq(dim);
+  for (int64_t j = 0; j < dim; ++j) sq[j] = S_host ? (float)std::sqrt(S_host[j]) : 1.0f;
+  auto* dsq = (float*)ctx_buf(ctx, "usr_sq", dim * sizeof(float));
+  if (!dsq) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemcpyAsync(dsq, sq.data(), dim * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  if (n <= 0) return LIGHTNE_OK;
+  const int64_t blocks = (n * 64 + 255) / 256;
+  hipLaunchKernelGGL(k_u_sigma_root, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, U, (const float*)dsq, n, rank,
+                     dim, normalize, emb);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}

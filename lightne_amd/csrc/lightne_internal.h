@@ -1,0 +1,145 @@
+// Internal declarations shared by the HIP translation units of liblightne_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "lightne.h"
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct lightne_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::unordered_map<std::string, Buf> bufs;  // grow-only named scratch
+  hipEvent_t ev[16] = {};
+  int num_cus = 256;
+};
+
+struct lightne_table {
+  uint64_t nkeys = 0;
+  uint64_t* keys = nullptr;  // device, sorted
+  uint32_t* cnt = nullptr;   // device
+  double* wgh = nullptr;     // device
+};
+
+struct SpmmPlan;
+
+struct lightne_csr {
+  int64_t n = 0, nnz = 0;
+  int64_t* row_ptr = nullptr;
+  int32_t* col = nullptr;
+  float* val = nullptr;
+  SpmmPlan* plan = nullptr;  // nnz-balanced SpMM work split, built on first use
+};
+
+int ctx_fail(lightne_ctx* ctx, int code, const char* fmt, ...);
+int global_fail(int code, const char* fmt, ...);
+void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes);  // nullptr on OOM
+void ctx_release(lightne_ctx* ctx, const char* name);
+
+#define LN_HIP(ctx, expr)                                                                        \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return ctx_fail((ctx), LIGHTNE_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #expr,         \
+                      hipGetErrorString(e_));                                                    \
+  } while (0)
+
+#define LN_LAUNCH(ctx)                                                                           \
+  do {                                                                                           \
+    hipError_t e_ = hipGetLastError();                                                           \
+    if (e_ != hipSuccess)                                                                        \
+      return ctx_fail((ctx), LIGHTNE_ERR_HIP, "%s:%d launch: %s", __FILE__, __LINE__,            \
+                      hipGetErrorString(e_));                                                    \
+  } while (0)
+
+#define LN_CHECK(expr)         \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != LIGHTNE_OK) return rc_; \
+  } while (0)
+
+// RAII device allocation owned by one call
+struct DevMem {
+  void* p = nullptr;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+  void* release() {
+    void* q = p;
+    p = nullptr;
+    return q;
+  }
+};
+int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
+
+// elapsed seconds between two events recorded on the context stream
+double ev_seconds(hipEvent_t a, hipEvent_t b);
+
+// ---------------------------------------------------------------------------
+// device primitives (prim.hip)
+// ---------------------------------------------------------------------------
+// exclusive prefix sum of u32 or u64 counts into u64 offsets; out has n+1
+// entries (out[n] = total).  Deterministic.
+int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n);
+int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n);
+// stable LSD radix sort of (key, val) pairs on the low `bits` bits of key.
+// keys/vals hold the result on return; tmp buffers come from the context.
+int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits);
+int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits);
+
+// ---------------------------------------------------------------------------
+// SpMM (spmm.hip)
+// ---------------------------------------------------------------------------
+enum PanelType { PANEL_F32 = 0, PANEL_BF16 = 1 };
+
+// ---------------------------------------------------------------------------
+// dense panels (dense.hip)
+// ---------------------------------------------------------------------------
+int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out);
+void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* out);
+// G(k1 x k2, f64, host) = X^T Y for X: n x k1, Y: n x k2 (row-major f32 device
+// panels); symmetric=1 when X == Y (only the upper tiles are computed).
+int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+             double* G_host);
+// Out(n x k2) = X(n x k1) W(k1 x k2) with W a host f64 matrix used as f32.
+// Optional row L2-normalisation epilogue (norm > eps).  Out may alias X only
+// when k2 == k1 (rows are staged through registers).
+int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,
+                      int64_t k2, float* Out, int normalize_rows, float* unused);
+// emb = sqrt(S) U[:, :dim] with optional row normalisation (device kernel)
+int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
+                     int normalize, float* emb);
+int f32_to_bf16(lightne_ctx* ctx, const float* in, __hip_bfloat16* out, int64_t count);
+
+// ---------------------------------------------------------------------------
+// host FP64 small linear algebra (linalg_host.cpp)
+// ---------------------------------------------------------------------------
+// upper Cholesky R^T R = G (k x k row-major, in place into R).  Returns false
+// if G is not numerically positive definite.
+bool chol_upper(const double* G, int k, double* R);
+// inverse of an upper-triangular matrix (row-major)
+void tri_upper_inverse(const double* R, int k, double* Rinv);
+// one-sided Jacobi SVD of A (m x n row-major, m >= n): A = U diag(S) V^T,
+// singular values descending; U m x n, V n x n (either may be NULL).
+void jacobi_svd(const double* A, int m, int n, double* U, double* S, double* V);
+// C = A (m x k) * B (k x n), row-major
+void matmul_host(const double* A, const double* B, int m, int k, int n, double* C);
+
+// orthonormalise the panel Y (n x k, f32, device) in place by CholeskyQR2 with
+// FP64 Gram and Cholesky; optionally returns R (k x k, host) with Y_in = Q R.
+int cholqr2(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_out_host, int* passes_out);

@@ -1,0 +1,240 @@
+// Device primitives: deterministic exclusive scan and a stable LSD radix sort
+// of (u64 key, 32-bit payload) pairs.  Written for 64-wide wavefronts: peer
+// detection inside a wave uses 8 ballots over the digit bits, cross-wave
+// ranking goes through LDS.
+#include "lightne_internal.h"
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread across a 256-thread block
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* s_wave, uint64_t& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint64_t inc = wave_incl_scan(v);
+  if (lane == 63) s_wave[wid] = inc;
+  __syncthreads();
+  uint64_t off = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+    uint64_t t = s_wave[w];
+    if (w < wid) off += t;
+    total += t;
+  }
+  __syncthreads();
+  return off + inc - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const T* __restrict__ in, int64_t n,
+                                                              uint64_t* __restrict__ sums) {
+  __shared__ uint64_t s_wave[SCAN_THREADS / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  uint64_t acc = 0;
+#pragma unroll
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    int64_t i = base + (int64_t)it * SCAN_THREADS + threadIdx.x;
+    if (i < n) acc += (uint64_t)in[i];
+  }
+  uint64_t total;
+  (void)block_excl_scan(acc, s_wave, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const T* __restrict__ in, int64_t n,
+                                                             uint64_t* __restrict__ out,
+                                                             const uint64_t* __restrict__ block_off) {
+  __shared__ uint64_t s_data[SCAN_TILE];
+  __shared__ uint64_t s_wave[SCAN_THREADS / 64];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    int idx = it * SCAN_THREADS + threadIdx.x;
+    int64_t i = base + idx;
+    s_data[idx] = (i < n) ? (uint64_t)in[i] : 0;
+  }
+  __syncthreads();
+  uint64_t loc[SCAN_ITEMS];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    loc[j] = sum;
+    sum += s_data[threadIdx.x * SCAN_ITEMS + j];
+  }
+  uint64_t total;
+  uint64_t off = block_excl_scan(sum, s_wave, total) + (block_off ? block_off[blockIdx.x] : 0);
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) s_data[threadIdx.x * SCAN_ITEMS + j] = loc[j] + off;
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    int idx = it * SCAN_THREADS + threadIdx.x;
+    int64_t i = base + idx;
+    if (i < n) out[i] = s_data[idx];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    uint64_t bo = block_off ? block_off[blockIdx.x] : 0;
+    out[n] = bo + total;
+  }
+}
+
+template <typename T>
+int scan_impl(lightne_ctx* ctx, const T* in, uint64_t* out, int64_t n, int depth) {
+  if (n <= 0) {
+    LN_HIP(ctx, hipMemsetAsync(out, 0, sizeof(uint64_t), ctx->stream));
+    return LIGHTNE_OK;
+  }
+  int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 1) {
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, in, n, out,
+                       (const uint64_t*)nullptr);
+    LN_LAUNCH(ctx);
+    return LIGHTNE_OK;
+  }
+  char name_s[32], name_o[32];
+  snprintf(name_s, sizeof(name_s), "scan_sums%d", depth);
+  snprintf(name_o, sizeof(name_o), "scan_offs%d", depth);
+  auto* sums = (uint64_t*)ctx_buf(ctx, name_s, nb * sizeof(uint64_t));
+  auto* offs = (uint64_t*)ctx_buf(ctx, name_o, (nb + 1) * sizeof(uint64_t));
+  if (!sums || !offs) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, sums);
+  LN_LAUNCH(ctx);
+  LN_CHECK(scan_impl<uint64_t>(ctx, sums, offs, nb, depth + 1));
+  hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, ctx->stream, in, n, out,
+                     (const uint64_t*)offs);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// radix sort
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                        uint32_t* __restrict__ hist, int64_t ntiles) {
+  __shared__ uint32_t s_h[256];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int64_t i = base + (int64_t)it * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+template <typename V>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                           const V* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                           V* __restrict__ vout, int64_t n, int shift,
+                                                           const uint64_t* __restrict__ offs, int64_t ntiles) {
+  __shared__ uint64_t s_base[256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_wcnt[RS_WAVES][256];
+  __shared__ uint32_t s_woff[RS_WAVES][256];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  s_base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
+  s_run[tid] = 0;
+#pragma unroll
+  for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int64_t i = base + (int64_t)it * RS_THREADS + tid;
+    const bool valid = i < n;
+    uint64_t k = valid ? kin[i] : 0;
+    V v = valid ? vin[i] : V(0);
+    const uint32_t d = (uint32_t)((k >> shift) & 255u);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+    if (valid && rank == 0) s_wcnt[wid][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t run = s_run[tid];
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) {
+        s_woff[w][tid] = run;
+        run += s_wcnt[w][tid];
+        s_wcnt[w][tid] = 0;
+      }
+      s_run[tid] = run;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint64_t pos = s_base[d] + s_woff[wid][d] + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+  }
+}
+
+template <typename V>
+int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits) {
+  if (n <= 1 || bits <= 0) return LIGHTNE_OK;
+  const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  auto* k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
+  auto* v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
+  auto* hist = (uint32_t*)ctx_buf(ctx, "rs_hist", 256 * ntiles * sizeof(uint32_t));
+  auto* offs = (uint64_t*)ctx_buf(ctx, "rs_offs", (256 * ntiles + 1) * sizeof(uint64_t));
+  if (!k2 || !v2 || !hist || !offs) return LIGHTNE_ERR_OOM;
+  uint64_t *ka = keys, *kb = k2;
+  V *va = vals, *vb = v2;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, n, shift, hist,
+                       ntiles);
+    LN_LAUNCH(ctx);
+    LN_CHECK(scan_u32_to_u64(ctx, hist, offs, 256 * ntiles));
+    hipLaunchKernelGGL(k_rs_scatter<V>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, va, kb, vb, n,
+                       shift, offs, ntiles);
+    LN_LAUNCH(ctx);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    LN_HIP(ctx, hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+    LN_HIP(ctx, hipMemcpyAsync(vals, va, n * sizeof(V), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return LIGHTNE_OK;
+}
+
+}  // namespace
+
+int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
+  return scan_impl<uint32_t>(ctx, in, out, n, 0);
+}
+int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n) {
+  return scan_impl<uint64_t>(ctx, in, out, n, 0);
+}
+int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits) {
+  return radix_sort_impl<float>(ctx, keys, vals, n, bits);
+}
+int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits) {
+  return radix_sort_impl<uint32_t>(ctx, keys, vals, n, bits);
+}

@@ -1,0 +1,260 @@
+// Stage 3: spectral propagation (LightNE/SpectralPropagation.h:45-229).
+//
+// M = (1 - mu) I - D~^-1 A~ over the INPUT graph with self loops (:51-75) is
+// never materialised: the SpMM epilogue forms (M x)_i from the neighbour sum.
+// The Chebyshev recurrence of order K (:94-208) runs as 2 gathers per step with
+// the AXPYs fused into the second gather's epilogue; y = a - conv is written by
+// the last step, z = (A + I) y by one more gather (:210-221), and the dense
+// tall SVD (:16-43) is CholeskyQR2 + a host Jacobi SVD of the R factor.
+#include "lightne_internal.h"
+#include "spmm_epi.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+// modified Bessel function of the first kind by its power series,
+// I_k(x) = sum_j (x/2)^(2j+k) / (j! (j+k)!)  (replaces boost::math::cyl_bessel_i,
+// SpectralPropagation.h:8,12)
+double bessel_i(int k, double x) {
+  const double h = 0.5 * x;
+  double term = 1.0;
+  for (int i = 1; i <= k; ++i) term *= h / i;  // (x/2)^k / k!
+  double sum = term;
+  for (int j = 1; j < 200; ++j) {
+    term *= (h * h) / ((double)j * (double)(j + k));
+    sum += term;
+    if (term < sum * 1e-18) break;
+  }
+  return sum;
+}
+
+struct GraphPlan {
+  SpmmPlan P;
+  ~GraphPlan() { spmm_plan_free(P); }
+};
+
+__global__ void k_y_from_conv(const float* __restrict__ a, const float* __restrict__ conv, float* __restrict__ y,
+                              int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) y[i] = a[i] + (-1.0f) * conv[i];
+}
+
+int check_prop(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim, const lightne_prop_params* p) {
+  if (!ctx || !g || !a || !p) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: NULL argument");
+  if (g->n <= 0 || !g->offsets || (g->m > 0 && !g->edges)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: bad graph");
+  if (dim <= 0 || dim % 16 || dim > 1024)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: dim must be a multiple of 16 in [16, 1024]");
+  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: unknown precision");
+  return LIGHTNE_OK;
+}
+
+}  // namespace
+
+// Chebyshev weights as the reference forms them: c_0 = I_0(s),
+// c_i = (i odd ? -2 : 2) I_i(s), each rounded to float at the AXPY (:132-197).
+void cheb_coeffs(int64_t order, double theta, std::vector<float>& c) {
+  const int64_t K = std::max<int64_t>(order, 2);
+  c.assign(K, 0.f);
+  c[0] = (float)bessel_i(0, theta);
+  c[1] = (float)(-2 * bessel_i(1, theta));
+  for (int64_t i = 2; i < K; ++i) c[i] = (float)(((i & 1) ? -2 : 2) * bessel_i((int)i, theta));
+}
+
+extern "C" double lightne_bessel_i(int k, double x) { return bessel_i(k, x); }
+
+// forward recurrence; optional iterate checkpoints for the adjoint
+int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                      const lightne_prop_params* p, float* y, float* z, lightne_prop_stats* st, SpmmPlan& P) {
+  const int64_t n = g->n, K = std::max<int64_t>(p->order, 2);
+  const bool bf = p->precision == LIGHTNE_BF16;
+  const int xt = bf ? PANEL_BF16 : PANEL_F32;
+  const size_t esz = bf ? 2 : 4;
+  const size_t pe = (size_t)n * dim;
+  void* B[3];
+  B[0] = ctx_buf(ctx, "pr_b0", pe * esz);
+  B[1] = ctx_buf(ctx, "pr_b1", pe * esz);
+  B[2] = ctx_buf(ctx, "pr_b2", pe * esz);
+  void* tmp = ctx_buf(ctx, "pr_tmp", pe * esz);
+  auto* conv = (float*)ctx_buf(ctx, "pr_conv", pe * sizeof(float));
+  if (!B[0] || !B[1] || !B[2] || !tmp || !conv) return LIGHTNE_ERR_OOM;
+  std::vector<float> c;
+  cheb_coeffs(p->order, (double)p->theta, c);
+  hipStream_t s = ctx->stream;
+  // Lx0 = a (as the gathered type)
+  const void* lx0;
+  if (bf) {
+    LN_CHECK(f32_to_bf16(ctx, a, (__hip_bfloat16*)B[0], (int64_t)pe));
+    lx0 = B[0];
+  } else {
+    lx0 = a;
+  }
+  SpmmArgs base;
+  base.n = n;
+  base.k = dim;
+  base.rp = g->offsets;
+  base.gcol = g->edges;
+  base.impl_mode = IMPL_PROP;
+  base.mu = p->mu;
+  base.a_in = a;
+  base.conv = conv;
+  int calls = 0;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[13], s));
+  // tmp = M a   (:101-111)
+  {
+    SpmmArgs t = base;
+    t.X = lx0;
+    t.out = tmp;
+    t.alpha = 1.f;
+    LN_CHECK(spmm_run(ctx, P, t, xt, EPI_STORE, true));
+    ++calls;
+  }
+  // Lx1 = 0.5 M tmp - a ; conv = c0 a + c1 Lx1   (:114-148)
+  void* lx1 = B[1];
+  {
+    SpmmArgs t = base;
+    t.X = tmp;
+    t.alpha = 0.5f;
+    t.c0 = c[0];
+    t.c1 = c[1];
+    t.lx_out = lx1;
+    LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB1, true));
+    ++calls;
+  }
+  float* yout = y;
+  if (K <= 2) {
+    hipLaunchKernelGGL(k_y_from_conv, dim3((unsigned)((pe + 255) / 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
+    LN_LAUNCH(ctx);
+  }
+  // free buffer for Lx2 (never the caller's a)
+  void* lx2 = bf ? B[2] : B[0];
+  for (int64_t i = 2; i < K; ++i) {
+    {
+      SpmmArgs t = base;  // tmp = M Lx1  (:153-163)
+      t.X = lx1;
+      t.out = tmp;
+      t.alpha = 1.f;
+      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_STORE, true));
+      ++calls;
+    }
+    SpmmArgs t = base;  // Lx2 = M tmp - Lx0 - 2 Lx1 ; conv += c_i Lx2  (:166-197)
+    t.X = tmp;
+    t.alpha = 1.f;
+    t.c1 = c[i];
+    t.lx1 = lx1;
+    t.lx0 = lx0;
+    if (i + 1 < K) {
+      t.lx_out = lx2;
+      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB, true));
+    } else {
+      t.y_out = yout;
+      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB_LAST, true));
+    }
+    ++calls;
+    // rotate: Lx0 <- Lx1, Lx1 <- Lx2, next Lx2 <- a free buffer (not a)
+    void* old0 = const_cast<void*>(lx0);
+    lx0 = lx1;
+    lx1 = lx2;
+    lx2 = (old0 == (const void*)a) ? B[2] : old0;
+  }
+  LN_HIP(ctx, hipEventRecord(ctx->ev[14], s));
+  if (z) {
+    SpmmArgs t = base;  // z = (A + I) y  (:210-221)
+    t.impl_mode = IMPL_AGG;
+    t.X = yout;
+    t.out = z;
+    t.alpha = 1.f;
+    LN_CHECK(spmm_run(ctx, P, t, PANEL_F32, EPI_STORE_F32, true));
+    ++calls;
+  }
+  LN_HIP(ctx, hipEventRecord(ctx->ev[15], s));
+  LN_HIP(ctx, hipEventSynchronize(ctx->ev[15]));
+  if (st) {
+    st->t_cheb = ev_seconds(ctx->ev[13], ctx->ev[14]);
+    st->t_aggregate = ev_seconds(ctx->ev[14], ctx->ev[15]);
+    st->spmm_calls = calls;
+  }
+  return LIGHTNE_OK;
+}
+
+int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
+                    float* S_host) {
+  if (rank % 16 || dim % 16 || dim > rank || rank > 256 || n < rank)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embed_dense: rank, dim must be multiples of 16, dim <= rank <= 256, n >= rank");
+  auto* Q = (float*)ctx_buf(ctx, "ed_q", (size_t)n * rank * sizeof(float));
+  if (!Q) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemcpyAsync(Q, mat, (size_t)n * rank * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  std::vector<double> R(rank * rank), Ur(rank * rank), Sr(rank);
+  int passes = 0;
+  LN_CHECK(cholqr2(ctx, Q, n, rank, R.data(), &passes));
+  jacobi_svd(R.data(), (int)rank, (int)rank, Ur.data(), Sr.data(), nullptr);
+  // emb = rownorm(Q U_R[:, :dim] diag(sqrt(S)))  (compute_u_sigma_root, normalize = true)
+  std::vector<double> W(rank * dim);
+  for (int64_t i = 0; i < rank; ++i)
+    for (int64_t j = 0; j < dim; ++j) {
+      const float sq = std::sqrt((float)Sr[j]);
+      W[i * dim + j] = (double)((float)Ur[i * rank + j] * sq);
+    }
+  LN_CHECK(panel_times_small(ctx, Q, n, rank, rank, W.data(), dim, emb_out, 1, nullptr));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (S_host)
+    for (int64_t j = 0; j < rank; ++j) S_host[j] = (float)Sr[j];
+  return LIGHTNE_OK;
+}
+
+extern "C" {
+
+int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                      const lightne_prop_params* p, float* y_out, float* z_out, lightne_prop_stats* stats) {
+  LN_CHECK(check_prop(ctx, g, a, dim, p));
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  GraphPlan gp;
+  LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
+  float* y = y_out;
+  if (!y) {
+    y = (float*)ctx_buf(ctx, "pr_y", (size_t)g->n * dim * sizeof(float));
+    if (!y) return LIGHTNE_ERR_OOM;
+  }
+  lightne_prop_stats st{};
+  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z_out, &st, gp.P));
+  auto t1 = std::chrono::steady_clock::now();
+  st.t_total = std::chrono::duration<double>(t1 - t0).count();
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+int lightne_embed_dense(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
+                        float* S_host) {
+  if (!ctx || !mat || !emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_embed_dense: NULL argument");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  return embed_dense_dev(ctx, mat, n, rank, dim, emb_out, S_host);
+}
+
+int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
+                                const lightne_prop_params* p, float* emb_out, lightne_prop_stats* stats) {
+  LN_CHECK(check_prop(ctx, g, a, dim, p));
+  if (!emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "chebyshev_expansion: NULL output");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  GraphPlan gp;
+  LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
+  auto* y = (float*)ctx_buf(ctx, "pr_y", (size_t)g->n * dim * sizeof(float));
+  auto* z = (float*)ctx_buf(ctx, "pr_z", (size_t)g->n * dim * sizeof(float));
+  if (!y || !z) return LIGHTNE_ERR_OOM;
+  lightne_prop_stats st{};
+  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z, &st, gp.P));
+  auto t1 = std::chrono::steady_clock::now();
+  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr));
+  auto t2 = std::chrono::steady_clock::now();
+  st.t_svd = std::chrono::duration<double>(t2 - t1).count();
+  st.t_total = std::chrono::duration<double>(t2 - t0).count();
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+// Stage 2: randomized SVD of the sparsifier (LightNE/mklredsvd/mklredsvd.cc:93-403)
+// and the U sqrt(Sigma) embedding (LightNE/PathEmbed.h:371-408), plus the CSR
+// handle API and the NetSMF driver (PathEmbed.h:494-563).
+//
+// Schedule (q = power_iters; q = 1 is the reference schedule):
+//   Omega ~ N(0,1) n x k            (mklredsvd.cc:229-231, own counter-based RNG)
+//   Y = A Omega                      (:233-250)
+//   Q = orth(Y)                      (:262)   CholeskyQR2, FP64 Gram + Cholesky
+//   repeat q-1 times: Q = orth(A Q)
+//   B = A Q                          (:268-287)
+//   P ~ N(0,1) k x k; Z = B P        (:289-317)
+//   Z = orth(Z)                      (:321)
+//   C = Z^T B                        (:327-350)   FP64 Gram kernel
+//   C = U_C S V^T                    (:352-377)   host FP64 Jacobi
+//   matU = Z U_C                     (:379-400)
+// In the BF16 form the gathered SpMM panels (Omega, Q) are BF16 with FP32
+// accumulation; every other panel stays FP32 and the Grams FP64.
+#include "lightne_internal.h"
+#include "spmm_epi.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+int csr_plan(lightne_ctx* ctx, lightne_csr* A) {
+  if (A->plan) return LIGHTNE_OK;
+  auto* P = new SpmmPlan();
+  int rc = spmm_plan_build(ctx, A->row_ptr, A->n, *P);
+  if (rc != LIGHTNE_OK) {
+    spmm_plan_free(*P);
+    delete P;
+    return rc;
+  }
+  A->plan = P;
+  return LIGHTNE_OK;
+}
+
+// Y(f32) = A X, X in FP32 or BF16
+int csr_mm(lightne_ctx* ctx, lightne_csr* A, const void* X, int xt, int64_t k, float* Y) {
+  LN_CHECK(csr_plan(ctx, A));
+  SpmmArgs a;
+  a.n = A->n;
+  a.k = k;
+  a.rp = A->row_ptr;
+  a.col = A->col;
+  a.val = A->val;
+  a.impl_mode = IMPL_NONE;
+  a.X = X;
+  a.out = Y;
+  a.alpha = 1.f;
+  return spmm_run(ctx, *A->plan, a, xt, EPI_STORE_F32, false);
+}
+
+}  // namespace
+
+int cholqr2(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_out_host, int* passes_out) {
+  const int kk = (int)k;
+  std::vector<double> G(k * k), Gs(k * k), Rs(k * k), R(k * k), Rinv(k * k), Rtot(k * k, 0.0), tmp(k * k);
+  for (int i = 0; i < kk; ++i) Rtot[i * k + i] = 1.0;
+  int passes = 0;
+  for (int pass = 0; pass < 4; ++pass) {
+    LN_CHECK(gram_f64(ctx, Y, Y, n, k, k, 1, G.data()));
+    // deviation from orthonormality of the current panel
+    double dev = 0.0;
+    for (int i = 0; i < kk; ++i)
+      for (int j = 0; j < kk; ++j) dev = std::max(dev, std::fabs(G[i * k + j] - (i == j ? 1.0 : 0.0)));
+    if (pass >= 2 && dev < 1e-5) break;
+    // diagonal equilibration: Gs = D G D, D = diag(G)^-1/2
+    std::vector<double> d(k);
+    for (int i = 0; i < kk; ++i) d[i] = G[i * k + i] > 0.0 ? 1.0 / std::sqrt(G[i * k + i]) : 1.0;
+    for (int i = 0; i < kk; ++i)
+      for (int j = 0; j < kk; ++j) Gs[i * k + j] = d[i] * G[i * k + j] * d[j];
+    if (!chol_upper(Gs.data(), kk, Rs.data())) {
+      // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020)
+      double fro = 0.0;
+      for (double v : Gs) fro += v * v;
+      fro = std::sqrt(fro);
+      const double u = std::ldexp(1.0, -53);
+      double shift = 11.0 * ((double)n * k + (double)k * (k + 1)) * u * fro;
+      bool ok = false;
+      for (int attempt = 0; attempt < 8 && !ok; ++attempt, shift *= 100.0) {
+        std::vector<double> Gt(Gs);
+        for (int i = 0; i < kk; ++i) Gt[i * k + i] += shift;
+        ok = chol_upper(Gt.data(), kk, Rs.data());
+      }
+      if (!ok) return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "CholeskyQR: Gram matrix is not positive definite");
+    }
+    // R = Rs D^-1
+    for (int i = 0; i < kk; ++i)
+      for (int j = 0; j < kk; ++j) R[i * k + j] = Rs[i * k + j] / d[j];
+    tri_upper_inverse(R.data(), kk, Rinv.data());
+    LN_CHECK(panel_times_small(ctx, Y, n, k, k, Rinv.data(), k, Y, 0, nullptr));
+    matmul_host(R.data(), Rtot.data(), kk, kk, kk, tmp.data());
+    Rtot.swap(tmp);
+    ++passes;
+    if (pass >= 1 && dev < 1e-1) break;  // input was already close to orthonormal
+  }
+  if (R_out_host) std::copy(Rtot.begin(), Rtot.end(), R_out_host);
+  if (passes_out) *passes_out = passes;
+  return LIGHTNE_OK;
+}
+
+extern "C" {
+
+int lightne_csr_create(lightne_ctx* ctx, int64_t n, int64_t nnz, const int64_t* row_ptr, const int32_t* col,
+                       const float* val, lightne_csr** out) {
+  if (!ctx || !out || n <= 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!col || !val)))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_csr_create: bad argument");
+  *out = nullptr;
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  DevMem a, b, c;
+  LN_CHECK(dev_alloc(ctx, a, (n + 1) * 8, "csr row_ptr"));
+  LN_CHECK(dev_alloc(ctx, b, std::max<int64_t>(nnz, 1) * 4, "csr col"));
+  LN_CHECK(dev_alloc(ctx, c, std::max<int64_t>(nnz, 1) * 4, "csr val"));
+  LN_HIP(ctx, hipMemcpyAsync(a.p, row_ptr, (n + 1) * 8, hipMemcpyDefault, ctx->stream));
+  if (nnz) {
+    LN_HIP(ctx, hipMemcpyAsync(b.p, col, nnz * 4, hipMemcpyDefault, ctx->stream));
+    LN_HIP(ctx, hipMemcpyAsync(c.p, val, nnz * 4, hipMemcpyDefault, ctx->stream));
+  }
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  auto* h = new lightne_csr();
+  h->n = n;
+  h->nnz = nnz;
+  h->row_ptr = (int64_t*)a.release();
+  h->col = (int32_t*)b.release();
+  h->val = (float*)c.release();
+  *out = h;
+  return LIGHTNE_OK;
+}
+
+int lightne_csr_info(const lightne_csr* c, int64_t* n, int64_t* nnz) {
+  if (!c) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_csr_info: NULL handle");
+  if (n) *n = c->n;
+  if (nnz) *nnz = c->nnz;
+  return LIGHTNE_OK;
+}
+
+int lightne_csr_device_ptrs(const lightne_csr* c, const int64_t** row_ptr, const int32_t** col, const float** val) {
+  if (!c) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_csr_device_ptrs: NULL handle");
+  if (row_ptr) *row_ptr = c->row_ptr;
+  if (col) *col = c->col;
+  if (val) *val = c->val;
+  return LIGHTNE_OK;
+}
+
+int lightne_csr_copy(lightne_ctx* ctx, const lightne_csr* c, int64_t* row_ptr, int32_t* col, float* val) {
+  if (!ctx || !c) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_csr_copy: NULL argument");
+  if (row_ptr) LN_HIP(ctx, hipMemcpyAsync(row_ptr, c->row_ptr, (c->n + 1) * 8, hipMemcpyDefault, ctx->stream));
+  if (c->nnz) {
+    if (col) LN_HIP(ctx, hipMemcpyAsync(col, c->col, c->nnz * 4, hipMemcpyDefault, ctx->stream));
+    if (val) LN_HIP(ctx, hipMemcpyAsync(val, c->val, c->nnz * 4, hipMemcpyDefault, ctx->stream));
+  }
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int lightne_csr_free(lightne_csr* c) {
+  if (!c) return LIGHTNE_OK;
+  if (c->plan) {
+    spmm_plan_free(*c->plan);
+    delete c->plan;
+  }
+  if (c->row_ptr) (void)hipFree(c->row_ptr);
+  if (c->col) (void)hipFree(c->col);
+  if (c->val) (void)hipFree(c->val);
+  delete c;
+  return LIGHTNE_OK;
+}
+
+int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_params* p, float* U_out, float* S_host,
+                 lightne_rsvd_stats* stats) {
+  if (!ctx || !Ac || !p || !U_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: NULL argument");
+  auto* A = const_cast<lightne_csr*>(Ac);
+  const int64_t n = A->n, k = p->rank;
+  if (k <= 0 || k % 16 || k > 256)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank must be a multiple of 16 in [16, 256] (got %lld)",
+                    (long long)k);
+  if (k > n) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank exceeds n");
+  if (p->upper) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: upper-triangle storage is not supported");
+  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
+  const int q = std::max(1, (int)p->power_iters);
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  lightne_rsvd_stats st{};
+  hipStream_t s = ctx->stream;
+  const bool bf = p->precision == LIGHTNE_BF16;
+  const size_t pf = (size_t)n * k * sizeof(float);
+  auto* P0 = (float*)ctx_buf(ctx, "rs_p0", pf);  // Omega, later Z
+  auto* P1 = (float*)ctx_buf(ctx, "rs_p1", pf);  // Y / Q
+  auto* P2 = (float*)ctx_buf(ctx, "rs_p2", pf);  // B
+  __hip_bfloat16* PB = nullptr;
+  if (bf) {
+    PB = (__hip_bfloat16*)ctx_buf(ctx, "rs_pb", (size_t)n * k * 2);
+    if (!PB) return LIGHTNE_ERR_OOM;
+  }
+  if (!P0 || !P1 || !P2) return LIGHTNE_ERR_OOM;
+  hipEvent_t* e = ctx->ev;
+  double t_spmm = 0, t_qr = 0, t_gemm = 0;
+  auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
+
+  LN_HIP(ctx, hipEventRecord(e[0], s));
+  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P0));
+  if (bf) LN_CHECK(f32_to_bf16(ctx, P0, PB, n * k));
+  LN_HIP(ctx, hipEventRecord(e[1], s));
+  LN_CHECK(csr_plan(ctx, A));
+  LN_HIP(ctx, hipEventRecord(e[2], s));
+  LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P0, bf ? PANEL_BF16 : PANEL_F32, k, P1));
+  LN_HIP(ctx, hipEventRecord(e[3], s));
+  LN_HIP(ctx, hipEventSynchronize(e[3]));
+  st.t_sketch = lap(e[0], e[1]);
+  t_spmm += lap(e[2], e[3]);
+  st.spmm_calls = 1;
+  if (p->random_project_only) {
+    LN_HIP(ctx, hipMemcpyAsync(U_out, P1, pf, hipMemcpyDeviceToDevice, s));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+    st.t_spmm = t_spmm;
+    st.t_total = lap(e[0], e[3]);
+    if (stats) *stats = st;
+    return LIGHTNE_OK;
+  }
+  int passes = 0;
+  for (int it = 0; it < q; ++it) {
+    LN_HIP(ctx, hipEventRecord(e[4], s));
+    LN_CHECK(cholqr2(ctx, P1, n, k, nullptr, &passes));
+    st.qr_calls++;
+    LN_HIP(ctx, hipEventRecord(e[5], s));
+    if (bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
+    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P1, bf ? PANEL_BF16 : PANEL_F32, k, P2));
+    st.spmm_calls++;
+    LN_HIP(ctx, hipEventRecord(e[6], s));
+    LN_HIP(ctx, hipEventSynchronize(e[6]));
+    t_qr += lap(e[4], e[5]);
+    t_spmm += lap(e[5], e[6]);
+    if (it + 1 < q) std::swap(P1, P2);  // next pass orthonormalises A Q
+  }
+  // B = P2.  P ~ N(0,1) k x k from stream 1 (mklredsvd.cc:290-291)
+  std::vector<float> Pf(k * k);
+  gaussian_fill_host(p->seed, 1u, k * k, Pf.data());
+  std::vector<double> Pd(Pf.begin(), Pf.end());
+  LN_HIP(ctx, hipEventRecord(e[7], s));
+  LN_CHECK(panel_times_small(ctx, P2, n, k, k, Pd.data(), k, P0, 0, nullptr));  // Z = B P
+  LN_HIP(ctx, hipEventRecord(e[8], s));
+  LN_CHECK(cholqr2(ctx, P0, n, k, nullptr, &passes));
+  st.qr_calls++;
+  LN_HIP(ctx, hipEventRecord(e[9], s));
+  std::vector<double> C(k * k), Uc(k * k), Sc(k);
+  LN_CHECK(gram_f64(ctx, P0, P2, n, k, k, 0, C.data()));  // C = Z^T B
+  LN_HIP(ctx, hipEventRecord(e[10], s));
+  LN_HIP(ctx, hipEventSynchronize(e[10]));
+  auto t0 = std::chrono::steady_clock::now();
+  jacobi_svd(C.data(), (int)k, (int)k, Uc.data(), Sc.data(), nullptr);
+  auto t1 = std::chrono::steady_clock::now();
+  st.t_small_svd = std::chrono::duration<double>(t1 - t0).count();
+  LN_HIP(ctx, hipEventRecord(e[11], s));
+  LN_CHECK(panel_times_small(ctx, P0, n, k, k, Uc.data(), k, U_out, 0, nullptr));  // matU = Z U_C
+  LN_HIP(ctx, hipEventRecord(e[12], s));
+  LN_HIP(ctx, hipEventSynchronize(e[12]));
+  t_gemm += lap(e[7], e[8]) + lap(e[9], e[10]) + lap(e[11], e[12]);
+  t_qr += lap(e[8], e[9]);
+  if (S_host)
+    for (int64_t j = 0; j < k; ++j) S_host[j] = (float)Sc[j];
+  st.t_spmm = t_spmm;
+  st.t_qr = t_qr;
+  st.t_gemm = t_gemm;
+  st.t_total = lap(e[0], e[12]);
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+int lightne_u_sigma_root(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
+                         int32_t normalize, float* emb_out) {
+  if (!ctx || !U || !emb_out || n <= 0 || dim <= 0 || dim > rank)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_u_sigma_root: bad argument");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  LN_CHECK(u_sigma_root_dev(ctx, U, S_host, n, rank, dim, normalize, emb_out));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* sp,
+                   const lightne_rsvd_params* rp, int64_t dim, int32_t normalize, float* emb_out, float* S_host,
+                   lightne_netsmf_stats* stats) {
+  if (!ctx || !g || !sp || !rp || !emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: NULL argument");
+  if (dim <= 0 || dim > rp->rank) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: dim must be in [1, rank]");
+  lightne_netsmf_stats st{};
+  auto t0 = std::chrono::steady_clock::now();
+  lightne_csr* A = nullptr;
+  LN_CHECK(lightne_sparsify(ctx, g, sp, &A, &st.sparsify));
+  auto t1 = std::chrono::steady_clock::now();
+  const int64_t n = g->n, k = rp->rank;
+  auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
+  if (!U) {
+    lightne_csr_free(A);
+    return LIGHTNE_ERR_OOM;
+  }
+  std::vector<float> S(k);
+  int rc = lightne_rsvd(ctx, A, rp, U, S.data(), &st.rsvd);
+  lightne_csr_free(A);
+  if (rc != LIGHTNE_OK) return rc;
+  if (rp->random_project_only) {
+    LN_CHECK(u_sigma_root_dev(ctx, U, nullptr, n, k, dim, normalize, emb_out));
+  } else {
+    LN_CHECK(u_sigma_root_dev(ctx, U, S.data(), n, k, dim, normalize, emb_out));
+  }
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  auto t2 = std::chrono::steady_clock::now();
+  if (S_host) std::copy(S.begin(), S.end(), S_host);
+  st.t_sparsify = std::chrono::duration<double>(t1 - t0).count();
+  st.t_svd = std::chrono::duration<double>(t2 - t1).count();
+  st.t_total = std::chrono::duration<double>(t2 - t0).count();
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+}  // extern "C"

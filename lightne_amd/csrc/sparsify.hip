@@ -1,0 +1,624 @@
+// Stage 1: NetSMF path-sampling sparsifier (LightNE/PathEmbed.h:81-316) on the
+// device.
+//
+//  k_edge_src      : source vertex of every directed edge (binary search in
+//                    the offsets), so the sampler can run one lane per edge.
+//  k_sample<MODE>  : one lane per directed edge (u, v); its pbbs::random chain
+//                    is seeded by fork((u << 32) | v) and drives exp_rv path
+//                    samples exactly as PathEmbed.h:128-171; every sample is
+//                    aggregated in an open-addressing table (par_table.h:86-109)
+//                    of 16-byte slots {key, count}, slot = hash64_2(key) & mask,
+//                    linear probing, one 64-bit CAS to claim a slot.
+//  k_table_reduce  : distinct keys and total weight (PathEmbed.h:175-188),
+//                    deterministic two-level reduction.
+//  k_trunc_emit    : trunc-log (PathEmbed.h:199-218) fused with the symmetric
+//                    expansion (PathEmbed.h:278-311): kept keys emit (row,col)
+//                    sort keys and float values.
+//  radix sort + k_csr_finish : rows sorted by column, row_ptr by boundary
+//                    detection (no atomics, deterministic).
+#include "lightne_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+struct __align__(16) Slot {
+  unsigned long long key;
+  unsigned int cnt;
+  unsigned int pad;
+};
+
+__host__ __device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_fill_slots(Slot* __restrict__ tab, uint64_t cap) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < cap; i += stride) {
+    Slot s;
+    s.key = EMPTY_KEY;
+    s.cnt = 0;
+    s.pad = 0;
+    tab[i] = s;
+  }
+}
+
+__global__ void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m, uint32_t* __restrict__ src) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  // largest u with off[u] <= j
+  int64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  src[j] = (uint32_t)lo;
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint64_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+struct SampleArgs {
+  const int64_t* off;
+  const uint32_t* adj;
+  const uint32_t* src;
+  int64_t m;
+  uint64_t seed;
+  uint64_t walk_len;
+  uint64_t exp_rv;
+  double frac;
+  double logn;
+  Slot* tab;
+  double* wtab;
+  uint64_t mask;
+  unsigned long long* stats;  // 0 samples, 1 steps, 2 attempts, 3 table overflow, 4 bad graph
+};
+
+// random_walk (PathEmbed.h:37-49): `len` steps from x, state advances one
+// draw per step.  rand = H(s); next = H(H(s)) = H(rand).
+__device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                         uint32_t x, uint64_t len, uint64_t& s, bool& bad) {
+  for (uint64_t t = 0; t < len; ++t) {
+    const int64_t o = off[x];
+    const uint64_t d = (uint64_t)(off[x + 1] - o);
+    if (d == 0) {
+      bad = true;
+      break;
+    }
+    const uint64_t r = hash64_2(s);
+    x = adj[o + (int64_t)(r % d)];
+    s = hash64_2(r);
+  }
+  return x;
+}
+
+template <int MODE>
+__device__ __forceinline__ bool table_add(const SampleArgs& a, uint64_t key, double w) {
+  uint64_t h = hash64_2(key) & a.mask;
+  for (uint64_t p = 0; p <= a.mask; ++p) {
+    unsigned long long cur = __hip_atomic_load(&a.tab[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == EMPTY_KEY) {
+      cur = atomicCAS(&a.tab[h].key, EMPTY_KEY, (unsigned long long)key);
+      if (cur == EMPTY_KEY) cur = key;
+    }
+    if (cur == key) {
+      atomicAdd(&a.tab[h].cnt, 1u);
+      if (MODE == 1) atomicAdd(&a.wtab[h], w);
+      return true;
+    }
+    h = (h + 1) & a.mask;
+  }
+  return false;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t n_samples = 0, n_steps = 0, n_attempts = 0;
+  bool overflow = false, bad = false;
+  if (j < a.m) {
+    const uint32_t u = a.src[j];
+    const uint32_t v = a.adj[j];
+    const uint64_t uv = ((uint64_t)u << 32) | (uint64_t)v;
+    uint64_t s = hash64_2(hash64_2(uv + a.seed));  // seed.fork(uv)
+    uint64_t e = a.exp_rv;
+    if (a.frac > 0.0000000001) {
+      const uint64_t r = hash64_2(s);
+      e += ((double)r * 0x1p-64 < a.frac) ? 1 : 0;  // map_uint64_t(rand) < frac
+      s = hash64_2(r);
+    }
+    n_attempts = e;
+    double p = 1.0, ipw = 1.0;
+    if (MODE == 1) {
+      const double du = (double)(a.off[u + 1] - a.off[u]);
+      const double dv = (double)(a.off[v + 1] - a.off[v]);
+      p = (1.0 / du + 1.0 / dv) * a.logn;
+      p = p < 1.0 ? p : 1.0;
+      ipw = 1.0 / p;
+    }
+    for (uint64_t k = 0; k < e; ++k) {
+      if (MODE == 1) {
+        const uint64_t q = hash64_2(s);
+        s = hash64_2(q);
+        if (!((double)q * 0x1p-64 < p)) continue;
+      }
+      uint64_t rr = hash64_2(s);
+      const uint64_t r = (uint32_t)(rr % a.walk_len + 1);
+      s = hash64_2(rr);
+      rr = hash64_2(s);
+      const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
+      s = hash64_2(rr);
+      const uint32_t x = walk(a.off, a.adj, u, (uint64_t)kk - 1, s, bad);
+      const uint32_t z = walk(a.off, a.adj, v, r - kk, s, bad);
+      n_steps += r - 1;
+      const uint64_t key = ((uint64_t)min(x, z) << 32) + (uint64_t)max(x, z);
+      if (!table_add<MODE>(a, key, ipw)) overflow = true;
+      ++n_samples;
+    }
+  }
+  wave_add_u64(&a.stats[0], n_samples);
+  wave_add_u64(&a.stats[1], n_steps);
+  wave_add_u64(&a.stats[2], n_attempts);
+  if (overflow) atomicOr(&a.stats[3], 1ull);
+  if (bad) atomicOr(&a.stats[4], 1ull);
+}
+
+// per-block partial (distinct, count sum, weight sum); fixed reduction order
+__global__ __launch_bounds__(256) void k_table_reduce(const Slot* __restrict__ tab, const double* __restrict__ wtab,
+                                                      uint64_t cap, unsigned long long* __restrict__ part_u,
+                                                      double* __restrict__ part_w) {
+  __shared__ unsigned long long s_d[256], s_c[256];
+  __shared__ double s_w[256];
+  uint64_t d = 0, c = 0;
+  double w = 0.0;
+  const uint64_t per = (cap + gridDim.x - 1) / gridDim.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = min(cap, b0 + per);
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    Slot s = tab[i];
+    if (s.key != EMPTY_KEY) {
+      ++d;
+      c += s.cnt;
+      if (wtab) w += wtab[i];
+    }
+  }
+  s_d[threadIdx.x] = d;
+  s_c[threadIdx.x] = c;
+  s_w[threadIdx.x] = w;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      s_d[threadIdx.x] += s_d[threadIdx.x + st];
+      s_c[threadIdx.x] += s_c[threadIdx.x + st];
+      s_w[threadIdx.x] += s_w[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_u[2 * blockIdx.x] = s_d[0];
+    part_u[2 * blockIdx.x + 1] = s_c[0];
+    part_w[blockIdx.x] = s_w[0];
+  }
+}
+
+// compact the table into (key, count[, weight]) arrays for the counts API
+__global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ tab, const double* __restrict__ wtab,
+                                                       uint64_t cap, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ idx,
+                                                       unsigned long long* __restrict__ counter) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool has = i < cap && tab[i].key != EMPTY_KEY;
+  const uint64_t bal = __ballot(has);
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0 && bal) base = atomicAdd(counter, (unsigned long long)__popcll(bal));
+  base = __shfl(base, 0, 64);
+  if (has) {
+    const uint64_t pos = base + __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    keys[pos] = tab[i].key;
+    idx[pos] = (uint32_t)i;  // slot index (cap <= 2^32 enforced by the host)
+  }
+}
+
+__global__ void k_gather_counts(const Slot* __restrict__ tab, const double* __restrict__ wtab,
+                                const uint32_t* __restrict__ idx, int64_t nk, uint32_t* __restrict__ cnt,
+                                double* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  const uint32_t s = idx[i];
+  cnt[i] = tab[s].cnt;
+  if (w) w[i] = wtab ? wtab[s] : (double)tab[s].cnt;
+}
+
+struct TruncArgs {
+  const Slot* tab;
+  const double* wtab;
+  uint64_t cap;
+  const int64_t* off;
+  double factor;
+  int upper;
+  int colbits;
+  uint64_t* skey;
+  float* sval;
+  unsigned long long* counter;  // emitted entries
+  unsigned long long* kept;     // kept upper keys
+};
+
+// PathEmbed.h:203-217 per key, then the CSR expansion of :278-311.
+__global__ __launch_bounds__(256) void k_trunc_emit(TruncArgs a) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int emit = 0;
+  uint32_t u = 0, v = 0;
+  float lw = 0.f;
+  if (i < a.cap) {
+    const Slot s = a.tab[i];
+    if (s.key != EMPTY_KEY) {
+      u = (uint32_t)(s.key >> 32);
+      v = (uint32_t)s.key;
+      const double du = (double)(a.off[u + 1] - a.off[u]);
+      const double dv = (double)(a.off[v + 1] - a.off[v]);
+      const double val = a.wtab ? a.wtab[i] : (double)s.cnt;
+      const double wgh = (u != v) ? val : val * 2;
+      const double adj = wgh * a.factor / du / dv;
+      // log(adj) > 0  <=>  adj > 1 for any faithful log
+      if (adj > 1.0) {
+        lw = (float)log(adj);
+        emit = (a.upper || u == v) ? 1 : 2;
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63;
+  // wave-aggregated allocation
+  int incl = emit;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  const int tot = __shfl(incl, 63, 64);
+  const uint64_t kbal = __ballot(emit > 0);
+  unsigned long long base = 0;
+  if (lane == 63 && tot) base = atomicAdd(a.counter, (unsigned long long)tot);
+  if (lane == 63 && kbal) atomicAdd(a.kept, (unsigned long long)__popcll(kbal));
+  base = __shfl(base, 63, 64);
+  if (emit) {
+    uint64_t pos = base + (uint64_t)(incl - emit);
+    a.skey[pos] = ((uint64_t)u << a.colbits) | v;
+    a.sval[pos] = lw;
+    if (emit == 2) {
+      a.skey[pos + 1] = ((uint64_t)v << a.colbits) | u;
+      a.sval[pos + 1] = lw;
+    }
+  }
+}
+
+// row_ptr by boundary detection over the sorted keys; col extraction
+__global__ void k_csr_finish(const uint64_t* __restrict__ skey, int64_t nnz, int64_t n, int colbits,
+                             int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > nnz) return;
+  const uint64_t cmask = (colbits >= 64) ? ~0ull : ((1ull << colbits) - 1);
+  const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
+  const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
+  for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = i;
+  if (i < nnz) col[i] = (int32_t)(skey[i] & cmask);
+}
+
+int bits_for(uint64_t n) {
+  int b = 0;
+  while (b < 63 && ((uint64_t)1 << b) < n) ++b;
+  return b;
+}
+
+uint64_t next_pow2(uint64_t x) {
+  uint64_t c = 1;
+  while (c < x) c <<= 1;
+  return c;
+}
+
+int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
+  if (!g || g->n <= 0 || g->m < 0 || !g->offsets || (g->m > 0 && !g->edges))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "invalid graph descriptor");
+  if (g->n >= (int64_t)1 << 32) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 vertices");
+  return LIGHTNE_OK;
+}
+
+// Sampler + table.  On success the table (slots, optional weights) lives in
+// the context buffers "tab" / "wtab"; stats are filled.
+int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                lightne_sparsify_stats* st, Slot** tab_out, double** wtab_out, uint64_t* cap_out) {
+  LN_CHECK(check_graph(ctx, g));
+  if (!p || p->walk_len == 0 || p->negative == 0)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "walk_len and negative must be >= 1");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  *st = lightne_sparsify_stats{};
+  LN_CHECK(lightne_sample_scalars(g->n, g->m, p, st));
+  const uint64_t per_edge = st->exp_rv_rounded + (st->fractional_bits > 0.0000000001 ? 1 : 0);
+  const uint64_t max_samples = (uint64_t)g->m * per_edge;
+  // capacity: load factor <= 1/2 for the worst case U = samples, never above
+  // the reference's own 2^ceil(log2(2 table_size)) unless that is smaller
+  // than what the samples need.
+  uint64_t cap = next_pow2(std::max<uint64_t>(2 * max_samples, 1024));
+  if (p->max_table_bytes) {
+    const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
+    uint64_t lim = 1024;
+    while (lim * 2 * slot_bytes <= p->max_table_bytes) lim *= 2;
+    if (cap > lim) cap = lim;
+  }
+  if (cap > (1ull << 32)) return ctx_fail(ctx, LIGHTNE_ERR_OOM, "hash table of %llu slots exceeds 2^32", (unsigned long long)cap);
+  st->table_capacity = cap;
+
+  auto* tab = (Slot*)ctx_buf(ctx, "tab", cap * sizeof(Slot));
+  double* wtab = nullptr;
+  if (p->sample) {
+    wtab = (double*)ctx_buf(ctx, "wtab", cap * sizeof(double));
+    if (!wtab) return LIGHTNE_ERR_OOM;
+  }
+  auto* src = (uint32_t*)ctx_buf(ctx, "edge_src", std::max<int64_t>(g->m, 1) * sizeof(uint32_t));
+  auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
+  if (!tab || !src || !stats) return LIGHTNE_ERR_OOM;
+
+  hipStream_t s = ctx->stream;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+  hipLaunchKernelGGL(k_fill_slots, dim3(4096), dim3(256), 0, s, tab, cap);
+  LN_LAUNCH(ctx);
+  if (wtab) LN_HIP(ctx, hipMemsetAsync(wtab, 0, cap * sizeof(double), s));
+  LN_HIP(ctx, hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
+  if (g->m > 0) {
+    const unsigned nb = (unsigned)((g->m + 255) / 256);
+    hipLaunchKernelGGL(k_edge_src, dim3(nb), dim3(256), 0, s, g->offsets, g->n, g->m, src);
+    LN_LAUNCH(ctx);
+    SampleArgs a;
+    a.off = g->offsets;
+    a.adj = g->edges;
+    a.src = src;
+    a.m = g->m;
+    a.seed = p->seed;
+    a.walk_len = p->walk_len;
+    a.exp_rv = st->exp_rv_rounded;
+    a.frac = st->fractional_bits;
+    a.logn = st->logn;
+    a.tab = tab;
+    a.wtab = wtab;
+    a.mask = cap - 1;
+    a.stats = stats;
+    LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    if (p->sample)
+      hipLaunchKernelGGL(k_sample<1>, dim3(nb), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_sample<0>, dim3(nb), dim3(256), 0, s, a);
+    LN_LAUNCH(ctx);
+  } else {
+    LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+  }
+  LN_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+  unsigned long long hs[8];
+  LN_HIP(ctx, hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  st->samples = hs[0];
+  st->walk_steps = hs[1];
+  st->attempts = hs[2];
+  st->t_sample = ev_seconds(ctx->ev[1], ctx->ev[2]);
+  if (hs[4]) return ctx_fail(ctx, LIGHTNE_ERR_BAD_GRAPH, "random walk reached a vertex of degree 0 (graph not symmetric?)");
+  if (hs[3])
+    return ctx_fail(ctx, LIGHTNE_ERR_TABLE_FULL,
+                    "hash table full (%llu slots); raise max_table_bytes (the reference exits here, par_table.h:104-107)",
+                    (unsigned long long)cap);
+  *tab_out = tab;
+  *wtab_out = wtab;
+  *cap_out = cap;
+  return LIGHTNE_OK;
+}
+
+int table_totals(lightne_ctx* ctx, const Slot* tab, const double* wtab, uint64_t cap, uint64_t* distinct,
+                 uint64_t* cnt_sum, double* w_sum) {
+  const int nb = 2048;
+  auto* pu = (unsigned long long*)ctx_buf(ctx, "tred_u", 2 * nb * sizeof(unsigned long long));
+  auto* pw = (double*)ctx_buf(ctx, "tred_w", nb * sizeof(double));
+  if (!pu || !pw) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_table_reduce, dim3(nb), dim3(256), 0, ctx->stream, tab, wtab, cap, pu, pw);
+  LN_LAUNCH(ctx);
+  std::vector<unsigned long long> hu(2 * nb);
+  std::vector<double> hw(nb);
+  LN_HIP(ctx, hipMemcpyAsync(hu.data(), pu, hu.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+  LN_HIP(ctx, hipMemcpyAsync(hw.data(), pw, hw.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t d = 0, c = 0;
+  double w = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    d += hu[2 * b];
+    c += hu[2 * b + 1];
+    w += hw[b];
+  }
+  *distinct = d;
+  *cnt_sum = c;
+  *w_sum = w;
+  return LIGHTNE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lightne_sample_scalars(int64_t n, int64_t m, const lightne_sparsify_params* p, lightne_sparsify_stats* out) {
+  if (!p || !out || n <= 0 || m < 0) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sample_scalars: bad argument");
+  // PathEmbed.h:89 logn = log(n); :99 table_size; :108-111 M, 1/lambda, rounding
+  const double logn = log((double)(uint64_t)n);
+  const uint64_t tn = p->walk_len * (uint64_t)n;
+  uint64_t ts = (uint64_t)((double)tn * logn * (double)p->mem_ratio);
+  if (p->tablesz > ts) ts = p->tablesz;
+  const double M = (double)tn * logn * (double)p->sample_ratio;
+  const double inv_lambda = M / (double)(uint64_t)m;
+  const uint64_t e = (uint64_t)inv_lambda;
+  out->logn = logn;
+  out->M = M;
+  out->one_over_lambda = inv_lambda;
+  out->exp_rv_rounded = e;
+  out->fractional_bits = inv_lambda - (double)e;
+  out->table_size = ts;
+  return LIGHTNE_OK;
+}
+
+int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                            lightne_table** out, lightne_sparsify_stats* stats) {
+  if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify_counts: NULL argument");
+  *out = nullptr;
+  lightne_sparsify_stats st{};
+  Slot* tab;
+  double* wtab;
+  uint64_t cap;
+  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  uint64_t distinct, csum;
+  double wsum;
+  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
+  st.distinct_keys = distinct;
+  st.M_exact = p->sample ? wsum : (double)csum;
+  DevMem mk, mc, mw, mi;
+  const uint64_t nk = distinct;
+  LN_CHECK(dev_alloc(ctx, mk, std::max<uint64_t>(nk, 1) * 8, "table keys"));
+  LN_CHECK(dev_alloc(ctx, mc, std::max<uint64_t>(nk, 1) * 4, "table counts"));
+  LN_CHECK(dev_alloc(ctx, mw, std::max<uint64_t>(nk, 1) * 8, "table weights"));
+  LN_CHECK(dev_alloc(ctx, mi, std::max<uint64_t>(nk, 1) * 4, "table index"));
+  auto* counter = (unsigned long long*)ctx_buf(ctx, "compact_ctr", 8);
+  if (!counter) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemsetAsync(counter, 0, 8, ctx->stream));
+  hipLaunchKernelGGL(k_table_compact, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, ctx->stream, tab, wtab, cap,
+                     (uint64_t*)mk.p, (uint32_t*)mi.p, counter);
+  LN_LAUNCH(ctx);
+  LN_CHECK(radix_sort_pairs_u32(ctx, (uint64_t*)mk.p, (uint32_t*)mi.p, (int64_t)nk, 32 + bits_for((uint64_t)g->n)));
+  if (nk) {
+    hipLaunchKernelGGL(k_gather_counts, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, tab, wtab,
+                       (const uint32_t*)mi.p, (int64_t)nk, (uint32_t*)mc.p, (double*)mw.p);
+    LN_LAUNCH(ctx);
+  }
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  auto* t = new lightne_table();
+  t->nkeys = nk;
+  t->keys = (uint64_t*)mk.release();
+  t->cnt = (uint32_t*)mc.release();
+  t->wgh = (double*)mw.release();
+  *out = t;
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+int lightne_table_info(const lightne_table* t, uint64_t* nkeys) {
+  if (!t || !nkeys) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_table_info: NULL argument");
+  *nkeys = t->nkeys;
+  return LIGHTNE_OK;
+}
+
+int lightne_table_copy(lightne_ctx* ctx, const lightne_table* t, uint64_t* keys, uint32_t* counts, double* weights) {
+  if (!ctx || !t) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_table_copy: NULL argument");
+  if (t->nkeys == 0) return LIGHTNE_OK;
+  if (keys) LN_HIP(ctx, hipMemcpyAsync(keys, t->keys, t->nkeys * 8, hipMemcpyDefault, ctx->stream));
+  if (counts) LN_HIP(ctx, hipMemcpyAsync(counts, t->cnt, t->nkeys * 4, hipMemcpyDefault, ctx->stream));
+  if (weights) LN_HIP(ctx, hipMemcpyAsync(weights, t->wgh, t->nkeys * 8, hipMemcpyDefault, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int lightne_table_free(lightne_table* t) {
+  if (!t) return LIGHTNE_OK;
+  if (t->keys) (void)hipFree(t->keys);
+  if (t->cnt) (void)hipFree(t->cnt);
+  if (t->wgh) (void)hipFree(t->wgh);
+  delete t;
+  return LIGHTNE_OK;
+}
+
+int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
+                     lightne_sparsify_stats* stats) {
+  if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify: NULL argument");
+  *out = nullptr;
+  lightne_sparsify_stats st{};
+  Slot* tab;
+  double* wtab;
+  uint64_t cap;
+  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  hipStream_t s = ctx->stream;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+  uint64_t distinct, csum;
+  double wsum;
+  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
+  st.distinct_keys = distinct;
+  // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
+  st.M_exact = p->sample ? wsum : (double)csum;
+  const double vol = (double)(uint64_t)g->m;
+  st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+
+  const uint64_t max_emit = std::max<uint64_t>(2 * distinct, 1);
+  DevMem mk, mv;
+  LN_CHECK(dev_alloc(ctx, mk, max_emit * 8, "sparsifier sort keys"));
+  LN_CHECK(dev_alloc(ctx, mv, max_emit * 4, "sparsifier values"));
+  auto* ctr = (unsigned long long*)ctx_buf(ctx, "emit_ctr", 16);
+  if (!ctr) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemsetAsync(ctr, 0, 16, s));
+  const int colbits = std::max(1, bits_for((uint64_t)g->n));
+  TruncArgs ta;
+  ta.tab = tab;
+  ta.wtab = wtab;
+  ta.cap = cap;
+  ta.off = g->offsets;
+  ta.factor = st.factor;
+  ta.upper = p->upper;
+  ta.colbits = colbits;
+  ta.skey = (uint64_t*)mk.p;
+  ta.sval = (float*)mv.p;
+  ta.counter = ctr;
+  ta.kept = ctr + 1;
+  hipLaunchKernelGGL(k_trunc_emit, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, ta);
+  LN_LAUNCH(ctx);
+  unsigned long long hc[2];
+  LN_HIP(ctx, hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  const int64_t nnz = (int64_t)hc[0];
+  st.kept_upper = hc[1];
+  st.nnz = (uint64_t)nnz;
+  LN_CHECK(radix_sort_pairs(ctx, (uint64_t*)mk.p, (float*)mv.p, nnz, 2 * colbits));
+
+  auto* csr = new lightne_csr();
+  DevMem mrp, mcol, mval;
+  int rc = dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr");
+  if (rc == LIGHTNE_OK) rc = dev_alloc(ctx, mcol, std::max<int64_t>(nnz, 1) * 4, "csr col");
+  if (rc != LIGHTNE_OK) {
+    delete csr;
+    return rc;
+  }
+  hipLaunchKernelGGL(k_csr_finish, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, s, (const uint64_t*)mk.p,
+                     nnz, g->n, colbits, (int64_t*)mrp.p, (int32_t*)mcol.p);
+  if (hipGetLastError() != hipSuccess) {
+    delete csr;
+    return ctx_fail(ctx, LIGHTNE_ERR_HIP, "k_csr_finish launch failed");
+  }
+  LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  csr->n = g->n;
+  csr->nnz = nnz;
+  csr->row_ptr = (int64_t*)mrp.release();
+  csr->col = (int32_t*)mcol.release();
+  csr->val = (float*)mv.release();
+  st.t_table = ev_seconds(ctx->ev[3], ctx->ev[4]);
+  st.t_csr = ev_seconds(ctx->ev[4], ctx->ev[5]);
+  // the table is the largest buffer of the pipeline; give it back
+  ctx_release(ctx, "tab");
+  ctx_release(ctx, "wtab");
+  *out = csr;
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+}  // extern "C"

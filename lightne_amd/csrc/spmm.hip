@@ -1,0 +1,311 @@
+// CSR x dense-panel products for stages 2 and 3.
+//
+// Replaces mkl_sparse_?_mm (LightNE/mklredsvd/mklredsvd.cc:235,271 and
+// LightNE/SpectralPropagation.h:101,116,153,168) and the neighbour loop of
+// SpectralPropagation.h:210-221.
+//
+// Layout: X is n x k row-major (k % 16 == 0), FP32 or BF16.  A lane owns 16
+// consecutive columns of one row, so a group of G = k/16 lanes reads a whole
+// X row (576 B at k=144 FP32, 288 B BF16) as G x 64 B (or 32 B) contiguous
+// pieces; a 64-wide wave holds NG = 64/G groups.
+//
+// Rows are split by length (power-law graphs):
+//   * short rows (nnz <= SHORT): one group per row, no cross-lane reduction;
+//   * long rows: chunks of CHUNK nnz, one wave per chunk, the NG groups of the
+//     wave stride over the chunk and reduce through LDS; rows with several
+//     chunks write partial sums that k_spmm_fixup adds in chunk order
+//     (deterministic, no float atomics).
+// Epilogues fuse the Chebyshev recurrence of SpectralPropagation.h:94-208 so
+// each step is exactly two gathers over the graph.
+#include "lightne_internal.h"
+#include "spmm_epi.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int SHORT_NNZ = 48;
+constexpr int CHUNK_NNZ = 512;
+
+template <typename XT>
+__device__ __forceinline__ void load16(const XT* __restrict__ p, float* v);
+
+template <>
+__device__ __forceinline__ void load16<float>(const float* __restrict__ p, float* v) {
+  const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float4 t = q[i];
+    v[4 * i + 0] = t.x;
+    v[4 * i + 1] = t.y;
+    v[4 * i + 2] = t.z;
+    v[4 * i + 3] = t.w;
+  }
+}
+
+template <>
+__device__ __forceinline__ void load16<__hip_bfloat16>(const __hip_bfloat16* __restrict__ p, float* v) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    uint4 t = q[i];
+    uint32_t w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[8 * i + 2 * j + 0] = __uint_as_float(w[j] << 16);
+      v[8 * i + 2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
+    }
+  }
+}
+
+template <typename XT, bool IMPL>
+__device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_t j1, int64_t stride, int colbase,
+                                           float* acc) {
+  const XT* __restrict__ X = reinterpret_cast<const XT*>(a.X);
+  int64_t j = j0;
+  for (; j + stride < j1; j += 2 * stride) {
+    int64_t c0, c1;
+    float w0 = 1.f, w1 = 1.f;
+    if (IMPL) {
+      c0 = a.gcol[j];
+      c1 = a.gcol[j + stride];
+    } else {
+      c0 = a.col[j];
+      c1 = a.col[j + stride];
+      w0 = a.val[j];
+      w1 = a.val[j + stride];
+    }
+    float x0[16], x1[16];
+    load16<XT>(X + c0 * a.k + colbase, x0);
+    load16<XT>(X + c1 * a.k + colbase, x1);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i] + w1 * x1[i];
+  }
+  if (j < j1) {
+    int64_t c0;
+    float w0 = 1.f;
+    if (IMPL) {
+      c0 = a.gcol[j];
+    } else {
+      c0 = a.col[j];
+      w0 = a.val[j];
+    }
+    float x0[16];
+    load16<XT>(X + c0 * a.k + colbase, x0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i];
+  }
+}
+
+// short rows: one group of G lanes per row
+template <typename XT, int EPI, bool IMPL>
+__global__ __launch_bounds__(256) void k_spmm_short(SpmmArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int G = a.G, NG = 64 / G;
+  const int g = lane / G, sub = lane - g * G;
+  if (g >= NG) return;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t r = wave * NG + g;
+  if (r >= a.n) return;
+  const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
+  if (j1 - j0 > a.short_nnz) return;
+  float acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+  gather_row<XT, IMPL>(a, j0, j1, 1, sub * 16, acc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float4 v = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+    epilogue<XT, EPI, IMPL>(a, r, sub * 16 + 4 * q, v);
+  }
+}
+
+// long rows: one wave per chunk
+template <typename XT, int EPI, bool IMPL>
+__global__ __launch_bounds__(256) void k_spmm_long(SpmmArgs a) {
+  extern __shared__ float s_red[];  // [waves][NG][k]
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const int G = a.G, NG = 64 / G;
+  const int g = lane / G, sub = lane - g * G;
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const bool active = c < a.nchunks;
+  float* red = s_red + (size_t)wib * NG * a.k;
+  float acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+  int64_t row = 0;
+  int slot = -1;
+  if (active) {
+    row = a.chunk_row[c];
+    slot = a.chunk_slot[c];
+    if (g < NG) gather_row<XT, IMPL>(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub * 16, acc);
+  }
+  if (active && g < NG) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) red[g * a.k + sub * 16 + i] = acc[i];
+  }
+  __syncthreads();
+  if (!active) return;
+  const int ncol4 = (int)(a.k >> 2);
+  if (lane < ncol4) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int gg = 0; gg < NG; ++gg) {
+      const float4 t = *reinterpret_cast<const float4*>(red + gg * a.k + 4 * lane);
+      s.x += t.x;
+      s.y += t.y;
+      s.z += t.z;
+      s.w += t.w;
+    }
+    if (slot < 0)
+      epilogue<XT, EPI, IMPL>(a, row, 4 * lane, s);
+    else
+      *reinterpret_cast<float4*>(a.partial + (int64_t)slot * a.k + 4 * lane) = s;
+  }
+}
+
+// rows with several chunks: add partials in chunk order, then the epilogue
+template <typename XT, int EPI, bool IMPL>
+__global__ __launch_bounds__(256) void k_spmm_fixup(SpmmArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (w >= a.nmulti) return;
+  const int ncol4 = (int)(a.k >> 2);
+  const int64_t row = a.multi_row[w];
+  const int64_t s0 = a.multi_first[w], s1 = a.multi_first[w + 1];
+  for (int q = lane; q < ncol4; q += 64) {
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int64_t sl = s0; sl < s1; ++sl) {
+      const float4 t = *reinterpret_cast<const float4*>(a.partial + sl * a.k + 4 * q);
+      s.x += t.x;
+      s.y += t.y;
+      s.z += t.z;
+      s.w += t.w;
+    }
+    epilogue<XT, EPI, IMPL>(a, row, 4 * q, s);
+  }
+}
+
+template <typename XT, int EPI, bool IMPL>
+int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
+  a.short_nnz = P.short_nnz;
+  a.nchunks = P.nchunks;
+  a.chunk_start = P.chunk_start;
+  a.chunk_row = P.chunk_row;
+  a.chunk_slot = P.chunk_slot;
+  a.nmulti = P.nmulti;
+  a.multi_row = P.multi_row;
+  a.multi_first = P.multi_first;
+  a.G = (int)(a.k / 16);
+  const int NG = 64 / a.G;
+  if (P.nparts > 0) {
+    a.partial = (float*)ctx_buf(ctx, "spmm_partial", (size_t)P.nparts * a.k * sizeof(float));
+    if (!a.partial) return LIGHTNE_ERR_OOM;
+  }
+  const int64_t waves_short = (a.n + NG - 1) / NG;
+  hipLaunchKernelGGL((k_spmm_short<XT, EPI, IMPL>), dim3((unsigned)((waves_short + 3) / 4)), dim3(256), 0,
+                     ctx->stream, a);
+  LN_LAUNCH(ctx);
+  if (P.nchunks > 0) {
+    const size_t lds = (size_t)4 * NG * a.k * sizeof(float);
+    hipLaunchKernelGGL((k_spmm_long<XT, EPI, IMPL>), dim3((unsigned)((P.nchunks + 3) / 4)), dim3(256), lds,
+                       ctx->stream, a);
+    LN_LAUNCH(ctx);
+  }
+  if (P.nmulti > 0) {
+    hipLaunchKernelGGL((k_spmm_fixup<XT, EPI, IMPL>), dim3((unsigned)((P.nmulti + 3) / 4)), dim3(256), 0,
+                       ctx->stream, a);
+    LN_LAUNCH(ctx);
+  }
+  return LIGHTNE_OK;
+}
+
+}  // namespace
+
+void spmm_plan_free(SpmmPlan& P) {
+  for (void* p : {(void*)P.chunk_start, (void*)P.chunk_row, (void*)P.chunk_slot, (void*)P.multi_row,
+                  (void*)P.multi_first})
+    if (p) (void)hipFree(p);
+  P = SpmmPlan{};
+}
+
+int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P) {
+  spmm_plan_free(P);
+  P.n = n;
+  P.short_nnz = SHORT_NNZ;
+  P.chunk_nnz = CHUNK_NNZ;
+  std::vector<int64_t> rp(n + 1);
+  LN_HIP(ctx, hipMemcpyAsync(rp.data(), rp_dev, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int64_t> cs;
+  std::vector<int32_t> crow, cslot, mrow;
+  std::vector<int64_t> mfirst;
+  int32_t nparts = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    const int64_t len = rp[r + 1] - rp[r];
+    if (len <= SHORT_NNZ) continue;
+    const int64_t nc = (len + CHUNK_NNZ - 1) / CHUNK_NNZ;
+    if (nc > 1) {
+      mrow.push_back((int32_t)r);
+      mfirst.push_back(nparts);
+    }
+    for (int64_t q = 0; q < nc; ++q) {
+      cs.push_back(rp[r] + q * CHUNK_NNZ);
+      crow.push_back((int32_t)r);
+      cslot.push_back(nc > 1 ? nparts++ : -1);
+    }
+  }
+  P.nchunks = (int64_t)crow.size();
+  P.nmulti = (int64_t)mrow.size();
+  P.nparts = nparts;
+  mfirst.push_back(nparts);
+  // (start, end) nnz range of every chunk
+  std::vector<int64_t> se(2 * P.nchunks + 2, 0);
+  for (int64_t c = 0; c < P.nchunks; ++c) {
+    se[2 * c] = cs[c];
+    se[2 * c + 1] = std::min<int64_t>(cs[c] + CHUNK_NNZ, rp[crow[c] + 1]);
+  }
+  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+    if (hipMalloc(dst, bytes ? bytes : 16) != hipSuccess) {
+      (void)hipGetLastError();
+      return ctx_fail(ctx, LIGHTNE_ERR_OOM, "spmm plan allocation failed");
+    }
+    if (bytes) LN_HIP(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return LIGHTNE_OK;
+  };
+  LN_CHECK(up((void**)&P.chunk_start, se.data(), se.size() * sizeof(int64_t)));
+  LN_CHECK(up((void**)&P.chunk_row, crow.data(), crow.size() * sizeof(int32_t)));
+  LN_CHECK(up((void**)&P.chunk_slot, cslot.data(), cslot.size() * sizeof(int32_t)));
+  LN_CHECK(up((void**)&P.multi_row, mrow.data(), mrow.size() * sizeof(int32_t)));
+  LN_CHECK(up((void**)&P.multi_first, mfirst.data(), mfirst.size() * sizeof(int64_t)));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+#define LN_DISPATCH(XT)                                                              \
+  switch (epi) {                                                                     \
+    case EPI_STORE: return impl ? launch<XT, EPI_STORE, true>(ctx, P, a)             \
+                                : launch<XT, EPI_STORE, false>(ctx, P, a);           \
+    case EPI_STORE_F32: return impl ? launch<XT, EPI_STORE_F32, true>(ctx, P, a)     \
+                                    : launch<XT, EPI_STORE_F32, false>(ctx, P, a);   \
+    case EPI_CHEB1: return launch<XT, EPI_CHEB1, true>(ctx, P, a);                   \
+    case EPI_CHEB: return launch<XT, EPI_CHEB, true>(ctx, P, a);                     \
+    case EPI_CHEB_LAST: return launch<XT, EPI_CHEB_LAST, true>(ctx, P, a);           \
+    default: break;                                                                  \
+  }
+
+int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, bool impl) {
+  if (a.k % 16 != 0 || a.k < 16 || a.k > 1024)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel width %lld must be a multiple of 16 in [16, 1024]",
+                    (long long)a.k);
+  if (impl && (epi == EPI_CHEB1 || epi == EPI_CHEB || epi == EPI_CHEB_LAST) && a.impl_mode != IMPL_PROP)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "Chebyshev epilogue needs the propagation operator");
+  if (!impl && epi != EPI_STORE && epi != EPI_STORE_F32)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "explicit SpMM supports store epilogues only");
+  if (xt == PANEL_F32) {
+    LN_DISPATCH(float)
+  } else {
+    LN_DISPATCH(__hip_bfloat16)
+  }
+  return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "unknown SpMM epilogue %d", epi);
+}

@@ -1,0 +1,300 @@
+"""CPU oracle for the LightNE embedding pipeline -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module, and only as the checker.  The product
+(lightne_amd) never imports it.
+
+Stage 1 is the C restatement in lightne_oracle.c (bit-exact contract: hash,
+RNG chain, walk, per-key counts).  Stages 2 and 3 are restated here in FP64
+numpy/scipy, following the reference's schedule line by line:
+
+* ``rsvd``        -- LightNE/mklredsvd/mklredsvd.cc:93-403 (Gaussian sketch from
+  the shared counter-based generator, Householder QR via numpy)
+* ``u_sigma_root``-- LightNE/PathEmbed.h:371-408
+* ``propagate``   -- LightNE/SpectralPropagation.h:51-221
+* ``embed_dense`` -- LightNE/SpectralPropagation.h:16-43
+
+Parity pins for every function are listed in the header of lightne_oracle.c
+and in tests/ (golden vectors from oracle/_ref, closed forms, library routines).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblightne_oracle.so")
+REF_HARNESS = os.path.join(_HERE, "_ref", "ref_harness")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    u64, i64, f64, f32, P = C.c_uint64, C.c_int64, C.c_double, C.c_float, C.c_void_p
+    sig = {
+        "orc_hash64_2": (u64, [u64]),
+        "orc_sample_scalars": (None, [u64, u64, u64, f32, f32, u64, P, P]),
+        "orc_sample": (P, [u64, P, P, u64, u64, C.c_int, u64, f64, f64, C.c_int, C.c_int]),
+        "orc_sample_nkeys": (u64, [P]),
+        "orc_sample_stats": (None, [P, P]),
+        "orc_sample_copy": (None, [P, P, P, P]),
+        "orc_sample_free": (None, [P]),
+        "orc_trunc_log_csr": (P, [u64, P, P, P, u64, u64, C.c_int, f64]),
+        "orc_csr_info": (None, [P, P, P]),
+        "orc_csr_copy": (None, [P, P, P, P]),
+        "orc_csr_free": (None, [P]),
+        "orc_spmm_f64": (None, [u64, P, P, P, P, u64, f64, f64, P, C.c_int]),
+        "orc_spmm_f32": (None, [u64, P, P, P, P, u64, f32, f32, P, C.c_int]),
+        "orc_prop_apply_f64": (None, [u64, P, P, f64, P, u64, f64, f64, P, C.c_int]),
+        "orc_prop_apply_f32": (None, [u64, P, P, f32, P, u64, f32, f32, P, C.c_int]),
+        "orc_aggregate_f64": (None, [u64, P, P, P, u64, P, C.c_int]),
+        "orc_philox4x32": (None, [P, P, P]),
+        "orc_gaussian": (None, [u64, C.c_uint32, u64, P]),
+        "orc_max_threads": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data
+
+
+def hash64_2(x: int) -> int:
+    return int(load().orc_hash64_2(x & 0xFFFFFFFFFFFFFFFF))
+
+
+def sample_scalars(n: int, m: int, walk_len: int, sample_ratio: float, mem_ratio: float = 1.0,
+                   tablesz: int = 0) -> dict:
+    d = np.zeros(4, np.float64)
+    u = np.zeros(2, np.uint64)
+    load().orc_sample_scalars(n, m, walk_len, sample_ratio, mem_ratio, tablesz, _p(d), _p(u))
+    return {"logn": d[0], "M": d[1], "one_over_lambda": d[2], "fractional_bits": d[3],
+            "exp_rv_rounded": int(u[0]), "table_size": int(u[1])}
+
+
+def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sample_ratio: float,
+           sample_mode: int = 0, threads: int = 0, cap_log2: int = 0):
+    """Per-directed-edge sampler + table (PathEmbed.h:128-188).
+
+    Returns (keys uint64 sorted, counts uint64, weights float64, stats dict)."""
+    lib = load()
+    off = np.ascontiguousarray(offsets, np.uint64)
+    adj = np.ascontiguousarray(edges, np.uint32)
+    n = off.shape[0] - 1
+    m = int(off[-1])
+    sc = sample_scalars(n, m, walk_len, sample_ratio)
+    per_edge = sc["exp_rv_rounded"] + (1 if sc["fractional_bits"] > 1e-10 else 0)
+    if cap_log2 == 0:
+        cap_log2 = max(10, int(np.ceil(np.log2(max(2 * m * per_edge, 1024)))))
+    if threads <= 0:
+        threads = int(lib.orc_max_threads())
+    h = lib.orc_sample(n, _p(off), _p(adj), seed & 0xFFFFFFFFFFFFFFFF, walk_len, sample_mode,
+                       sc["exp_rv_rounded"], sc["fractional_bits"], sc["logn"], cap_log2, threads)
+    try:
+        nk = int(lib.orc_sample_nkeys(h))
+        st = np.zeros(4, np.uint64)
+        lib.orc_sample_stats(h, _p(st))
+        keys = np.empty(nk, np.uint64)
+        cnt = np.empty(nk, np.uint64)
+        wgh = np.empty(nk, np.float64)
+        lib.orc_sample_copy(h, _p(keys), _p(cnt), _p(wgh))
+    finally:
+        lib.orc_sample_free(h)
+    stats = dict(sc)
+    stats.update(samples=int(st[0]), attempts=int(st[1]), walk_steps=int(st[2]), overflow=int(st[3]))
+    return keys, cnt, wgh, stats
+
+
+def trunc_log_csr(offsets: np.ndarray, keys: np.ndarray, vals: np.ndarray, negative: int = 1, upper: int = 0,
+                  M_exact: float = 0.0):
+    """Trunc-log + CSR (PathEmbed.h:175-311).  Returns (row_ptr, col, val, info).
+    ``M_exact`` > 0 overrides the total weight (its summation order is only
+    visible with -sample 1)."""
+    lib = load()
+    off = np.ascontiguousarray(offsets, np.uint64)
+    k = np.ascontiguousarray(keys, np.uint64)
+    v = np.ascontiguousarray(vals, np.float64)
+    n = off.shape[0] - 1
+    h = lib.orc_trunc_log_csr(n, _p(off), _p(k), _p(v), k.shape[0], negative, upper, float(M_exact))
+    try:
+        iu = np.zeros(3, np.uint64)
+        dd = np.zeros(2, np.float64)
+        lib.orc_csr_info(h, _p(iu), _p(dd))
+        nnz = int(iu[1])
+        rp = np.empty(n + 1, np.int64)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float32)
+        lib.orc_csr_copy(h, _p(rp), _p(col), _p(val))
+    finally:
+        lib.orc_csr_free(h)
+    return rp, col, val, {"kept_upper": int(iu[2]), "M_exact": float(dd[0]), "factor": float(dd[1]), "nnz": nnz}
+
+
+def philox4x32(ctr, key) -> np.ndarray:
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    load().orc_philox4x32(_p(c), _p(k), _p(out))
+    return out
+
+
+def gaussian(seed: int, stream: int, count: int) -> np.ndarray:
+    out = np.empty(count, np.float32)
+    load().orc_gaussian(seed & 0xFFFFFFFFFFFFFFFF, stream, count, _p(out))
+    return out
+
+
+def spmm_f64(rp, col, val, X: np.ndarray, threads: int = 0) -> np.ndarray:
+    lib = load()
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float32)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.empty_like(X)
+    lib.orc_spmm_f64(rp.shape[0] - 1, _p(rp), _p(col), _p(val), _p(X), X.shape[1], 1.0, 0.0, _p(Y),
+                     threads or int(lib.orc_max_threads()))
+    return Y
+
+
+def spmm_f32(rp, col, val, X: np.ndarray, threads: int = 0) -> np.ndarray:
+    lib = load()
+    rp = np.ascontiguousarray(rp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    Y = np.empty_like(X)
+    lib.orc_spmm_f32(rp.shape[0] - 1, _p(rp), _p(col), _p(val), _p(X), X.shape[1], 1.0, 0.0, _p(Y),
+                     threads or int(lib.orc_max_threads()))
+    return Y
+
+
+# ---------------------------------------------------------------------------
+# stage 2: randomized SVD, FP64 (mklredsvd.cc:93-403)
+# ---------------------------------------------------------------------------
+class _Op:
+    """Sparse operator for the FP64 restatement (explicit CSR)."""
+
+    def __init__(self, rp, col, val, threads=0):
+        self.rp, self.col, self.val, self.threads = rp, col, val, threads
+
+    def __matmul__(self, X):
+        return spmm_f64(self.rp, self.col, self.val, X, self.threads)
+
+
+def rsvd(rp, col, val, rank: int, seed: int = 777, power_iters: int = 1, random_project_only: bool = False,
+         threads: int = 0, dtype=np.float64):
+    """Returns (U n x rank, S rank) following mklredsvd.cc:141-400; S is None
+    with random_project_only."""
+    n = rp.shape[0] - 1
+    A = _Op(rp, col, val, threads)
+    Om = gaussian(seed, 0, n * rank).reshape(n, rank).astype(dtype)
+    Y = A @ Om
+    if random_project_only:
+        return Y, None
+    Q = np.linalg.qr(Y)[0]
+    for _ in range(max(1, power_iters) - 1):
+        Q = np.linalg.qr(A @ Q)[0]
+    B = A @ Q
+    P = gaussian(seed, 1, rank * rank).reshape(rank, rank).astype(dtype)
+    Z = np.linalg.qr(B @ P)[0]
+    Cm = Z.T @ B
+    Uc, S, _ = np.linalg.svd(Cm)
+    return Z @ Uc, S
+
+
+def u_sigma_root(U: np.ndarray, S, dim: int, normalize: bool) -> np.ndarray:
+    """PathEmbed.h:371-408 in the dtype of U."""
+    s = np.ones(dim, U.dtype) if S is None else np.sqrt(np.asarray(S[:dim], U.dtype))
+    emb = U[:, :dim] * s[None, :]
+    if normalize:
+        nrm = np.linalg.norm(emb, axis=1)
+        scale = np.where(nrm > 1e-5, 1.0 / np.where(nrm > 0, nrm, 1.0), 1.0)
+        emb = emb * scale[:, None]
+    return emb
+
+
+# ---------------------------------------------------------------------------
+# stage 3: Chebyshev propagation, FP64 (SpectralPropagation.h:45-229)
+# ---------------------------------------------------------------------------
+def bessel_iv(k: int, x: float) -> float:
+    """I_k(x) by its power series (replaces boost cyl_bessel_i, SpectralPropagation.h:12)."""
+    h = 0.5 * x
+    term = 1.0
+    for i in range(1, k + 1):
+        term *= h / i
+    s = term
+    for j in range(1, 200):
+        term *= h * h / (j * (j + k))
+        s += term
+        if term < s * 1e-18:
+            break
+    return s
+
+
+def cheb_coeffs(order: int, theta: float) -> np.ndarray:
+    K = max(order, 2)
+    c = np.zeros(K)
+    c[0] = bessel_iv(0, theta)
+    c[1] = -2 * bessel_iv(1, theta)
+    for i in range(2, K):
+        c[i] = (-2 if (i & 1) else 2) * bessel_iv(i, theta)
+    return c
+
+
+def prop_matrix(offsets, edges, mu: float):
+    """Explicit M = (1 - mu) I - D~^-1 A~ as scipy CSR (self loop first in each row,
+    SpectralPropagation.h:51-75), FP64 values."""
+    import scipy.sparse as sp
+
+    off = np.asarray(offsets, np.int64)
+    n = off.shape[0] - 1
+    deg = np.diff(off).astype(np.float64) + 1.0
+    rows = np.repeat(np.arange(n), np.diff(off))
+    data = np.concatenate([1.0 - mu - 1.0 / deg, -1.0 / deg[rows]])
+    r = np.concatenate([np.arange(n), rows])
+    c = np.concatenate([np.arange(n), np.asarray(edges, np.int64)])
+    return sp.csr_matrix((data, (r, c)), shape=(n, n))
+
+
+def propagate(offsets, edges, a: np.ndarray, order: int = 10, mu: float = 0.2, theta: float = 0.5):
+    """y = a - sum_i c_i T_i a, z = (A + I) y in FP64; returns (y, z)."""
+    import scipy.sparse as sp
+
+    a = np.asarray(a, np.float64)
+    M = prop_matrix(offsets, edges, float(np.float32(mu)))
+    c = cheb_coeffs(order, float(np.float32(theta)))
+    Lx0 = a.copy()
+    tmp = M @ a
+    Lx1 = 0.5 * (M @ tmp) - a
+    conv = c[0] * Lx0 + c[1] * Lx1
+    for i in range(2, max(order, 2)):
+        tmp = M @ Lx1
+        Lx2 = (M @ tmp - Lx0) - 2.0 * Lx1
+        conv = conv + c[i] * Lx2
+        Lx0, Lx1 = Lx1, Lx2
+    y = a - conv
+    off = np.asarray(offsets, np.int64)
+    n = off.shape[0] - 1
+    rows = np.repeat(np.arange(n), np.diff(off))
+    A = sp.csr_matrix((np.ones(len(edges)), (rows, np.asarray(edges, np.int64))), shape=(n, n))
+    z = y + A @ y
+    return y, z
+
+
+def embed_dense(mat: np.ndarray, dim: int):
+    """SpectralPropagation.h:16-43: thin SVD, rownorm(U[:, :dim] sqrt(S)); returns (emb, S)."""
+    U, S, _ = np.linalg.svd(np.asarray(mat, np.float64), full_matrices=False)
+    return u_sigma_root(U, S, dim, True), S

@@ -1,0 +1,147 @@
+"""Stage 2 (randomized SVD) on the device vs the FP64 oracle of the same
+algorithm with the same Gaussian sketch."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import lightne_amd as L
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _csr_np(A):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data.astype(np.float32)
+
+
+def _power_law_sym(n, seed, hub_nnz=3000):
+    """Symmetric sparse matrix with short rows, empty rows and hub rows longer
+    than several SpMM chunks."""
+    rng = np.random.default_rng(seed)
+    deg = np.minimum((rng.pareto(1.3, n) * 3).astype(int), n - 1)
+    deg[:3] = hub_nnz
+    deg[10:20] = 0
+    r = np.repeat(np.arange(n), deg)
+    c = rng.integers(0, n, r.shape[0])
+    c[deg[r] == 0] = 0
+    v = rng.standard_normal(r.shape[0])
+    A = sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+    A = A + A.T
+    A.sum_duplicates()
+    return A.astype(np.float32)
+
+
+@pytest.mark.parametrize("k", [16, 128, 144, 256])
+def test_spmm_sketch_matches_oracle(gpu, k):
+    """random_project_only returns Y = A Omega: the SpMM (short, long and
+    multi-chunk rows) and the Gaussian sketch in one check."""
+    n = 6000
+    rp, col, val = _csr_np(_power_law_sym(n, 1))
+    assert np.diff(rp).max() > 2048
+    csr = L.CSR.from_arrays(rp, col, val, ctx=gpu)
+    svd = L.MKLRedSVD(csr, k, random_project_only=True, seed=777, ctx=gpu).run()
+    Y = svd.matU.cpu().numpy()
+    assert svd.S is None
+    Om = O.gaussian(777, 0, n * k).reshape(n, k)
+    ref = O.spmm_f64(rp, col, val, Om.astype(np.float64))
+    scale = np.abs(val).max() * np.abs(Om).max() * np.sqrt(np.diff(rp).max())
+    err = np.abs(Y - ref).max() / scale
+    assert err < 2e-6, err
+
+
+def test_gaussian_sketch_bitwise_close(gpu):
+    n, k = 4096, 144
+    eye = sp.identity(n, dtype=np.float32, format="csr")
+    csr = L.CSR.from_arrays(*_csr_np(eye), ctx=gpu)
+    Y = L.MKLRedSVD(csr, k, random_project_only=True, seed=2024, ctx=gpu).run().matU.cpu().numpy()
+    ref = O.gaussian(2024, 0, n * k).reshape(n, k)
+    d = np.abs(Y.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert d.max() <= 2, d.max()
+    assert np.mean(d == 0) > 0.99
+
+
+def test_spmm_bf16_panel(gpu):
+    n, k = 5000, 144
+    rp, col, val = _csr_np(_power_law_sym(n, 2))
+    csr = L.CSR.from_arrays(rp, col, val, ctx=gpu)
+    Y = L.MKLRedSVD(csr, k, random_project_only=True, precision=L.BF16, seed=5, ctx=gpu).run().matU.cpu().numpy()
+    Om = O.gaussian(5, 0, n * k).reshape(n, k)
+    import torch
+
+    Omb = torch.from_numpy(Om).to(torch.bfloat16).to(torch.float64).numpy()
+    ref = O.spmm_f64(rp, col, val, Omb)
+    scale = np.abs(val).max() * 4 * np.sqrt(np.diff(rp).max())
+    assert np.abs(Y - ref).max() / scale < 2e-6
+
+
+def test_exact_rank_recovery(gpu):
+    rng = np.random.default_rng(7)
+    n, r, k = 2000, 24, 64
+    Q = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    lam = np.linspace(9, 1, r) * rng.choice([-1, 1], r)
+    A = ((Q * lam) @ Q.T).astype(np.float32)
+    csr = L.CSR.from_arrays(*_csr_np(A), ctx=gpu)
+    svd = L.MKLRedSVD(csr, k, seed=1, ctx=gpu).run()
+    np.testing.assert_allclose(svd.S[:r], np.sort(np.abs(lam))[::-1], rtol=2e-5)
+    assert svd.S[r:].max() < 1e-4 * svd.S[0]
+    U = svd.matU.cpu().numpy()
+    assert H.principal_angles(U[:, :r], Q) < 1e-3
+
+
+def _c1_sparsifier(gpu):
+    hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
+    csr, st = L.generate_trunc_log_matrix(hg.to_device(), 1, walk_len=10, sample_ratio=ratio, ctx=gpu)
+    return hg, csr
+
+
+@pytest.mark.parametrize("q", [1, 2])
+def test_rsvd_c1_matches_fp64_oracle(gpu, q):
+    """Config 1: k = 144, d = 128.  Singular values within 1e-4 relative, top-d
+    subspace within 1e-3 rad of the FP64 run with the same sketch."""
+    hg, csr = _c1_sparsifier(gpu)
+    k, d = 144, 128
+    svd = L.MKLRedSVD(csr, k, power_iters=q, seed=777, ctx=gpu).run()
+    rp, col, val = csr.to_numpy()
+    U64, S64 = O.rsvd(rp, col, val, k, seed=777, power_iters=q)
+    rel = np.abs(svd.S[:d] - S64[:d]) / S64[:d]
+    ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d], U64[:, :d])
+    gap = S64[d - 1] / S64[d]
+    print(f"q={q} max rel sv err={rel.max():.2e} angle={ang:.2e} rad gap={gap:.4f} S0={S64[0]:.4g}")
+    assert rel.max() <= 1e-4
+    assert ang <= 1e-3
+    emb = L.compute_u_sigma_root(svd.matU, svd.S, hg.n, k, d, False, ctx=gpu).cpu().numpy()
+    ref = O.u_sigma_root(svd.matU.cpu().numpy().astype(np.float64), svd.S.astype(np.float64), d, False)
+    np.testing.assert_allclose(emb, ref, rtol=1e-6, atol=1e-7)
+    embn = L.compute_u_sigma_root(svd.matU, svd.S, hg.n, k, d, True, ctx=gpu).cpu().numpy()
+    refn = O.u_sigma_root(svd.matU.cpu().numpy().astype(np.float64), svd.S.astype(np.float64), d, True)
+    np.testing.assert_allclose(embn, refn, rtol=1e-5, atol=1e-6)
+
+
+def test_rsvd_bf16_panels_error_vs_fp64(gpu):
+    hg, csr = _c1_sparsifier(gpu)
+    k, d = 144, 128
+    svd = L.MKLRedSVD(csr, k, precision=L.BF16, seed=777, ctx=gpu).run()
+    rp, col, val = csr.to_numpy()
+    U64, S64 = O.rsvd(rp, col, val, k, seed=777)
+    rel = np.abs(svd.S[:d] - S64[:d]) / S64[:d]
+    ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d], U64[:, :d])
+    print(f"BF16: max rel sv err={rel.max():.2e} angle={ang:.2e}")
+    assert rel.max() < 1e-2
+    assert ang < 5e-2
+
+
+def test_netsmf_end_to_end(gpu):
+    hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
+    g = hg.to_device()
+    emb, S, st = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, ctx=gpu)
+    csr, _ = L.generate_trunc_log_matrix(g, 1, walk_len=10, sample_ratio=ratio, ctx=gpu)
+    svd = L.MKLRedSVD(csr, 144, seed=777, ctx=gpu).run()
+    ref = L.compute_u_sigma_root(svd.matU, svd.S, hg.n, 144, 128, False, ctx=gpu)
+    np.testing.assert_array_equal(emb.cpu().numpy(), ref.cpu().numpy())
+    np.testing.assert_array_equal(S, svd.S)
+    assert st["t_total"] > 0

@@ -1,0 +1,212 @@
+"""The CPU oracle pinned against the reference's own code and against what the
+mathematics fixes (CPU only)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import helpers as H
+
+MASK = (1 << 64) - 1
+
+
+def py_hash64_2(x):
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK
+    return x ^ (x >> 31)
+
+
+def test_hash64_2_known_answers():
+    # 0 is a fixed point of the splitmix finaliser; others against big-int arithmetic
+    assert O.hash64_2(0) == 0
+    for x in [1, 2, 0xDEADBEEF, 1 << 63, MASK, 0x123456789ABCDEF0]:
+        assert O.hash64_2(x) == py_hash64_2(x)
+
+
+@pytest.mark.parametrize("rec", H.manifest(), ids=lambda r: r["case"])
+def test_sampler_matches_reference_harness(rec):
+    """Key -> value tables of the reference-header harness (pbbs::random, hash64_2,
+    par_table, ligra reader) reproduced bit for bit, both -sample modes."""
+    off, edg = H.golden_graph(rec["graph"])
+    gd = H.golden_case(rec["case"])
+    keys, cnt, wgh, st = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"],
+                                  rec["sample"], threads=1)
+    sc = gd["scalars"]
+    assert st["logn"] == sc[0] and st["M"] == sc[1] and st["one_over_lambda"] == sc[2]
+    assert st["fractional_bits"] == sc[3] and st["exp_rv_rounded"] == int(gd["scalars_u"][0])
+    assert st["overflow"] == 0
+    assert len(keys) == rec["nkeys"]
+    if rec["full"]:
+        np.testing.assert_array_equal(keys, gd["keys"])
+        np.testing.assert_array_equal(wgh, gd["vals"])
+    else:
+        h = hashlib.sha256()
+        h.update(keys.tobytes())
+        h.update(wgh.tobytes())
+        assert h.hexdigest() == rec["sha256_keys_vals"]
+        np.testing.assert_array_equal(keys[gd["sub_idx"]], gd["sub_keys"])
+    # invariants: counts sum to the emitted samples; -sample 0 weights are the counts
+    assert int(cnt.sum()) == st["samples"]
+    if rec["sample"] == 0:
+        np.testing.assert_array_equal(wgh, cnt.astype(np.float64))
+        assert st["samples"] == st["attempts"]
+
+
+@pytest.mark.parametrize("rec", [r for r in H.manifest()], ids=lambda r: r["case"])
+def test_trunc_log_matches_reference_harness(rec):
+    off, edg = H.golden_graph(rec["graph"])
+    gd = H.golden_case(rec["case"])
+    if rec["full"]:
+        keys, vals, tl = gd["keys"], gd["vals"], gd["trunclog"]
+    else:
+        keys, _, vals, _ = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"],
+                                    rec["sample"], threads=1)
+        tl = None
+    rp, col, val, info = O.trunc_log_csr(off, keys, vals, M_exact=float(gd["scalars"][4]))
+    assert info["factor"] == gd["scalars"][5]
+    assert info["kept_upper"] == rec["kept"]
+    # CSR invariants: symmetric, rows sorted, nnz = 2 kept - diagonal
+    n = len(off) - 1
+    assert rp[0] == 0 and rp[-1] == len(col)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert np.all(np.diff(rows.astype(np.int64) * n + col) > 0)
+    diag = int(np.sum(rows == col))
+    assert len(col) == 2 * info["kept_upper"] - diag
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    assert (A != A.T).nnz == 0
+    if tl is not None:
+        keep = np.isfinite(tl)
+        k = keys[keep]
+        u = (k >> np.uint64(32)).astype(np.int64)
+        v = (k & np.uint64(0xFFFFFFFF)).astype(np.int64)
+        np.testing.assert_array_equal(np.asarray(A[u, v]).ravel(), tl[keep].astype(np.float32))
+
+
+def test_walk_len_one_closed_form():
+    """T = 1: every sample is the source edge itself, so count(u<v) is the sum of
+    the two directed edges' exp_rv draws, in {2E, 2E+1, 2E+2}."""
+    off, edg = H.golden_graph("cl200")
+    n = len(off) - 1
+    m = int(off[-1])
+    ratio = H.c1_sample_ratio(n, m, T=1, per_edge=3.4)
+    keys, cnt, _, st = O.sample(off, edg, 5, 1, ratio, 0, threads=1)
+    E = st["exp_rv_rounded"]
+    rows = np.repeat(np.arange(n), np.diff(off))
+    und = np.unique(np.minimum(rows, edg).astype(np.uint64) << np.uint64(32) | np.maximum(rows, edg).astype(np.uint64))
+    np.testing.assert_array_equal(keys, und)
+    assert np.all((cnt >= 2 * E) & (cnt <= 2 * E + 2))
+    assert st["walk_steps"] == 0
+
+
+def test_expected_counts_match_netmf_matrix():
+    """E[count(x,z)] = (M/2m)(1/T) sum_r [(A (D^-1 A)^(r-1))_xz + sym], i.e. the
+    sampled matrix concentrates on the DeepWalk/NetMF matrix (toy graph, many seeds)."""
+    off, edg = H.golden_graph("toy")
+    n = len(off) - 1
+    m = int(off[-1])
+    T = 3
+    A = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(off))
+    A[rows, edg] = 1.0
+    P = A / A.sum(1, keepdims=True)
+    ratio = float(np.float32(400.0 * m / (T * n * np.log(n))))
+    tot = np.zeros((n, n))
+    nrep = 12
+    for s in range(nrep):
+        keys, cnt, _, st = O.sample(off, edg, 1000 + s, T, ratio, 0, threads=1)
+        u = (keys >> np.uint64(32)).astype(int)
+        v = (keys & np.uint64(0xFFFFFFFF)).astype(int)
+        tot[u, v] += cnt
+    per_edge = st["one_over_lambda"]
+    # directed edge (u,v) of length r: x ~ (P^(k-1))[u], z ~ (P^(r-k))[v], k uniform
+    exp = np.zeros((n, n))
+    for r in range(1, T + 1):
+        for k in range(1, r + 1):
+            exp += (np.linalg.matrix_power(P.T, k - 1) @ A @ np.linalg.matrix_power(P, r - k)) / (r * T)
+    exp = per_edge * nrep * (exp + exp.T - np.diag(np.diag(exp)))
+    iu = np.triu_indices(n)
+    e, o = exp[iu], tot[iu]
+    assert np.all(o[e == 0] == 0)  # unreachable pairs are never sampled
+    big = e > 20
+    z = (o[big] - e[big]) / np.sqrt(e[big])  # counts are (close to) Poisson
+    assert np.abs(z).max() < 5.0, np.abs(z).max()
+    assert 0.3 < np.mean(z ** 2) < 2.0, np.mean(z ** 2)
+    assert np.abs(o.sum() - e.sum()) / e.sum() < 0.01
+
+
+def test_philox_known_answers():
+    """Philox4x32-10 known-answer vectors (Random123 kat_vectors)."""
+    out = O.philox4x32([0, 0, 0, 0], [0, 0])
+    np.testing.assert_array_equal(out, np.array([0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8], np.uint32))
+    out = O.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)
+    np.testing.assert_array_equal(out, np.array([0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD], np.uint32))
+    out = O.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0])
+    np.testing.assert_array_equal(out, np.array([0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1], np.uint32))
+
+
+def test_gaussian_moments():
+    x = O.gaussian(777, 0, 2_000_000).astype(np.float64)
+    assert abs(x.mean()) < 3e-3 and abs(x.var() - 1) < 5e-3
+    assert abs(np.mean(x ** 4) - 3.0) < 3e-2
+    y = O.gaussian(777, 1, 1000)
+    assert not np.array_equal(y, x[:1000].astype(np.float32))
+
+
+def test_bessel_series_vs_scipy():
+    from scipy.special import iv
+
+    for k in range(12):
+        for x in [0.0, 0.1, 0.5, 1.0, 2.5]:
+            assert abs(O.bessel_iv(k, x) - iv(k, x)) <= 4e-15 * max(1.0, iv(k, x))
+
+
+def test_spmm_oracle_vs_scipy():
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(0)
+    A = sp.random(300, 300, density=0.05, random_state=1, format="csr", dtype=np.float32)
+    X = rng.standard_normal((300, 32))
+    Y = O.spmm_f64(A.indptr, A.indices, A.data, X)
+    np.testing.assert_allclose(Y, A.astype(np.float64) @ X, rtol=1e-12, atol=1e-12)
+
+
+def test_propagation_oracle_is_truncated_exponential():
+    """y = a - sum_{k<K} c_k T_k(X) a with X = M^2/2 - I and c_k from I_k(theta):
+    exp(-theta X) = sum c_k T_k(X), so for large K, y -> a - expm(-theta X) a."""
+    from scipy.linalg import expm
+
+    off, edg = H.golden_graph("toy")
+    n = len(off) - 1
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((n, 4))
+    M = O.prop_matrix(off, edg, float(np.float32(0.2))).toarray()
+    X = 0.5 * M @ M - np.eye(n)
+    ev = np.linalg.eigvals(X).real
+    assert ev.min() >= -1 - 1e-12 and ev.max() < 1
+    theta = float(np.float32(0.5))
+    y, z = O.propagate(off, edg, a, order=30, mu=0.2, theta=0.5)
+    np.testing.assert_allclose(y, a - expm(-theta * X) @ a, atol=1e-12)
+    y10, _ = O.propagate(off, edg, a, order=10, mu=0.2, theta=0.5)
+    assert np.abs(y10 - y).max() < 1e-8  # K = 10 truncation error at theta = 0.5
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(off)), edg] = 1
+    np.testing.assert_allclose(z, (A + np.eye(n)) @ y, atol=1e-12)
+
+
+def test_rsvd_oracle_exact_rank():
+    """A symmetric matrix of rank <= k is recovered to rounding (singular values)."""
+    rng = np.random.default_rng(7)
+    n, r = 400, 20
+    Q = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    lam = np.linspace(5, 1, r) * rng.choice([-1, 1], r)
+    A = (Q * lam) @ Q.T
+    import scipy.sparse as sp
+
+    S_ = sp.csr_matrix(A.astype(np.float32))
+    U, S = O.rsvd(S_.indptr, S_.indices, S_.data, 32, seed=777)
+    np.testing.assert_allclose(S[:r], np.sort(np.abs(lam))[::-1], rtol=1e-5)
+    assert S[r:].max() < 1e-5
+    assert H.principal_angles(U[:, :r], Q) < 1e-5
