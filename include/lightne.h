@@ -220,6 +220,23 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
                    const lightne_rsvd_params* rp, int64_t dim, int32_t normalize, float* emb_out,
                    float* S_host, lightne_netsmf_stats* stats);
 
+/* ---- dense panel kernels (building blocks of stages 2 and 3) ---------------
+ * No reference counterpart as separate calls: they are the pieces that replace
+ * the reference's cblas_?gemm (mklredsvd.cc:297,330,382) and the
+ * geqrf/orgqr orthonormalisation (mklredsvd/util.cc:41-79); exported so each
+ * can be checked on its own.  Panels are n x k row-major FP32 on the device,
+ * k and k2 multiples of 16 up to 256. */
+/* G_host (k1 x k2, FP64) = X^T Y; symmetric != 0 requires X == Y, k1 == k2 */
+int lightne_dense_gram(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                       int32_t symmetric, double* G_host);
+/* Out (n x k2) = X (n x k1) W_host (k1 x k2, used as FP32); optional row L2
+ * normalisation; Out may alias X when k1 == k2 */
+int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, const double* W_host,
+                             int64_t k2, float* Out, int32_t normalize_rows);
+/* CholeskyQR2 with FP64 Gram/Cholesky: Y <- Q in place, R_host (k x k) with
+ * Y_in = Q R; passes_out (may be NULL) receives the number of passes */
+int lightne_dense_cholqr(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_host, int32_t* passes_out);
+
 /* ---- host graph utilities (no device work) ---------------------------------- */
 /* Ligra AdjacencyGraph text reader/writer (ligra/graph_io.h:185-231).
  * *offsets_host has n+1 entries; free both with lightne_host_free. */

@@ -129,6 +129,9 @@ _SIGS = {
                                              _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
     "lightne_netsmf": (C.c_int, [_P, C.POINTER(Graph), C.POINTER(SparsifyParams), C.POINTER(RsvdParams), _I64,
                                  C.c_int32, _P, _P, C.POINTER(NetsmfStats)]),
+    "lightne_dense_gram": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, C.c_int32, _P]),
+    "lightne_dense_panel_gemm": (C.c_int, [_P, _P, _I64, _I64, _P, _I64, _P, C.c_int32]),
+    "lightne_dense_cholqr": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int32)]),
     "lightne_graph_read_adj": (C.c_int, [C.c_char_p, _PI64, _PI64, _PP, _PP]),
     "lightne_graph_write_adj": (C.c_int, [C.c_char_p, _I64, _P, _P]),
     "lightne_gen_chung_lu": (C.c_int, [_I64, C.c_double, C.c_double, C.c_double, C.c_uint64, _PI64, _PP, _PP]),

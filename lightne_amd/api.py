@@ -414,6 +414,45 @@ def chebyshev_expansion(a, graph: DeviceGraph, rank: int, dim: int, order: int =
     return emb, _stats_dict(st)
 
 
+# ---------------------------------------------------------------------------
+# dense panel building blocks
+# ---------------------------------------------------------------------------
+def dense_gram(X, Y=None, ctx: Optional[Context] = None) -> np.ndarray:
+    """X^T Y in FP64 (device FP32 panels)."""
+    ctx = ctx or context()
+    sym = Y is None
+    Y = X if sym else Y
+    n, k1 = X.shape
+    k2 = Y.shape[1]
+    G = np.zeros((k1, k2), np.float64)
+    ctx.check(ctx.lib.lightne_dense_gram(ctx.ptr, _dev(X), _dev(Y), n, k1, k2, int(sym), G.ctypes.data))
+    return G
+
+
+def dense_panel_gemm(X, W: np.ndarray, normalize_rows: bool = False, out=None, ctx: Optional[Context] = None):
+    """X (n x k1, device FP32) times W (k1 x k2 host) -> device FP32 n x k2."""
+    torch = _torch()
+    ctx = ctx or context()
+    n, k1 = X.shape
+    W = np.ascontiguousarray(W, np.float64)
+    k2 = W.shape[1]
+    if out is None:
+        out = torch.empty((n, k2), dtype=torch.float32, device=X.device)
+    ctx.check(ctx.lib.lightne_dense_panel_gemm(ctx.ptr, _dev(X), n, k1, W.ctypes.data, k2, _dev(out),
+                                               int(bool(normalize_rows))))
+    return out
+
+
+def dense_cholqr(Y, ctx: Optional[Context] = None):
+    """CholeskyQR2 in place: returns (Q (= Y), R, passes)."""
+    ctx = ctx or context()
+    n, k = Y.shape
+    R = np.zeros((k, k), np.float64)
+    passes = C.c_int32()
+    ctx.check(ctx.lib.lightne_dense_cholqr(ctx.ptr, _dev(Y), n, k, R.ctypes.data, C.byref(passes)))
+    return Y, R, passes.value
+
+
 def bessel_i(k: int, x: float) -> float:
     """Modified Bessel I_k(x) as used for the Chebyshev weights."""
     return load().lightne_bessel_i(k, x)

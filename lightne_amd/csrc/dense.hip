@@ -56,8 +56,10 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
 // ---------------------------------------------------------------------------
 // Gram: rows are staged through LDS in slabs of GR_ROWS; every wave owns a
 // fixed subset of the 16x16 output tiles and walks the slab 4 rows at a time.
-// MFMA 16x16x4 operand layout: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the accumulator holds D[4*(l/16)+v][l%16] in register v.
+// v_mfma_f64_16x16x4f64 operand layout: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; accumulator register v of lane l holds D[(l/16) + 4v][l%16]
+// (rows interleaved, unlike the f32 16x16x4 form whose register v holds row
+// 4(l/16) + v).
 // ---------------------------------------------------------------------------
 constexpr int GR_ROWS = 32;
 constexpr int GR_THREADS = 256;
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict
     if (pi < npairs) {
       const int2 pr = pairs[pi];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) out[(int64_t)(pr.x * 16 + 4 * lk + v) * k2 + pr.y * 16 + li] = acc[q][v];
+      for (int v = 0; v < 4; ++v) out[(int64_t)(pr.x * 16 + lk + 4 * v) * k2 + pr.y * 16 + li] = acc[q][v];
     }
   }
 }
@@ -315,6 +317,8 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
   auto* dW = (float*)ctx_buf(ctx, "small_w", wf.size() * sizeof(float));
   if (!dW) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemcpyAsync(dW, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  // wf is a local: the copy must have consumed it before this function returns
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (n <= 0) return LIGHTNE_OK;
   float* dst = Out;
   if (kb < k2 && Out == X) {  // later column blocks must still read the old X
@@ -362,6 +366,7 @@ int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int6
   auto* dsq = (float*)ctx_buf(ctx, "usr_sq", dim * sizeof(float));
   if (!dsq) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemcpyAsync(dsq, sq.data(), dim * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));  // sq is a local
   if (n <= 0) return LIGHTNE_OK;
   const int64_t blocks = (n * 64 + 255) / 256;
   hipLaunchKernelGGL(k_u_sigma_root, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, U, (const float*)dsq, n, rank,

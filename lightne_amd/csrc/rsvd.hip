@@ -270,6 +270,34 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
   return LIGHTNE_OK;
 }
 
+int lightne_dense_gram(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                       int32_t symmetric, double* G_host) {
+  if (!ctx || !X || !Y || !G_host || n < 0) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_gram: bad argument");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  return gram_f64(ctx, X, Y, n, k1, k2, symmetric, G_host);
+}
+
+int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, const double* W_host,
+                             int64_t k2, float* Out, int32_t normalize_rows) {
+  if (!ctx || !X || !W_host || !Out || n < 0)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_panel_gemm: bad argument");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  LN_CHECK(panel_times_small(ctx, X, n, k1, k1, W_host, k2, Out, normalize_rows, nullptr));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int lightne_dense_cholqr(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_host, int32_t* passes_out) {
+  if (!ctx || !Y || n < k || k <= 0 || k % 16 || k > 256)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_cholqr: bad argument");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  int passes = 0;
+  LN_CHECK(cholqr2(ctx, Y, n, k, R_host, &passes));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (passes_out) *passes_out = passes;
+  return LIGHTNE_OK;
+}
+
 int lightne_u_sigma_root(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
                          int32_t normalize, float* emb_out) {
   if (!ctx || !U || !emb_out || n <= 0 || dim <= 0 || dim > rank)

@@ -182,17 +182,22 @@ static int bits_for(uint64_t n) { int b = 0; while (((uint64_t)1 << b) < n) ++b;
  *   seed      : pbbs::random state handed to generate_trunc_log_matrix_v2
  *   cap_log2  : table capacity 2^cap_log2 (must exceed the distinct keys)
  *   nthreads  : OpenMP threads for the edge loop (1 = serial, deterministic weights)
+ *   u_begin, u_end, u_stride : sample only the edges of source vertices
+ *               u_begin, u_begin + u_stride, ... < u_end (u_end = 0: n)
  * pinned by: golden key->count tables from the reference-header harness;
  * T=1 closed form; zero-state fixed point; sum(cnt) == samples invariant. */
 orc_sample_result* orc_sample(uint64_t n, const uint64_t* off, const uint32_t* adj, uint64_t seed,
                               uint64_t walk_len, int sample, uint64_t exp_rv, double frac,
-                              double logn, int cap_log2, int nthreads) {
+                              double logn, int cap_log2, int nthreads, uint64_t u_begin, uint64_t u_end,
+                              uint64_t u_stride) {
   orc_table t;
   tab_init(&t, (uint64_t)1 << cap_log2);
   uint64_t samples = 0, attempts = 0, steps = 0;
   if (nthreads < 1) nthreads = 1;
+  if (u_end == 0 || u_end > n) u_end = n;
+  if (u_stride == 0) u_stride = 1;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(+ : samples, attempts, steps)
-  for (uint64_t u = 0; u < n; ++u) {
+  for (uint64_t u = u_begin; u < u_end; u += u_stride) {
     for (uint64_t j = off[u]; j < off[u + 1]; ++j) {
       uint32_t v = adj[j];
       uint64_t uv = (u << 32) | (uint64_t)v;

@@ -42,7 +42,7 @@ def load():
     sig = {
         "orc_hash64_2": (u64, [u64]),
         "orc_sample_scalars": (None, [u64, u64, u64, f32, f32, u64, P, P]),
-        "orc_sample": (P, [u64, P, P, u64, u64, C.c_int, u64, f64, f64, C.c_int, C.c_int]),
+        "orc_sample": (P, [u64, P, P, u64, u64, C.c_int, u64, f64, f64, C.c_int, C.c_int, u64, u64, u64]),
         "orc_sample_nkeys": (u64, [P]),
         "orc_sample_stats": (None, [P, P]),
         "orc_sample_copy": (None, [P, P, P, P]),
@@ -86,8 +86,10 @@ def sample_scalars(n: int, m: int, walk_len: int, sample_ratio: float, mem_ratio
 
 
 def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sample_ratio: float,
-           sample_mode: int = 0, threads: int = 0, cap_log2: int = 0):
-    """Per-directed-edge sampler + table (PathEmbed.h:128-188).
+           sample_mode: int = 0, threads: int = 0, cap_log2: int = 0, vertex_range=None):
+    """Per-directed-edge sampler + table (PathEmbed.h:128-188).  ``vertex_range``
+    = (u0, u1, stride) samples only the edges of source vertices u0, u0 + stride,
+    ... < u1.
 
     Returns (keys uint64 sorted, counts uint64, weights float64, stats dict)."""
     lib = load()
@@ -101,8 +103,9 @@ def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sam
         cap_log2 = max(10, int(np.ceil(np.log2(max(2 * m * per_edge, 1024)))))
     if threads <= 0:
         threads = int(lib.orc_max_threads())
+    u0, u1, us = vertex_range if vertex_range is not None else (0, 0, 1)
     h = lib.orc_sample(n, _p(off), _p(adj), seed & 0xFFFFFFFFFFFFFFFF, walk_len, sample_mode,
-                       sc["exp_rv_rounded"], sc["fractional_bits"], sc["logn"], cap_log2, threads)
+                       sc["exp_rv_rounded"], sc["fractional_bits"], sc["logn"], cap_log2, threads, u0, u1, us)
     try:
         nk = int(lib.orc_sample_nkeys(h))
         st = np.zeros(4, np.uint64)
