@@ -62,49 +62,72 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
 // 4(l/16) + v).
 // ---------------------------------------------------------------------------
 constexpr int GR_ROWS = 32;
-constexpr int GR_THREADS = 256;
-constexpr int GR_MAX_PAIRS_PER_WAVE = 24;
+constexpr int GR_THREADS = 512;
+constexpr int GR_WAVES = GR_THREADS / 64;
+constexpr int GR_MAX_PAIRS_PER_WAVE = 12;
+constexpr int GR_MAX_F4 = 9;  // float4 per thread for one slab: 32 rows x (256 + 256) / 4 / 512 = 8, +1
 
+// One block owns a contiguous row range; the 32-row slab of X (and Y) is
+// staged in LDS while the next slab is already in flight in registers.
 __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict__ X, const float* __restrict__ Y,
                                                          int64_t n, int k1, int k2, int npairs,
                                                          const int2* __restrict__ pairs, int64_t rows_per_block,
                                                          double* __restrict__ part) {
   extern __shared__ float s_mem[];
+  const bool same = X == Y;
   float* sX = s_mem;
-  float* sY = (X == Y) ? s_mem : s_mem + GR_ROWS * k1;
+  float* sY = same ? s_mem : s_mem + GR_ROWS * k1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nw = GR_THREADS / 64;
-  // pairs of this wave: wid, wid + nw, ...
+  const int li = lane & 15, lk = lane >> 4;
+  int2 pr[GR_MAX_PAIRS_PER_WAVE];
+#pragma unroll
+  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
+    const int pi = wid + q * GR_WAVES;
+    pr[q] = pi < npairs ? pairs[pi] : make_int2(0, 0);
+  }
   doublex4 acc[GR_MAX_PAIRS_PER_WAVE];
 #pragma unroll
   for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) acc[q] = doublex4{0.0, 0.0, 0.0, 0.0};
   const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r_end = min(n, r_begin + rows_per_block);
-  const int li = lane & 15, lk = lane >> 4;
+  const int nx4 = GR_ROWS * k1 / 4, ny4 = same ? 0 : GR_ROWS * k2 / 4;
+  const int row4x = k1 / 4, row4y = k2 / 4;
+  float4 buf[GR_MAX_F4];
+  auto fetch = [&](int64_t r0) {
+    const int nrow = (int)min<int64_t>(GR_ROWS, r_end - r0);
+    const float4* gx = reinterpret_cast<const float4*>(X + r0 * k1);
+    const float4* gy = reinterpret_cast<const float4*>(Y + r0 * k2);
+#pragma unroll
+    for (int t = 0; t < GR_MAX_F4; ++t) {
+      const int f = threadIdx.x + t * GR_THREADS;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (f < nx4) {
+        if (f / row4x < nrow) v = gx[f];
+      } else if (f < nx4 + ny4) {
+        const int g = f - nx4;
+        if (g / row4y < nrow) v = gy[g];
+      }
+      buf[t] = v;
+    }
+  };
+  if (r_begin < r_end) fetch(r_begin);
   for (int64_t r0 = r_begin; r0 < r_end; r0 += GR_ROWS) {
     __syncthreads();
-    const int nrow = (int)min<int64_t>(GR_ROWS, r_end - r0);
-    for (int e = threadIdx.x; e < GR_ROWS * k1; e += GR_THREADS) {
-      const int rr = e / k1;
-      sX[e] = (rr < nrow) ? X[(r0 + rr) * k1 + (e - rr * k1)] : 0.f;
-    }
-    if (X != Y) {
-      for (int e = threadIdx.x; e < GR_ROWS * k2; e += GR_THREADS) {
-        const int rr = e / k2;
-        sY[e] = (rr < nrow) ? Y[(r0 + rr) * k2 + (e - rr * k2)] : 0.f;
-      }
+#pragma unroll
+    for (int t = 0; t < GR_MAX_F4; ++t) {
+      const int f = threadIdx.x + t * GR_THREADS;
+      if (f < nx4 + ny4) reinterpret_cast<float4*>(s_mem)[f] = buf[t];
     }
     __syncthreads();
+    if (r0 + GR_ROWS < r_end) fetch(r0 + GR_ROWS);
     for (int s = 0; s < GR_ROWS; s += 4) {
       const float* xr = sX + (s + lk) * k1 + li;
       const float* yr = sY + (s + lk) * k2 + li;
 #pragma unroll
       for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
-        const int pi = wid + q * nw;
-        if (pi < npairs) {
-          const int2 pr = pairs[pi];
-          const double av = (double)xr[pr.x * 16];
-          const double bv = (double)yr[pr.y * 16];
+        if (wid + q * GR_WAVES < npairs) {
+          const double av = (double)xr[pr[q].x * 16];
+          const double bv = (double)yr[pr[q].y * 16];
           acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
         }
       }
@@ -113,11 +136,9 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict
   double* out = part + (int64_t)blockIdx.x * k1 * k2;
 #pragma unroll
   for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
-    const int pi = wid + q * nw;
-    if (pi < npairs) {
-      const int2 pr = pairs[pi];
+    if (wid + q * GR_WAVES < npairs) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) out[(int64_t)(pr.x * 16 + lk + 4 * v) * k2 + pr.y * 16 + li] = acc[q][v];
+      for (int v = 0; v < 4; ++v) out[(int64_t)(pr[q].x * 16 + lk + 4 * v) * k2 + pr[q].y * 16 + li] = acc[q][v];
     }
   }
 }
@@ -264,8 +285,8 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
     for (int b = 0; b < T2; ++b)
       if (!symmetric || a <= b) pairs.push_back(make_int2(a, b));
   const int npairs_all = (int)pairs.size();
-  const int per_launch = GR_MAX_PAIRS_PER_WAVE * (GR_THREADS / 64);
-  int nblk = std::max(1, ctx->num_cus * 2);
+  const int per_launch = GR_MAX_PAIRS_PER_WAVE * GR_WAVES;
+  int nblk = std::max(1, ctx->num_cus);
   int64_t rpb = (n + nblk - 1) / nblk;
   rpb = ((rpb + GR_ROWS - 1) / GR_ROWS) * GR_ROWS;
   if (rpb == 0) rpb = GR_ROWS;

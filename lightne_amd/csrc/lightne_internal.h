@@ -137,6 +137,10 @@ void tri_upper_inverse(const double* R, int k, double* Rinv);
 // one-sided Jacobi SVD of A (m x n row-major, m >= n): A = U diag(S) V^T,
 // singular values descending; U m x n, V n x n (either may be NULL).
 void jacobi_svd(const double* A, int m, int n, double* U, double* S, double* V);
+// symmetric eigendecomposition, eigenvalues descending, Z row-major n x n
+bool sym_eig(const double* A, int n, double* w, double* Z);
+// left singular vectors / values of a square matrix via eig(A A^T)
+bool svd_left_via_eig(const double* A, int n, double* U, double* S);
 // C = A (m x k) * B (k x n), row-major
 void matmul_host(const double* A, const double* B, int m, int k, int n, double* C);
 

@@ -2,15 +2,19 @@
 // randomized SVD that are too small for the device.
 //   * chol_upper / tri_upper_inverse: CholeskyQR2, replacing the Householder
 //     QR of LAPACKE_?geqrf + ?orgqr (LightNE/mklredsvd/util.cc:41-79).
-//   * jacobi_svd: one-sided (Hestenes) Jacobi SVD, replacing LAPACKE_?gesvd
-//     on C (mklredsvd.cc:352-377) and on the R factor of the tall panel of
-//     get_embedding_dense (SpectralPropagation.h:16-43).
-// Written from the textbook algorithms (Golub & Van Loan, 4th ed., 4.2.5 and
-// 8.6.3); singular vectors are determined up to sign.
+//   * sym_eig / svd_left_via_eig: symmetric eigensolver used for the k x k
+//     SVDs that replace LAPACKE_?gesvd on C (mklredsvd.cc:352-377) and on the
+//     R factor of the tall panel of get_embedding_dense
+//     (SpectralPropagation.h:16-43).
+//   * jacobi_svd: one-sided (Hestenes) Jacobi SVD, kept as an independent
+//     cross-check of sym_eig.
+// Written from the textbook algorithms (Golub & Van Loan, 4th ed., 4.2.5,
+// 8.3 and 8.6.3); singular vectors are determined up to sign.
 #include "lightne_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <numeric>
 #include <vector>
 
@@ -115,4 +119,142 @@ void jacobi_svd(const double* A, int m, int n, double* U, double* S, double* V) 
     if (V)
       for (int i = 0; i < n; ++i) V[(size_t)i * n + jj] = Vw[(size_t)j * n + i];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric eigendecomposition A = Z diag(w) Z^T (FP64), eigenvalues
+// descending.  Householder tridiagonalisation with accumulated reflectors,
+// then implicit symmetric QR steps with Wilkinson shifts on the tridiagonal
+// (Golub & Van Loan, 4th ed., Algorithms 8.3.1 and 8.3.3).  Z is returned
+// row-major (column j is the eigenvector of w[j]).
+// ---------------------------------------------------------------------------
+bool sym_eig(const double* A_in, int n, double* w, double* Z_out) {
+  // column-major working copies: A (symmetric) and Q (accumulated transforms)
+  std::vector<double> A(A_in, A_in + (size_t)n * n), Q((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) Q[(size_t)i * n + i] = 1.0;
+  std::vector<double> v(n), p(n), wv(n), sq(n);
+  auto at = [&](int r, int c) -> double& { return A[(size_t)c * n + r]; };
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;  // length of x = A[k+1:, k]
+    double alpha = 0.0;
+    for (int i = 0; i < m; ++i) alpha += at(k + 1 + i, k) * at(k + 1 + i, k);
+    alpha = std::sqrt(alpha);
+    if (alpha == 0.0) continue;
+    const double x0 = at(k + 1, k);
+    const double beta = x0 >= 0 ? -alpha : alpha;  // H x = beta e1
+    for (int i = 0; i < m; ++i) v[i] = at(k + 1 + i, k);
+    v[0] -= beta;
+    double vv = 0.0;
+    for (int i = 0; i < m; ++i) vv += v[i] * v[i];
+    if (vv == 0.0) continue;
+    const double tau = 2.0 / vv;
+    // p = tau * A22 v ; K = tau/2 v^T p ; w = p - K v ; A22 -= v w^T + w v^T
+    for (int i = 0; i < m; ++i) p[i] = 0.0;
+    for (int j = 0; j < m; ++j) {
+      const double vj = v[j];
+      const double* col = &A[(size_t)(k + 1 + j) * n + (k + 1)];
+      for (int i = 0; i < m; ++i) p[i] += col[i] * vj;
+    }
+    double K = 0.0;
+    for (int i = 0; i < m; ++i) {
+      p[i] *= tau;
+      K += v[i] * p[i];
+    }
+    K *= 0.5 * tau;
+    for (int i = 0; i < m; ++i) wv[i] = p[i] - K * v[i];
+    for (int j = 0; j < m; ++j) {
+      double* col = &A[(size_t)(k + 1 + j) * n + (k + 1)];
+      const double vj = v[j], wj = wv[j];
+      for (int i = 0; i < m; ++i) col[i] -= v[i] * wj + wv[i] * vj;
+    }
+    at(k + 1, k) = beta;
+    at(k, k + 1) = beta;
+    for (int i = 1; i < m; ++i) {
+      at(k + 1 + i, k) = 0.0;
+      at(k, k + 1 + i) = 0.0;
+    }
+    // Q <- Q H, H = I - tau v v^T acting on coordinates k+1..n-1:
+    // sq = tau * Q[:, k+1:] v, then Q[:, k+1+i] -= sq v_i (contiguous columns)
+    std::fill(sq.begin(), sq.end(), 0.0);
+    for (int i = 0; i < m; ++i) {
+      const double vi = v[i];
+      const double* qc = &Q[(size_t)(k + 1 + i) * n];
+      for (int r = 0; r < n; ++r) sq[r] += qc[r] * vi;
+    }
+    for (int r = 0; r < n; ++r) sq[r] *= tau;
+    for (int i = 0; i < m; ++i) {
+      const double vi = v[i];
+      double* qc = &Q[(size_t)(k + 1 + i) * n];
+      for (int r = 0; r < n; ++r) qc[r] -= sq[r] * vi;
+    }
+  }
+  std::vector<double> a(n), b(n, 0.0);
+  for (int i = 0; i < n; ++i) a[i] = at(i, i);
+  for (int i = 0; i + 1 < n; ++i) b[i] = at(i + 1, i);
+  // implicit QR with Wilkinson shift; Q columns are contiguous (column-major)
+  const double eps = std::numeric_limits<double>::epsilon();
+  int h = n - 1, iters = 0;
+  while (h > 0) {
+    if (std::fabs(b[h - 1]) <= eps * (std::fabs(a[h - 1]) + std::fabs(a[h]))) {
+      b[h - 1] = 0.0;
+      --h;
+      continue;
+    }
+    int l = h - 1;
+    while (l > 0 && std::fabs(b[l - 1]) > eps * (std::fabs(a[l - 1]) + std::fabs(a[l]))) --l;
+    if (++iters > 60 * n) return false;
+    const double d = 0.5 * (a[h - 1] - a[h]);
+    const double bb = b[h - 1] * b[h - 1];
+    const double mu = a[h] - bb / (d + (d >= 0 ? 1.0 : -1.0) * std::hypot(d, b[h - 1]));
+    double x = a[l] - mu, z = b[l];
+    for (int k = l; k < h; ++k) {
+      const double r = std::hypot(x, z);
+      const double c = r == 0.0 ? 1.0 : x / r, s = r == 0.0 ? 0.0 : z / r;
+      if (k > l) b[k - 1] = r;  // the bulge is annihilated into b[k-1]
+      const double ak = a[k], ak1 = a[k + 1], bk = b[k];
+      a[k] = c * c * ak + 2.0 * c * s * bk + s * s * ak1;
+      a[k + 1] = s * s * ak - 2.0 * c * s * bk + c * c * ak1;
+      b[k] = c * s * (ak1 - ak) + (c * c - s * s) * bk;
+      if (k + 1 < h) {
+        x = b[k];
+        z = s * b[k + 1];  // new bulge at (k, k+2)
+        b[k + 1] *= c;
+      }
+      double* qk = &Q[(size_t)k * n];
+      double* qk1 = &Q[(size_t)(k + 1) * n];
+      for (int i = 0; i < n; ++i) {
+        const double u = qk[i], t = qk1[i];
+        qk[i] = c * u + s * t;
+        qk1[i] = -s * u + c * t;
+      }
+    }
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int i, int j) { return a[i] > a[j]; });
+  for (int jj = 0; jj < n; ++jj) {
+    const int j = ord[jj];
+    w[jj] = a[j];
+    for (int i = 0; i < n; ++i) Z_out[(size_t)i * n + jj] = Q[(size_t)j * n + i];
+  }
+  return true;
+}
+
+// Left singular vectors and singular values of a square A (n x n) through the
+// eigendecomposition of A A^T: A A^T = U diag(S^2) U^T.  Relative accuracy of
+// S_i is about eps (S_0/S_i)^2, ample for the FP32 panels it serves.
+bool svd_left_via_eig(const double* A, int n, double* U, double* S) {
+  std::vector<double> G((size_t)n * n, 0.0), w(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) {
+      double s = 0.0;
+      const double* ai = A + (size_t)i * n;
+      const double* aj = A + (size_t)j * n;
+      for (int k = 0; k < n; ++k) s += ai[k] * aj[k];
+      G[(size_t)i * n + j] = s;
+      G[(size_t)j * n + i] = s;
+    }
+  if (!sym_eig(G.data(), n, w.data(), U)) return false;
+  for (int j = 0; j < n; ++j) S[j] = std::sqrt(std::max(w[j], 0.0));
+  return true;
 }

@@ -143,28 +143,53 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// Stable scatter of one 8-bit digit.  Every tile ranks its elements in LDS
+// (wave peers by 8 ballots, waves in order), stages them in local digit order
+// and then writes each digit run to its global offset with consecutive lanes
+// on consecutive addresses.
 template <typename V>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                            const V* __restrict__ vin, uint64_t* __restrict__ kout,
                                                            V* __restrict__ vout, int64_t n, int shift,
-                                                           const uint64_t* __restrict__ offs, int64_t ntiles) {
+                                                           const uint64_t* __restrict__ offs,
+                                                           const uint32_t* __restrict__ hist, int64_t ntiles) {
+  __shared__ uint64_t s_keys[RS_TILE];
+  __shared__ V s_vals[RS_TILE];
   __shared__ uint64_t s_base[256];
+  __shared__ uint32_t s_dstart[256];
   __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_wcnt[RS_WAVES][256];
   __shared__ uint32_t s_woff[RS_WAVES][256];
+  __shared__ uint32_t s_scan[RS_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // tile-local start of every digit: exclusive scan of this tile's histogram
+  {
+    const uint32_t c = hist[(int64_t)tid * ntiles + blockIdx.x];
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_scan[wid] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int w = 0; w < wid; ++w) off += s_scan[w];
+    s_dstart[tid] = off + incl - c;
+  }
   s_base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
   s_run[tid] = 0;
 #pragma unroll
   for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const int nvalid = (int)min<int64_t>(RS_TILE, n - base);
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const int64_t i = base + (int64_t)it * RS_THREADS + tid;
-    const bool valid = i < n;
-    uint64_t k = valid ? kin[i] : 0;
-    V v = valid ? vin[i] : V(0);
+    const int li = it * RS_THREADS + tid;
+    const bool valid = li < nvalid;
+    const uint64_t k = valid ? kin[base + li] : 0;
+    const V v = valid ? vin[base + li] : V(0);
     const uint32_t d = (uint32_t)((k >> shift) & 255u);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -188,10 +213,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     }
     __syncthreads();
     if (valid) {
-      const uint64_t pos = s_base[d] + s_woff[wid][d] + rank;
-      kout[pos] = k;
-      vout[pos] = v;
+      const uint32_t lp = s_dstart[d] + s_woff[wid][d] + rank;
+      s_keys[lp] = k;
+      s_vals[lp] = v;
     }
+  }
+  __syncthreads();
+  for (int li = tid; li < nvalid; li += RS_THREADS) {
+    const uint64_t k = s_keys[li];
+    const uint32_t d = (uint32_t)((k >> shift) & 255u);
+    const uint64_t pos = s_base[d] + (uint64_t)(li - (int)s_dstart[d]);
+    kout[pos] = k;
+    vout[pos] = s_vals[li];
   }
 }
 
@@ -212,7 +245,7 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
     LN_LAUNCH(ctx);
     LN_CHECK(scan_u32_to_u64(ctx, hist, offs, 256 * ntiles));
     hipLaunchKernelGGL(k_rs_scatter<V>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, va, kb, vb, n,
-                       shift, offs, ntiles);
+                       shift, (const uint64_t*)offs, (const uint32_t*)hist, ntiles);
     LN_LAUNCH(ctx);
     std::swap(ka, kb);
     std::swap(va, vb);

@@ -5,7 +5,7 @@
 // The Chebyshev recurrence of order K (:94-208) runs as 2 gathers per step with
 // the AXPYs fused into the second gather's epilogue; y = a - conv is written by
 // the last step, z = (A + I) y by one more gather (:210-221), and the dense
-// tall SVD (:16-43) is CholeskyQR2 + a host Jacobi SVD of the R factor.
+// tall SVD (:16-43) is CholeskyQR2 + a host FP64 SVD of the R factor.
 #include "lightne_internal.h"
 #include "spmm_epi.h"
 
@@ -191,7 +191,8 @@ int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank,
   std::vector<double> R(rank * rank), Ur(rank * rank), Sr(rank);
   int passes = 0;
   LN_CHECK(cholqr2(ctx, Q, n, rank, R.data(), &passes));
-  jacobi_svd(R.data(), (int)rank, (int)rank, Ur.data(), Sr.data(), nullptr);
+  if (!svd_left_via_eig(R.data(), (int)rank, Ur.data(), Sr.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "embed_dense: small SVD did not converge");
   // emb = rownorm(Q U_R[:, :dim] diag(sqrt(S)))  (compute_u_sigma_root, normalize = true)
   std::vector<double> W(rank * dim);
   for (int64_t i = 0; i < rank; ++i)

@@ -11,7 +11,7 @@
 //   P ~ N(0,1) k x k; Z = B P        (:289-317)
 //   Z = orth(Z)                      (:321)
 //   C = Z^T B                        (:327-350)   FP64 Gram kernel
-//   C = U_C S V^T                    (:352-377)   host FP64 Jacobi
+//   C = U_C S V^T                    (:352-377)   host FP64 eig(C C^T)
 //   matU = Z U_C                     (:379-400)
 // In the BF16 form the gathered SpMM panels (Omega, Q) are BF16 with FP32
 // accumulation; every other panel stays FP32 and the Grams FP64.
@@ -251,7 +251,8 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
   LN_HIP(ctx, hipEventRecord(e[10], s));
   LN_HIP(ctx, hipEventSynchronize(e[10]));
   auto t0 = std::chrono::steady_clock::now();
-  jacobi_svd(C.data(), (int)k, (int)k, Uc.data(), Sc.data(), nullptr);
+  if (!svd_left_via_eig(C.data(), (int)k, Uc.data(), Sc.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "lightne_rsvd: small SVD did not converge");
   auto t1 = std::chrono::steady_clock::now();
   st.t_small_svd = std::chrono::duration<double>(t1 - t0).count();
   LN_HIP(ctx, hipEventRecord(e[11], s));

@@ -11,9 +11,10 @@
 //                    linear probing, one 64-bit CAS to claim a slot.
 //  k_table_reduce  : distinct keys and total weight (PathEmbed.h:175-188),
 //                    deterministic two-level reduction.
-//  k_trunc_emit    : trunc-log (PathEmbed.h:199-218) fused with the symmetric
-//                    expansion (PathEmbed.h:278-311): kept keys emit (row,col)
-//                    sort keys and float values.
+//  k_trunc_count / k_trunc_write : trunc-log (PathEmbed.h:199-218) fused with
+//                    the symmetric expansion (PathEmbed.h:278-311): kept keys emit
+//                    (row, col) sort keys and float values; per-block counts, a
+//                    scan, then writes in slot order (no contended atomics).
 //  radix sort + k_csr_finish : rows sorted by column, row_ptr by boundary
 //                    detection (no atomics, deterministic).
 #include "lightne_internal.h"
@@ -65,10 +66,27 @@ __global__ void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m
   src[j] = (uint32_t)lo;
 }
 
-__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint64_t v) {
+// block-wide sum of three counters, one atomic per counter per block
+__device__ __forceinline__ void block_add3(unsigned long long* dst, uint64_t a, uint64_t b, uint64_t c) {
+  __shared__ uint64_t s_r[3][4];
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+  for (int d = 32; d >= 1; d >>= 1) {
+    a += __shfl_xor(a, d, 64);
+    b += __shfl_xor(b, d, 64);
+    c += __shfl_xor(c, d, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_r[0][wid] = a;
+    s_r[1][wid] = b;
+    s_r[2][wid] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    uint64_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_r[threadIdx.x][w];
+    if (t) atomicAdd(&dst[threadIdx.x], (unsigned long long)t);
+  }
 }
 
 struct SampleArgs {
@@ -169,9 +187,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       ++n_samples;
     }
   }
-  wave_add_u64(&a.stats[0], n_samples);
-  wave_add_u64(&a.stats[1], n_steps);
-  wave_add_u64(&a.stats[2], n_attempts);
+  block_add3(a.stats, n_samples, n_steps, n_attempts);
   if (overflow) atomicOr(&a.stats[3], 1ull);
   if (bad) atomicOr(&a.stats[4], 1ull);
 }
@@ -252,55 +268,99 @@ struct TruncArgs {
   int colbits;
   uint64_t* skey;
   float* sval;
-  unsigned long long* counter;  // emitted entries
-  unsigned long long* kept;     // kept upper keys
 };
 
-// PathEmbed.h:203-217 per key, then the CSR expansion of :278-311.
-__global__ __launch_bounds__(256) void k_trunc_emit(TruncArgs a) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int emit = 0;
-  uint32_t u = 0, v = 0;
-  float lw = 0.f;
-  if (i < a.cap) {
-    const Slot s = a.tab[i];
-    if (s.key != EMPTY_KEY) {
-      u = (uint32_t)(s.key >> 32);
-      v = (uint32_t)s.key;
-      const double du = (double)(a.off[u + 1] - a.off[u]);
-      const double dv = (double)(a.off[v + 1] - a.off[v]);
-      const double val = a.wtab ? a.wtab[i] : (double)s.cnt;
-      const double wgh = (u != v) ? val : val * 2;
-      const double adj = wgh * a.factor / du / dv;
-      // log(adj) > 0  <=>  adj > 1 for any faithful log
-      if (adj > 1.0) {
-        lw = (float)log(adj);
-        emit = (a.upper || u == v) ? 1 : 2;
+// PathEmbed.h:203-217 per slot: returns the number of CSR entries it emits
+// (0, 1 or 2) and the row / column / float value.
+__device__ __forceinline__ int trunc_slot(const TruncArgs& a, uint64_t i, uint32_t& u, uint32_t& v, double& adj) {
+  const Slot s = a.tab[i];
+  if (s.key == EMPTY_KEY) return 0;
+  u = (uint32_t)(s.key >> 32);
+  v = (uint32_t)s.key;
+  const double du = (double)(a.off[u + 1] - a.off[u]);
+  const double dv = (double)(a.off[v + 1] - a.off[v]);
+  const double val = a.wtab ? a.wtab[i] : (double)s.cnt;
+  const double wgh = (u != v) ? val : val * 2;
+  adj = wgh * a.factor / du / dv;
+  // log(adj) > 0  <=>  adj > 1 for any faithful log
+  if (!(adj > 1.0)) return 0;
+  return (a.upper || u == v) ? 1 : 2;
+}
+
+constexpr int TR_THREADS = 256;
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_w[wid] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int w = 0; w < TR_THREADS / 64; ++w) t += s_w[w];
+  return t;
+}
+
+// pass 1: per-block (contiguous slot range) emitted entries and kept keys
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_t per_block,
+                                                            uint32_t* __restrict__ blk_emit,
+                                                            uint32_t* __restrict__ blk_kept) {
+  __shared__ uint32_t s_w[TR_THREADS / 64];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.cap, b0 + per_block);
+  uint32_t e = 0, kk = 0;
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += TR_THREADS) {
+    uint32_t u, v;
+    double adj;
+    const int c = trunc_slot(a, i, u, v, adj);
+    e += c;
+    kk += c > 0;
+  }
+  const uint32_t te = block_sum_u32(e, s_w);
+  const uint32_t tk = block_sum_u32(kk, s_w);
+  if (threadIdx.x == 0) {
+    blk_emit[blockIdx.x] = te;
+    blk_kept[blockIdx.x] = tk;
+  }
+}
+
+// pass 2: same partition, entries written in slot order at the block's offset
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_t per_block,
+                                                            const uint64_t* __restrict__ blk_off) {
+  __shared__ uint32_t s_w[TR_THREADS / 64];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.cap, b0 + per_block);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint64_t run = blk_off[blockIdx.x];
+  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS) {
+    const uint64_t i = t0 + threadIdx.x;
+    uint32_t u = 0, v = 0;
+    double adj = 0.0;
+    const int c = (i < b1) ? trunc_slot(a, i, u, v, adj) : 0;
+    // block-wide exclusive scan of c
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    __syncthreads();
+    if (lane == 63) s_w[wid] = (uint32_t)incl;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int w = 0; w < TR_THREADS / 64; ++w) {
+      if (w < wid) wbase += s_w[w];
+      tot += s_w[w];
+    }
+    if (c) {
+      const float lw = (float)log(adj);
+      const uint64_t pos = run + wbase + (uint32_t)(incl - c);
+      a.skey[pos] = ((uint64_t)u << a.colbits) | v;
+      a.sval[pos] = lw;
+      if (c == 2) {
+        a.skey[pos + 1] = ((uint64_t)v << a.colbits) | u;
+        a.sval[pos + 1] = lw;
       }
     }
-  }
-  const int lane = threadIdx.x & 63;
-  // wave-aggregated allocation
-  int incl = emit;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += t;
-  }
-  const int tot = __shfl(incl, 63, 64);
-  const uint64_t kbal = __ballot(emit > 0);
-  unsigned long long base = 0;
-  if (lane == 63 && tot) base = atomicAdd(a.counter, (unsigned long long)tot);
-  if (lane == 63 && kbal) atomicAdd(a.kept, (unsigned long long)__popcll(kbal));
-  base = __shfl(base, 63, 64);
-  if (emit) {
-    uint64_t pos = base + (uint64_t)(incl - emit);
-    a.skey[pos] = ((uint64_t)u << a.colbits) | v;
-    a.sval[pos] = lw;
-    if (emit == 2) {
-      a.skey[pos + 1] = ((uint64_t)v << a.colbits) | u;
-      a.sval[pos + 1] = lw;
-    }
+    run += tot;
   }
 }
 
@@ -560,13 +620,6 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
   st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
   LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
 
-  const uint64_t max_emit = std::max<uint64_t>(2 * distinct, 1);
-  DevMem mk, mv;
-  LN_CHECK(dev_alloc(ctx, mk, max_emit * 8, "sparsifier sort keys"));
-  LN_CHECK(dev_alloc(ctx, mv, max_emit * 4, "sparsifier values"));
-  auto* ctr = (unsigned long long*)ctx_buf(ctx, "emit_ctr", 16);
-  if (!ctr) return LIGHTNE_ERR_OOM;
-  LN_HIP(ctx, hipMemsetAsync(ctr, 0, 16, s));
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
   TruncArgs ta;
   ta.tab = tab;
@@ -576,18 +629,33 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
   ta.factor = st.factor;
   ta.upper = p->upper;
   ta.colbits = colbits;
+  // contiguous slot ranges, one per block; counts first, then a scan, then the
+  // entries in slot order (deterministic, no contended atomics)
+  const int nblk = 4096;
+  const uint64_t per_block = (cap + nblk - 1) / nblk;
+  auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
+  auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
+  auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
+  if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
+  LN_LAUNCH(ctx);
+  LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+  std::vector<uint32_t> hk(nblk);
+  uint64_t htot = 0;
+  LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  const int64_t nnz = (int64_t)htot;
+  st.kept_upper = 0;
+  for (uint32_t v : hk) st.kept_upper += v;
+  st.nnz = (uint64_t)nnz;
+  DevMem mk, mv;
+  LN_CHECK(dev_alloc(ctx, mk, std::max<int64_t>(nnz, 1) * 8, "sparsifier sort keys"));
+  LN_CHECK(dev_alloc(ctx, mv, std::max<int64_t>(nnz, 1) * 4, "sparsifier values"));
   ta.skey = (uint64_t*)mk.p;
   ta.sval = (float*)mv.p;
-  ta.counter = ctr;
-  ta.kept = ctr + 1;
-  hipLaunchKernelGGL(k_trunc_emit, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, ta);
+  hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
   LN_LAUNCH(ctx);
-  unsigned long long hc[2];
-  LN_HIP(ctx, hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  const int64_t nnz = (int64_t)hc[0];
-  st.kept_upper = hc[1];
-  st.nnz = (uint64_t)nnz;
   LN_CHECK(radix_sort_pairs(ctx, (uint64_t*)mk.p, (float*)mv.p, nnz, 2 * colbits));
 
   auto* csr = new lightne_csr();

@@ -5,9 +5,9 @@
 // SpectralPropagation.h:210-221.
 //
 // Layout: X is n x k row-major (k % 16 == 0), FP32 or BF16.  A lane owns 16
-// consecutive columns of one row, so a group of G = k/16 lanes reads a whole
-// X row (576 B at k=144 FP32, 288 B BF16) as G x 64 B (or 32 B) contiguous
-// pieces; a 64-wide wave holds NG = 64/G groups.
+// columns of one row, so a group of G = k/16 lanes reads a whole X row (576 B
+// at k=144 FP32, 288 B BF16); every load instruction of a group covers G
+// consecutive 16-byte pieces.  A 64-wide wave holds NG = 64/G groups.
 //
 // Rows are split by length (power-law graphs):
 //   * short rows (nnz <= SHORT): one group per row, no cross-lane reduction;
@@ -28,41 +28,63 @@ namespace {
 constexpr int SHORT_NNZ = 48;
 constexpr int CHUNK_NNZ = 512;
 
+// Lane column layout.  A group of G = k/16 lanes covers one k-wide row; lane
+// `sub` owns 16 columns as chunks of CW consecutive columns (CW = 4 floats or
+// 8 bf16, i.e. 16 bytes), chunk q at column CW*sub + CW*G*q, so each load
+// instruction of a group reads G consecutive 16-byte pieces.
 template <typename XT>
-__device__ __forceinline__ void load16(const XT* __restrict__ p, float* v);
+struct Lay;
+template <>
+struct Lay<float> {
+  static constexpr int CW = 4;
+};
+template <>
+struct Lay<__hip_bfloat16> {
+  static constexpr int CW = 8;
+};
+
+// column of accumulator quad i4 (accumulators 4*i4 .. 4*i4+3)
+template <typename XT>
+__device__ __forceinline__ int quad_col(int i4, int sub, int G) {
+  constexpr int CW = Lay<XT>::CW;
+  return CW * sub + CW * G * ((4 * i4) / CW) + ((4 * i4) % CW);
+}
+
+template <typename XT>
+__device__ __forceinline__ void load16(const XT* __restrict__ row, int sub, int G, float* v);
 
 template <>
-__device__ __forceinline__ void load16<float>(const float* __restrict__ p, float* v) {
-  const float4* q = reinterpret_cast<const float4*>(p);
+__device__ __forceinline__ void load16<float>(const float* __restrict__ row, int sub, int G, float* v) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float4 t = q[i];
-    v[4 * i + 0] = t.x;
-    v[4 * i + 1] = t.y;
-    v[4 * i + 2] = t.z;
-    v[4 * i + 3] = t.w;
+  for (int q = 0; q < 4; ++q) {
+    const float4 t = *reinterpret_cast<const float4*>(row + 4 * sub + 4 * G * q);
+    v[4 * q + 0] = t.x;
+    v[4 * q + 1] = t.y;
+    v[4 * q + 2] = t.z;
+    v[4 * q + 3] = t.w;
   }
 }
 
 template <>
-__device__ __forceinline__ void load16<__hip_bfloat16>(const __hip_bfloat16* __restrict__ p, float* v) {
-  const uint4* q = reinterpret_cast<const uint4*>(p);
+__device__ __forceinline__ void load16<__hip_bfloat16>(const __hip_bfloat16* __restrict__ row, int sub, int G,
+                                                       float* v) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    uint4 t = q[i];
-    uint32_t w[4] = {t.x, t.y, t.z, t.w};
+  for (int q = 0; q < 2; ++q) {
+    const uint4 t = *reinterpret_cast<const uint4*>(row + 8 * sub + 8 * G * q);
+    const uint32_t w[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      v[8 * i + 2 * j + 0] = __uint_as_float(w[j] << 16);
-      v[8 * i + 2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
+      v[8 * q + 2 * j + 0] = __uint_as_float(w[j] << 16);
+      v[8 * q + 2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
     }
   }
 }
 
 template <typename XT, bool IMPL>
-__device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_t j1, int64_t stride, int colbase,
+__device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_t j1, int64_t stride, int sub,
                                            float* acc) {
   const XT* __restrict__ X = reinterpret_cast<const XT*>(a.X);
+  const int G = a.G;
   int64_t j = j0;
   for (; j + stride < j1; j += 2 * stride) {
     int64_t c0, c1;
@@ -77,8 +99,8 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
       w1 = a.val[j + stride];
     }
     float x0[16], x1[16];
-    load16<XT>(X + c0 * a.k + colbase, x0);
-    load16<XT>(X + c1 * a.k + colbase, x1);
+    load16<XT>(X + c0 * a.k, sub, G, x0);
+    load16<XT>(X + c1 * a.k, sub, G, x1);
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i] + w1 * x1[i];
   }
@@ -92,7 +114,7 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
       w0 = a.val[j];
     }
     float x0[16];
-    load16<XT>(X + c0 * a.k + colbase, x0);
+    load16<XT>(X + c0 * a.k, sub, G, x0);
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i];
   }
@@ -113,11 +135,11 @@ __global__ __launch_bounds__(256) void k_spmm_short(SpmmArgs a) {
   float acc[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-  gather_row<XT, IMPL>(a, j0, j1, 1, sub * 16, acc);
+  gather_row<XT, IMPL>(a, j0, j1, 1, sub, acc);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     float4 v = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
-    epilogue<XT, EPI, IMPL>(a, r, sub * 16 + 4 * q, v);
+    epilogue<XT, EPI, IMPL>(a, r, quad_col<XT>(q, sub, G), v);
   }
 }
 
@@ -139,11 +161,13 @@ __global__ __launch_bounds__(256) void k_spmm_long(SpmmArgs a) {
   if (active) {
     row = a.chunk_row[c];
     slot = a.chunk_slot[c];
-    if (g < NG) gather_row<XT, IMPL>(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub * 16, acc);
+    if (g < NG) gather_row<XT, IMPL>(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub, acc);
   }
   if (active && g < NG) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) red[g * a.k + sub * 16 + i] = acc[i];
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<float4*>(red + g * a.k + quad_col<XT>(q, sub, G)) =
+          make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
   }
   __syncthreads();
   if (!active) return;
@@ -229,56 +253,93 @@ void spmm_plan_free(SpmmPlan& P) {
   P = SpmmPlan{};
 }
 
+namespace {
+
+// per row: chunks (0 for short rows), multi-chunk flag, partial slots
+__global__ void k_plan_count(const int64_t* __restrict__ rp, int64_t n, int short_nnz, int chunk_nnz,
+                             uint32_t* __restrict__ nch, uint32_t* __restrict__ multi, uint32_t* __restrict__ parts) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t len = rp[r + 1] - rp[r];
+  const uint32_t nc = len <= short_nnz ? 0u : (uint32_t)((len + chunk_nnz - 1) / chunk_nnz);
+  nch[r] = nc;
+  multi[r] = nc > 1 ? 1u : 0u;
+  parts[r] = nc > 1 ? nc : 0u;
+}
+
+__global__ void k_plan_fill(const int64_t* __restrict__ rp, int64_t n, int chunk_nnz, const uint64_t* __restrict__ coff,
+                            const uint64_t* __restrict__ moff, const uint64_t* __restrict__ poff,
+                            int64_t* __restrict__ chunk_se, int32_t* __restrict__ chunk_row,
+                            int32_t* __restrict__ chunk_slot, int32_t* __restrict__ multi_row,
+                            int64_t* __restrict__ multi_first) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) multi_first[moff[n]] = (int64_t)poff[n];
+  if (r >= n) return;
+  const uint64_t c0 = coff[r], nc = coff[r + 1] - c0;
+  if (nc == 0) return;
+  const int64_t b = rp[r], e = rp[r + 1];
+  for (uint64_t q = 0; q < nc; ++q) {
+    const uint64_t c = c0 + q;
+    chunk_se[2 * c] = b + (int64_t)q * chunk_nnz;
+    chunk_se[2 * c + 1] = min(e, b + (int64_t)(q + 1) * chunk_nnz);
+    chunk_row[c] = (int32_t)r;
+    chunk_slot[c] = nc > 1 ? (int32_t)(poff[r] + q) : -1;
+  }
+  if (nc > 1) {
+    multi_row[moff[r]] = (int32_t)r;
+    multi_first[moff[r]] = (int64_t)poff[r];
+  }
+}
+
+}  // namespace
+
+// Work split for one CSR, built on the device: rows longer than SHORT_NNZ are
+// cut into CHUNK_NNZ chunks; rows with several chunks get partial slots.
 int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P) {
   spmm_plan_free(P);
   P.n = n;
   P.short_nnz = SHORT_NNZ;
   P.chunk_nnz = CHUNK_NNZ;
-  std::vector<int64_t> rp(n + 1);
-  LN_HIP(ctx, hipMemcpyAsync(rp.data(), rp_dev, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<int64_t> cs;
-  std::vector<int32_t> crow, cslot, mrow;
-  std::vector<int64_t> mfirst;
-  int32_t nparts = 0;
-  for (int64_t r = 0; r < n; ++r) {
-    const int64_t len = rp[r + 1] - rp[r];
-    if (len <= SHORT_NNZ) continue;
-    const int64_t nc = (len + CHUNK_NNZ - 1) / CHUNK_NNZ;
-    if (nc > 1) {
-      mrow.push_back((int32_t)r);
-      mfirst.push_back(nparts);
-    }
-    for (int64_t q = 0; q < nc; ++q) {
-      cs.push_back(rp[r] + q * CHUNK_NNZ);
-      crow.push_back((int32_t)r);
-      cslot.push_back(nc > 1 ? nparts++ : -1);
-    }
-  }
-  P.nchunks = (int64_t)crow.size();
-  P.nmulti = (int64_t)mrow.size();
-  P.nparts = nparts;
-  mfirst.push_back(nparts);
-  // (start, end) nnz range of every chunk
-  std::vector<int64_t> se(2 * P.nchunks + 2, 0);
-  for (int64_t c = 0; c < P.nchunks; ++c) {
-    se[2 * c] = cs[c];
-    se[2 * c + 1] = std::min<int64_t>(cs[c] + CHUNK_NNZ, rp[crow[c] + 1]);
-  }
-  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+  if (n >= ((int64_t)1 << 31)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: more than 2^31 rows");
+  hipStream_t s = ctx->stream;
+  auto* nch = (uint32_t*)ctx_buf(ctx, "plan_nch", std::max<int64_t>(n, 1) * 4);
+  auto* mul = (uint32_t*)ctx_buf(ctx, "plan_mul", std::max<int64_t>(n, 1) * 4);
+  auto* par = (uint32_t*)ctx_buf(ctx, "plan_par", std::max<int64_t>(n, 1) * 4);
+  auto* coff = (uint64_t*)ctx_buf(ctx, "plan_coff", (n + 1) * 8);
+  auto* moff = (uint64_t*)ctx_buf(ctx, "plan_moff", (n + 1) * 8);
+  auto* poff = (uint64_t*)ctx_buf(ctx, "plan_poff", (n + 1) * 8);
+  if (!nch || !mul || !par || !coff || !moff || !poff) return LIGHTNE_ERR_OOM;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(256), 0, s, rp_dev, n, SHORT_NNZ, CHUNK_NNZ, nch, mul, par);
+  LN_LAUNCH(ctx);
+  LN_CHECK(scan_u32_to_u64(ctx, nch, coff, n));
+  LN_CHECK(scan_u32_to_u64(ctx, mul, moff, n));
+  LN_CHECK(scan_u32_to_u64(ctx, par, poff, n));
+  uint64_t tot[3];
+  LN_HIP(ctx, hipMemcpyAsync(&tot[0], coff + n, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipMemcpyAsync(&tot[1], moff + n, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipMemcpyAsync(&tot[2], poff + n, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  P.nchunks = (int64_t)tot[0];
+  P.nmulti = (int64_t)tot[1];
+  P.nparts = (int64_t)tot[2];
+  if (P.nparts >= ((int64_t)1 << 31)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: too many partial chunks");
+  auto al = [&](void** dst, size_t bytes) -> int {
     if (hipMalloc(dst, bytes ? bytes : 16) != hipSuccess) {
       (void)hipGetLastError();
       return ctx_fail(ctx, LIGHTNE_ERR_OOM, "spmm plan allocation failed");
     }
-    if (bytes) LN_HIP(ctx, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     return LIGHTNE_OK;
   };
-  LN_CHECK(up((void**)&P.chunk_start, se.data(), se.size() * sizeof(int64_t)));
-  LN_CHECK(up((void**)&P.chunk_row, crow.data(), crow.size() * sizeof(int32_t)));
-  LN_CHECK(up((void**)&P.chunk_slot, cslot.data(), cslot.size() * sizeof(int32_t)));
-  LN_CHECK(up((void**)&P.multi_row, mrow.data(), mrow.size() * sizeof(int32_t)));
-  LN_CHECK(up((void**)&P.multi_first, mfirst.data(), mfirst.size() * sizeof(int64_t)));
-  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  LN_CHECK(al((void**)&P.chunk_start, (2 * P.nchunks + 2) * sizeof(int64_t)));
+  LN_CHECK(al((void**)&P.chunk_row, P.nchunks * sizeof(int32_t)));
+  LN_CHECK(al((void**)&P.chunk_slot, P.nchunks * sizeof(int32_t)));
+  LN_CHECK(al((void**)&P.multi_row, P.nmulti * sizeof(int32_t)));
+  LN_CHECK(al((void**)&P.multi_first, (P.nmulti + 1) * sizeof(int64_t)));
+  hipLaunchKernelGGL(k_plan_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, CHUNK_NNZ, (const uint64_t*)coff,
+                     (const uint64_t*)moff, (const uint64_t*)poff, P.chunk_start, P.chunk_row, P.chunk_slot,
+                     P.multi_row, P.multi_first);
+  LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
 
