@@ -118,3 +118,17 @@ def test_sample_scalars_match_oracle(n, m, T, ratio, mem, tsz):
 def test_bessel_coefficients_match_oracle():
     for k in range(10):
         assert L.bessel_i(k, 0.5) == O.bessel_iv(k, 0.5)
+
+
+CLI = os.path.join(H.ROOT, "lightne_amd", "lightne")
+
+
+def test_cli_usage_and_errors(tmp_path):
+    out = subprocess.run([CLI], capture_output=True, text=True)
+    assert out.returncode == 2 and "usage" in out.stdout
+    out = subprocess.run([CLI, "-rounds", "1", "-s", str(tmp_path / "missing.adj")], capture_output=True, text=True)
+    assert out.returncode == 1 and "cannot open" in out.stderr
+    out = subprocess.run([CLI, "-c", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
+    assert out.returncode == 2 and "compressed" in out.stderr
+    out = subprocess.run([CLI, "-ne_method", "nope", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
+    assert out.returncode == 2
