@@ -7,13 +7,14 @@ stage API (see ``lightne_amd.api``).
 """
 from ._lib import BF16, FP32, LightNEError, LIB_PATH  # noqa: F401
 from .api import (CSR, Context, DeviceGraph, HostGraph, MKLRedSVD, NetSMF, SparseTable, bessel_i,  # noqa: F401
-                  chebyshev_expansion, dense_cholqr, dense_gram, dense_panel_gemm, chung_lu, compute_u_sigma_root, context, generate_trunc_log_matrix,
-                  get_embedding_dense, propagate, read_unweighted_symmetric_graph, rmat, sample_scalars,
-                  sparsify_counts, write_adjacency_graph)
+                  chebyshev_expansion, chung_lu, compute_u_sigma_root, context, dense_cholqr, dense_gram,
+                  dense_panel_gemm, generate_trunc_log_matrix, get_embedding_dense, propagate, propagate_backward,
+                  read_unweighted_symmetric_graph, rmat, sample_scalars, sparsify_counts, write_adjacency_graph)
 
 __all__ = [
     "BF16", "FP32", "LightNEError", "CSR", "Context", "DeviceGraph", "HostGraph", "MKLRedSVD", "NetSMF",
-    "SparseTable", "bessel_i", "chebyshev_expansion", "dense_cholqr", "dense_gram", "dense_panel_gemm", "chung_lu", "compute_u_sigma_root", "context",
-    "generate_trunc_log_matrix", "get_embedding_dense", "propagate", "read_unweighted_symmetric_graph", "rmat",
-    "sample_scalars", "sparsify_counts", "write_adjacency_graph",
+    "SparseTable", "bessel_i", "chebyshev_expansion", "chung_lu", "compute_u_sigma_root", "context",
+    "dense_cholqr", "dense_gram", "dense_panel_gemm", "generate_trunc_log_matrix", "get_embedding_dense",
+    "propagate", "propagate_backward", "read_unweighted_symmetric_graph", "rmat", "sample_scalars",
+    "sparsify_counts", "write_adjacency_graph",
 ]

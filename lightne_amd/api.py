@@ -456,3 +456,25 @@ def dense_cholqr(Y, ctx: Optional[Context] = None):
 def bessel_i(k: int, x: float) -> float:
     """Modified Bessel I_k(x) as used for the Chebyshev weights."""
     return load().lightne_bessel_i(k, x)
+
+
+# ---------------------------------------------------------------------------
+# differentiation form
+# ---------------------------------------------------------------------------
+def propagate_backward(a, grad_out, graph: DeviceGraph, order: int = 10, mu: float = 0.2, theta: float = 0.5,
+                       aggregate: bool = False, checkpoint_every: int = 3, ctx: Optional[Context] = None):
+    """dL/da, dL/dmu, dL/dtheta of y (or z = (A+I) y when ``aggregate``) given
+    dL/dy (dL/dz) -- the hand-written adjoint of the Chebyshev recurrence."""
+    torch = _torch()
+    ctx = ctx or context()
+    a = a.contiguous()
+    g_out = grad_out.contiguous().to(torch.float32)
+    n, dim = a.shape
+    grad_a = torch.empty_like(a)
+    gm, gt = C.c_double(), C.c_double()
+    p = PropParams(order=order, mu=mu, theta=theta, precision=FP32)
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_propagate_backward(ctx.ptr, C.byref(g), _dev(a), dim, C.byref(p), int(bool(aggregate)),
+                                                 _dev(g_out), _dev(grad_a), C.byref(gm), C.byref(gt),
+                                                 checkpoint_every))
+    return grad_a, gm.value, gt.value

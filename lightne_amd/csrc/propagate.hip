@@ -65,6 +65,7 @@ void cheb_coeffs(int64_t order, double theta, std::vector<float>& c) {
   for (int64_t i = 2; i < K; ++i) c[i] = (float)(((i & 1) ? -2 : 2) * bessel_i((int)i, theta));
 }
 
+double lightne_bessel_i_impl(int k, double x) { return bessel_i(k, x); }
 extern "C" double lightne_bessel_i(int k, double x) { return bessel_i(k, x); }
 
 // forward recurrence; optional iterate checkpoints for the adjoint

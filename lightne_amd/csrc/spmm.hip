@@ -352,6 +352,7 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
     case EPI_CHEB1: return launch<XT, EPI_CHEB1, true>(ctx, P, a);                   \
     case EPI_CHEB: return launch<XT, EPI_CHEB, true>(ctx, P, a);                     \
     case EPI_CHEB_LAST: return launch<XT, EPI_CHEB_LAST, true>(ctx, P, a);           \
+    case EPI_ADJ: return launch<XT, EPI_ADJ, true>(ctx, P, a);                       \
     default: break;                                                                  \
   }
 
@@ -359,7 +360,8 @@ int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, b
   if (a.k % 16 != 0 || a.k < 16 || a.k > 1024)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel width %lld must be a multiple of 16 in [16, 1024]",
                     (long long)a.k);
-  if (impl && (epi == EPI_CHEB1 || epi == EPI_CHEB || epi == EPI_CHEB_LAST) && a.impl_mode != IMPL_PROP)
+  if (impl && (epi == EPI_CHEB1 || epi == EPI_CHEB || epi == EPI_CHEB_LAST || epi == EPI_ADJ) &&
+      a.impl_mode != IMPL_PROP)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "Chebyshev epilogue needs the propagation operator");
   if (!impl && epi != EPI_STORE && epi != EPI_STORE_F32)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "explicit SpMM supports store epilogues only");

@@ -10,6 +10,7 @@ enum SpmmEpi {
   EPI_CHEB1 = 2,      // Lx1 = alpha * M X - a ; conv = c0 a + c1 Lx1   (SpectralPropagation.h:114-148)
   EPI_CHEB = 3,       // Lx2 = M X - Lx0 - 2 Lx1 ; conv += c1 Lx2       (:151-197)
   EPI_CHEB_LAST = 4,  // as EPI_CHEB, then y = a - conv, Lx2 not stored  (:151-208)
+  EPI_ADJ = 5,        // adjoint step: out = c1 g / (d+1) + c0 (alpha M X - lx1) - lx0
 };
 
 enum SpmmImpl {
@@ -137,12 +138,23 @@ __device__ __forceinline__ void epilogue(const SpmmArgs& a, int64_t r, int c, fl
       cv[i] = a.c0 * av[i] + a.c1 * v[i];
     }
     store4<XT>(reinterpret_cast<XT*>(a.lx_out) + o, v);
-    store4<float>(a.conv + o, cv);
+    if (a.conv) store4<float>(a.conv + o, cv);
+  } else if (EPI == EPI_ADJ) {
+    // scaled adjoint of the Chebyshev recurrence: m_{i-1} = -c_{i-1} D~^-1 g
+    //   + coef (X m_i) - m_{i+1}, with X m_i = 0.5 M(M m_i) - m_i
+    float g[4], l1[4], l0[4] = {0.f, 0.f, 0.f, 0.f};
+    load4<float>(a.a_in + o, g);
+    load4<XT>(reinterpret_cast<const XT*>(a.lx1) + o, l1);
+    if (a.lx0) load4<XT>(reinterpret_cast<const XT*>(a.lx0) + o, l0);
+    const float inv_deg = (float)(1.0 / (double)(a.rp[r + 1] - a.rp[r] + 1));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = a.c1 * (g[i] * inv_deg) + a.c0 * (v[i] - l1[i]) - l0[i];
+    store4<XT>(reinterpret_cast<XT*>(a.lx_out) + o, v);
   } else {
-    float l1[4], l0[4], cv[4];
+    float l1[4], l0[4], cv[4] = {0.f, 0.f, 0.f, 0.f};
     load4<XT>(reinterpret_cast<const XT*>(a.lx1) + o, l1);
     load4<XT>(reinterpret_cast<const XT*>(a.lx0) + o, l0);
-    load4<float>(a.conv + o, cv);
+    if (a.conv) load4<float>(a.conv + o, cv);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       v[i] = (v[i] - l0[i]) + (-2.0f) * l1[i];
@@ -150,7 +162,7 @@ __device__ __forceinline__ void epilogue(const SpmmArgs& a, int64_t r, int c, fl
     }
     if (EPI == EPI_CHEB) {
       store4<XT>(reinterpret_cast<XT*>(a.lx_out) + o, v);
-      store4<float>(a.conv + o, cv);
+      if (a.conv) store4<float>(a.conv + o, cv);  // conv == NULL: plain recurrence
     } else {
       float av[4];
       load4<float>(a.a_in + o, av);

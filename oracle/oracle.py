@@ -301,3 +301,46 @@ def embed_dense(mat: np.ndarray, dim: int):
     """SpectralPropagation.h:16-43: thin SVD, rownorm(U[:, :dim] sqrt(S)); returns (emb, S)."""
     U, S, _ = np.linalg.svd(np.asarray(mat, np.float64), full_matrices=False)
     return u_sigma_root(U, S, dim, True), S
+
+
+# ---------------------------------------------------------------------------
+# differentiable FP64 restatement (torch autograd on the CPU) for the adjoint
+# ---------------------------------------------------------------------------
+def propagate_torch(offsets, edges, a, mu, theta, order: int = 10, aggregate: bool = False):
+    """y (or z = (A+I) y) with torch float64 autograd through a, mu and theta:
+    the same recurrence as ``propagate``; c_i(theta) from the Bessel series so
+    that d c_i / d theta is exact."""
+    import torch
+
+    off = np.asarray(offsets, np.int64)
+    n = off.shape[0] - 1
+    deg = torch.from_numpy(np.diff(off).astype(np.float64) + 1.0)
+    rows = np.repeat(np.arange(n), np.diff(off))
+    idx = torch.from_numpy(np.vstack([rows, np.asarray(edges, np.int64)]))
+    A = torch.sparse_coo_tensor(idx, torch.ones(idx.shape[1], dtype=torch.float64), (n, n)).coalesce()
+
+    def M(x):  # (1 - mu) x - D~^-1 (A x + x)
+        return (1.0 - mu) * x - (torch.sparse.mm(A, x) + x) / deg[:, None]
+
+    def iv(k, x):
+        h = 0.5 * x
+        term = h ** k / float(np.prod(np.arange(1, k + 1))) if k > 0 else torch.ones_like(x)
+        s = term
+        for j in range(1, 40):
+            term = term * (h * h) / (j * (j + k))
+            s = s + term
+        return s
+
+    K = max(order, 2)
+    c = [iv(0, theta), -2.0 * iv(1, theta)] + [(-2.0 if (i & 1) else 2.0) * iv(i, theta) for i in range(2, K)]
+    Lx0 = a
+    Lx1 = 0.5 * M(M(a)) - a
+    conv = c[0] * Lx0 + c[1] * Lx1
+    for i in range(2, K):
+        Lx2 = M(M(Lx1)) - 2.0 * Lx1 - Lx0
+        conv = conv + c[i] * Lx2
+        Lx0, Lx1 = Lx1, Lx2
+    y = a - conv
+    if aggregate:
+        return torch.sparse.mm(A, y) + y
+    return y

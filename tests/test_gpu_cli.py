@@ -33,7 +33,8 @@ def test_cli_matches_api(gpu, tmp_path):
     np.testing.assert_array_equal(e1, emb.cpu().numpy())
     p2, _ = L.chebyshev_expansion(emb, g, 32, 32)
     np.testing.assert_array_equal(p1, p2.cpu().numpy())
-    assert np.allclose(np.linalg.norm(p1, axis=1), 1.0, atol=1e-5)
+    nrm = np.linalg.norm(p1, axis=1)
+    assert np.allclose(nrm[nrm > 1e-4], 1.0, atol=1e-5)  # rows of isolated vertices stay zero
     # -ne_in runs stage 3 alone from the saved stage-2 embedding
     pro2 = str(tmp_path / "pro2.emb")
     out = subprocess.run([CLI, "-dim", "32", "-rank", "64", "-rounds", "1", "-ne_in", ne, "-pro_out", pro2, "-s", adj],
