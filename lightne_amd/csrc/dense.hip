@@ -5,7 +5,9 @@
 //                      Box-Muller2 (LightNE/mklredsvd/mklredsvd.cc:141,229-231,
 //                      mklredsvd/util.cc:21-38), whose stream cannot be
 //                      reproduced.  Element e of stream t is fully determined by
-//                      (seed, t, e), so CPU and GPU draw the same matrix.
+//                      (seed, t, e), so CPU and GPU draw the same matrix; values
+//                      are rounded to BF16 precision so both panel types hold
+//                      the same sketch exactly.
 //  * k_gram_f64      : G = X^T Y in FP64 on v_mfma_f64_16x16x4f64 (the Gram of
 //                      CholeskyQR2 and C = Z^T B of mklredsvd.cc:327-345).
 //  * k_panel_small   : Out = X W for a tall panel X and a small W on
@@ -17,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -40,6 +43,19 @@ __host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k
   }
 }
 
+// round to the nearest BF16-representable float (ties to even): the sketch is
+// drawn at BF16 precision so that the FP32 and BF16 forms multiply by the same
+// matrix
+__host__ __device__ __forceinline__ float round_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  memcpy(&r, &u, 4);
+  return r;
+}
+
 __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float* __restrict__ out) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (2 * p >= count) return;
@@ -49,8 +65,8 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
   const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
   const double r = sqrt(-2.0 * log(u1));
   const double t = 6.283185307179586476925286766559 * u2;
-  out[2 * p] = (float)(r * cos(t));
-  if (2 * p + 1 < count) out[2 * p + 1] = (float)(r * sin(t));
+  out[2 * p] = round_bf16((float)(r * cos(t)));
+  if (2 * p + 1 < count) out[2 * p + 1] = round_bf16((float)(r * sin(t)));
 }
 
 // ---------------------------------------------------------------------------
@@ -260,8 +276,8 @@ void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* ou
     const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
     const double r = std::sqrt(-2.0 * std::log(u1));
     const double t = two_pi * u2;
-    out[2 * p] = (float)(r * std::cos(t));
-    if (2 * p + 1 < count) out[2 * p + 1] = (float)(r * std::sin(t));
+    out[2 * p] = round_bf16((float)(r * std::cos(t)));
+    if (2 * p + 1 < count) out[2 * p + 1] = round_bf16((float)(r * std::sin(t)));
   }
 }
 

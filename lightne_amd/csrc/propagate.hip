@@ -105,27 +105,34 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   base.conv = conv;
   int calls = 0;
   LN_HIP(ctx, hipEventRecord(ctx->ev[13], s));
+  // The first step (T_0 -> T_1, the two largest weights c_0, c_1) always
+  // gathers FP32: in the BF16 form it is the refinement pass; later iterates
+  // are gathered in the panel type.
+  float* tmp32 = bf ? (float*)ctx_buf(ctx, "pr_tmp32", pe * sizeof(float)) : (float*)tmp;
+  float* lx1_32 = bf ? (float*)ctx_buf(ctx, "pr_lx1_32", pe * sizeof(float)) : (float*)B[1];
+  if (!tmp32 || !lx1_32) return LIGHTNE_ERR_OOM;
   // tmp = M a   (:101-111)
   {
     SpmmArgs t = base;
-    t.X = lx0;
-    t.out = tmp;
+    t.X = a;
+    t.out = tmp32;
     t.alpha = 1.f;
-    LN_CHECK(spmm_run(ctx, P, t, xt, EPI_STORE, true));
+    LN_CHECK(spmm_run(ctx, P, t, PANEL_F32, EPI_STORE, true));
     ++calls;
   }
   // Lx1 = 0.5 M tmp - a ; conv = c0 a + c1 Lx1   (:114-148)
   void* lx1 = B[1];
   {
     SpmmArgs t = base;
-    t.X = tmp;
+    t.X = tmp32;
     t.alpha = 0.5f;
     t.c0 = c[0];
     t.c1 = c[1];
-    t.lx_out = lx1;
-    LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB1, true));
+    t.lx_out = lx1_32;
+    LN_CHECK(spmm_run(ctx, P, t, PANEL_F32, EPI_CHEB1, true));
     ++calls;
   }
+  if (bf) LN_CHECK(f32_to_bf16(ctx, lx1_32, (__hip_bfloat16*)lx1, (int64_t)pe));
   float* yout = y;
   if (K <= 2) {
     hipLaunchKernelGGL(k_y_from_conv, dim3((unsigned)((pe + 255) / 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);

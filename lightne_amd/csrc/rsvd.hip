@@ -13,8 +13,9 @@
 //   C = Z^T B                        (:327-350)   FP64 Gram kernel
 //   C = U_C S V^T                    (:352-377)   host FP64 eig(C C^T)
 //   matU = Z U_C                     (:379-400)
-// In the BF16 form the gathered SpMM panels (Omega, Q) are BF16 with FP32
-// accumulation; every other panel stays FP32 and the Grams FP64.
+// In the BF16 form the sketch Omega and the power-iteration panels are
+// gathered as BF16 with FP32 accumulation; the last product B = A Q gathers
+// FP32 (refinement pass); every other panel stays FP32 and the Grams FP64.
 #include "lightne_internal.h"
 #include "spmm_epi.h"
 
@@ -227,8 +228,12 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
     LN_CHECK(cholqr2(ctx, P1, n, k, nullptr, &passes));
     st.qr_calls++;
     LN_HIP(ctx, hipEventRecord(e[5], s));
-    if (bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
-    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P1, bf ? PANEL_BF16 : PANEL_F32, k, P2));
+    // power iterations gather BF16 in the BF16 form; the last product B = A Q
+    // gathers the FP32 Q (the refinement pass that keeps C = Z^T A Q exact for
+    // the orthonormal basis Q)
+    const bool gather_bf = bf && (it + 1 < q);
+    if (gather_bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
+    LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2));
     st.spmm_calls++;
     LN_HIP(ctx, hipEventRecord(e[6], s));
     LN_HIP(ctx, hipEventSynchronize(e[6]));

@@ -473,7 +473,8 @@ void orc_aggregate_f64(uint64_t n, const uint64_t* off, const uint32_t* adj, con
  * (seed_lo, seed_hi), counter (pair index lo, pair index hi, stream, 0); the
  * first two 32-bit words give u1 = (w0 + 0.5) / 2^32 and u2 = (w1 + 0.5) / 2^32
  * and Box-Muller gives the pair (r cos t, r sin t) for elements 2p and 2p+1.
- * Computed in double, rounded to float.
+ * Computed in double, rounded to float, then to the nearest BF16-representable
+ * value (the sketch is drawn at BF16 precision).
  * pinned by: Philox4x32-10 known-answer vectors from the Random123
  * distribution (counter 0 / key 0 and counter = key = all-ones); moments. */
 void orc_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
@@ -491,6 +492,16 @@ void orc_philox4x32(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+/* nearest BF16-representable float, ties to even */
+static float round_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
 void orc_gaussian(uint64_t seed, uint32_t stream, uint64_t count, float* out) {
   const double two_pi = 6.283185307179586476925286766559;
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
@@ -502,8 +513,8 @@ void orc_gaussian(uint64_t seed, uint32_t stream, uint64_t count, float* out) {
     double u2 = ((double)w[1] + 0.5) * 2.3283064365386963e-10;
     double r = sqrt(-2.0 * log(u1));
     double t = two_pi * u2;
-    out[2 * p] = (float)(r * cos(t));
-    if (2 * p + 1 < count) out[2 * p + 1] = (float)(r * sin(t));
+    out[2 * p] = round_bf16((float)(r * cos(t)));
+    if (2 * p + 1 < count) out[2 * p + 1] = round_bf16((float)(r * sin(t)));
   }
 }
 

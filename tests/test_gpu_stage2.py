@@ -121,17 +121,27 @@ def test_rsvd_c1_matches_fp64_oracle(gpu, q):
     np.testing.assert_allclose(embn, refn, rtol=1e-5, atol=1e-6)
 
 
-def test_rsvd_bf16_panels_error_vs_fp64(gpu):
+@pytest.mark.parametrize("q", [1, 2])
+def test_rsvd_bf16_panels_error_vs_fp64(gpu, q):
+    """BF16 form: the sketch and power-iteration panels are gathered as BF16, the
+    last product B = A Q in FP32.  Same sketch as the FP64 run (drawn at BF16
+    precision).  Bar: top-d principal angle <= 1e-3 rad."""
     hg, csr = _c1_sparsifier(gpu)
     k, d = 144, 128
-    svd = L.MKLRedSVD(csr, k, precision=L.BF16, seed=777, ctx=gpu).run()
+    svd = L.MKLRedSVD(csr, k, power_iters=q, precision=L.BF16, seed=777, ctx=gpu).run()
     rp, col, val = csr.to_numpy()
-    U64, S64 = O.rsvd(rp, col, val, k, seed=777)
+    U64, S64 = O.rsvd(rp, col, val, k, seed=777, power_iters=q)
     rel = np.abs(svd.S[:d] - S64[:d]) / S64[:d]
     ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d], U64[:, :d])
-    print(f"BF16: max rel sv err={rel.max():.2e} angle={ang:.2e}")
-    assert rel.max() < 1e-2
-    assert ang < 5e-2
+    gap = S64[d - 1] / S64[d]
+    print(f"BF16 q={q}: max rel sv err={rel.max():.2e} angle={ang:.2e} gap={gap:.4f}")
+    assert rel.max() <= 1e-4
+    # q = 1 gathers BF16 only for the BF16-exact sketch (bar 1e-3 rad); a BF16
+    # power iteration perturbs Q by ~2^-9, amplified by 1/(gap - 1) ~ 800 here
+    assert ang <= (1e-3 if q == 1 else 1e-2)
+    if q == 1:  # one BF16 gather of a BF16-exact sketch: identical to FP32
+        f32 = L.MKLRedSVD(csr, k, power_iters=1, seed=777, ctx=gpu).run()
+        np.testing.assert_array_equal(svd.S, f32.S)
 
 
 def test_netsmf_end_to_end(gpu):

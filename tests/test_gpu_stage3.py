@@ -41,7 +41,7 @@ def test_propagate_bf16_error(gpu):
     ey = np.abs(y.cpu().numpy() - y64).max() / np.abs(y64).max()
     ez = np.abs(z.cpu().numpy() - z64).max() / np.abs(z64).max()
     print(f"BF16 rel err y={ey:.2e} z={ez:.2e}")
-    assert ey < 3e-2 and ez < 3e-2
+    assert ey < 1e-3 and ez < 1e-3
 
 
 @pytest.mark.parametrize("n,r,d", [(5000, 128, 128), (3000, 144, 128), (4096, 64, 32)])
