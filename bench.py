@@ -174,6 +174,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-bf16", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device 3 times (a kernel of known traffic "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU (the whole pipeline on one MI355X); --gpus {args.gpus} "
@@ -205,6 +208,13 @@ def main():
         log(f"step {s}: {times[-1]:.3f}s")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ms_per_step = 1000.0 * wall / args.steps
     sp, rs = st["sparsify"], st["rsvd"]
     n, k, d = hg.n, cfg["k"], cfg["d"]
@@ -222,14 +232,20 @@ def main():
               "rsvd_small_svd": rs["t_small_svd"], "rsvd_total": rs["t_total"], "netsmf_total": st["t_total"],
               "prop_cheb": pst["t_cheb"], "prop_aggregate": pst["t_aggregate"], "prop_svd": pst["t_svd"],
               "prop_total": pst["t_total"]}
-    # dominant kernel by measured device time
+    # dominant kernel by measured device time; stage-2 SpMM = k_spmm<float,1,false>
+    # (+ k_spmm_fixup for rows split over several chunks)
     cands = {"spmm_stage2": (rs["t_spmm"], spmm_bytes * rs["spmm_calls"], rs["spmm_calls"]),
              "sampler": (sp["t_sample"], sampler_bytes, 1)}
     dom = max(cands, key=lambda c: cands[c][0])
     t_dom, bytes_dom, launches = cands[dom]
     achieved = bytes_dom / t_dom / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{args.config}_{dom}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": int(bytes_dom / launches), "t_per_launch_ms": round(1e3 * t_dom / launches, 3)}
     bf16 = None
     if not args.skip_bf16:

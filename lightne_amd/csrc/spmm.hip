@@ -9,11 +9,11 @@
 // at k=144 FP32, 288 B BF16); every load instruction of a group covers G
 // consecutive 16-byte pieces.  A 64-wide wave holds NG = 64/G groups.
 //
-// Rows are split by length (power-law graphs):
+// Rows are split by length (power-law graphs), in one launch:
 //   * short rows (nnz <= SHORT): one group per row, no cross-lane reduction;
 //   * long rows: chunks of CHUNK nnz, one wave per chunk, the NG groups of the
-//     wave stride over the chunk and reduce through LDS; rows with several
-//     chunks write partial sums that k_spmm_fixup adds in chunk order
+//     wave stride over the chunk and are reduced by in-wave shuffles; rows with
+//     several chunks write partial sums that k_spmm_fixup adds in chunk order
 //     (deterministic, no float atomics).
 // Epilogues fuse the Chebyshev recurrence of SpectralPropagation.h:94-208 so
 // each step is exactly two gathers over the graph.
@@ -120,71 +120,57 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
   }
 }
 
-// short rows: one group of G lanes per row
+// One launch for both row classes.  Waves [0, W_short) take NG consecutive
+// rows each, one group per row, and skip rows longer than short_nnz; the
+// remaining waves take one chunk of a long row each: the NG groups stride over
+// the chunk and are summed into group 0 by in-wave shuffles (group order, so
+// the result is deterministic), which applies the epilogue or writes the
+// chunk's partial sum.
 template <typename XT, int EPI, bool IMPL>
-__global__ __launch_bounds__(256) void k_spmm_short(SpmmArgs a) {
+__global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int64_t w_short) {
   const int lane = threadIdx.x & 63;
   const int G = a.G, NG = 64 / G;
   const int g = lane / G, sub = lane - g * G;
-  if (g >= NG) return;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t r = wave * NG + g;
-  if (r >= a.n) return;
-  const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
-  if (j1 - j0 > a.short_nnz) return;
   float acc[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-  gather_row<XT, IMPL>(a, j0, j1, 1, sub, acc);
+  if (wave < w_short) {
+    if (g >= NG) return;
+    const int64_t r = wave * NG + g;
+    if (r >= a.n) return;
+    const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
+    if (j1 - j0 > a.short_nnz) return;
+    gather_row<XT, IMPL>(a, j0, j1, 1, sub, acc);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float4 v = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+      epilogue<XT, EPI, IMPL>(a, r, quad_col<XT>(q, sub, G), v);
+    }
+    return;
+  }
+  const int64_t c = wave - w_short;
+  if (c >= a.nchunks) return;  // uniform across the wave
+  if (g < NG) gather_row<XT, IMPL>(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub, acc);
+  for (int q = 1; q < NG; ++q) {
+    const int src = (lane + q * G) & 63;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const float t = __shfl(acc[i], src, 64);
+      if (lane < G) acc[i] += t;
+    }
+  }
+  if (lane >= G) return;
+  const int64_t row = a.chunk_row[c];
+  const int slot = a.chunk_slot[c];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    float4 v = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
-    epilogue<XT, EPI, IMPL>(a, r, quad_col<XT>(q, sub, G), v);
-  }
-}
-
-// long rows: one wave per chunk
-template <typename XT, int EPI, bool IMPL>
-__global__ __launch_bounds__(256) void k_spmm_long(SpmmArgs a) {
-  extern __shared__ float s_red[];  // [waves][NG][k]
-  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
-  const int G = a.G, NG = 64 / G;
-  const int g = lane / G, sub = lane - g * G;
-  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const bool active = c < a.nchunks;
-  float* red = s_red + (size_t)wib * NG * a.k;
-  float acc[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-  int64_t row = 0;
-  int slot = -1;
-  if (active) {
-    row = a.chunk_row[c];
-    slot = a.chunk_slot[c];
-    if (g < NG) gather_row<XT, IMPL>(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub, acc);
-  }
-  if (active && g < NG) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      *reinterpret_cast<float4*>(red + g * a.k + quad_col<XT>(q, sub, G)) =
-          make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
-  }
-  __syncthreads();
-  if (!active) return;
-  const int ncol4 = (int)(a.k >> 2);
-  if (lane < ncol4) {
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int gg = 0; gg < NG; ++gg) {
-      const float4 t = *reinterpret_cast<const float4*>(red + gg * a.k + 4 * lane);
-      s.x += t.x;
-      s.y += t.y;
-      s.z += t.z;
-      s.w += t.w;
-    }
+    const float4 v = make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+    const int col = quad_col<XT>(q, sub, G);
     if (slot < 0)
-      epilogue<XT, EPI, IMPL>(a, row, 4 * lane, s);
+      epilogue<XT, EPI, IMPL>(a, row, col, v);
     else
-      *reinterpret_cast<float4*>(a.partial + (int64_t)slot * a.k + 4 * lane) = s;
+      *reinterpret_cast<float4*>(a.partial + (int64_t)slot * a.k + col) = v;
   }
 }
 
@@ -227,15 +213,10 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
     if (!a.partial) return LIGHTNE_ERR_OOM;
   }
   const int64_t waves_short = (a.n + NG - 1) / NG;
-  hipLaunchKernelGGL((k_spmm_short<XT, EPI, IMPL>), dim3((unsigned)((waves_short + 3) / 4)), dim3(256), 0,
-                     ctx->stream, a);
+  const int64_t waves = waves_short + P.nchunks;
+  hipLaunchKernelGGL((k_spmm<XT, EPI, IMPL>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a,
+                     waves_short);
   LN_LAUNCH(ctx);
-  if (P.nchunks > 0) {
-    const size_t lds = (size_t)4 * NG * a.k * sizeof(float);
-    hipLaunchKernelGGL((k_spmm_long<XT, EPI, IMPL>), dim3((unsigned)((P.nchunks + 3) / 4)), dim3(256), lds,
-                       ctx->stream, a);
-    LN_LAUNCH(ctx);
-  }
   if (P.nmulti > 0) {
     hipLaunchKernelGGL((k_spmm_fixup<XT, EPI, IMPL>), dim3((unsigned)((P.nmulti + 3) / 4)), dim3(256), 0,
                        ctx->stream, a);
