@@ -59,6 +59,12 @@ void ctx_release(lightne_ctx* ctx, const char* name) {
   ctx->bufs.erase(it);
 }
 
+void ctx_trim(lightne_ctx* ctx) {
+  for (auto& kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  ctx->bufs.clear();
+}
+
 int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what) {
   if (m.p) {
     (void)hipFree(m.p);

@@ -50,6 +50,7 @@ int ctx_fail(lightne_ctx* ctx, int code, const char* fmt, ...);
 int global_fail(int code, const char* fmt, ...);
 void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes);  // nullptr on OOM
 void ctx_release(lightne_ctx* ctx, const char* name);
+void ctx_trim(lightne_ctx* ctx);  // frees every cached buffer (none may be in use)
 
 #define LN_HIP(ctx, expr)                                                                        \
   do {                                                                                           \

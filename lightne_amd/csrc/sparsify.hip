@@ -407,19 +407,28 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   LN_CHECK(lightne_sample_scalars(g->n, g->m, p, st));
   const uint64_t per_edge = st->exp_rv_rounded + (st->fractional_bits > 0.0000000001 ? 1 : 0);
   const uint64_t max_samples = (uint64_t)g->m * per_edge;
-  // capacity: load factor <= 1/2 for the worst case U = samples, never above
-  // the reference's own 2^ceil(log2(2 table_size)) unless that is smaller
-  // than what the samples need.
-  uint64_t cap = next_pow2(std::max<uint64_t>(2 * max_samples, 1024));
+  // capacity: load factor <= 0.7 even if every sample is a distinct key (the
+  // reference sizes by -tablesz / -mem_ratio, 2^ceil(log2(2 table_size)), and
+  // exits when that is too small, par_table.h:104-107)
+  uint64_t cap = next_pow2(std::max<uint64_t>((max_samples / 7) * 10 + 10, 1024));
   if (p->max_table_bytes) {
     const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
     uint64_t lim = 1024;
     while (lim * 2 * slot_bytes <= p->max_table_bytes) lim *= 2;
     if (cap > lim) cap = lim;
   }
-  if (cap > (1ull << 32)) return ctx_fail(ctx, LIGHTNE_ERR_OOM, "hash table of %llu slots exceeds 2^32", (unsigned long long)cap);
   st->table_capacity = cap;
 
+  // the table, the kept entries (at most two per sample) and their sort scratch
+  // are the pipeline's memory peak: drop buffers cached by earlier calls when
+  // they would not fit beside it
+  {
+    const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
+    const uint64_t need = cap * slot_bytes + (uint64_t)g->m * 4 + 24 * max_samples;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < need) ctx_trim(ctx);
+    (void)hipGetLastError();
+  }
   auto* tab = (Slot*)ctx_buf(ctx, "tab", cap * sizeof(Slot));
   double* wtab = nullptr;
   if (p->sample) {
@@ -540,6 +549,10 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   double* wtab;
   uint64_t cap;
   LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  // the compacted table addresses slots with 32-bit indices
+  if (cap > (1ull << 32))
+    return ctx_fail(ctx, LIGHTNE_ERR_OOM, "lightne_sparsify_counts: table of %llu slots exceeds 2^32",
+                    (unsigned long long)cap);
   uint64_t distinct, csum;
   double wsum;
   LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
@@ -656,6 +669,11 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
   ta.sval = (float*)mv.p;
   hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
   LN_LAUNCH(ctx);
+  // the table is the largest buffer of the pipeline: give it back before the
+  // sort allocates its ping-pong buffers
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  ctx_release(ctx, "tab");
+  ctx_release(ctx, "wtab");
   LN_CHECK(radix_sort_pairs(ctx, (uint64_t*)mk.p, (float*)mv.p, nnz, 2 * colbits));
 
   auto* csr = new lightne_csr();
@@ -681,9 +699,12 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
   csr->val = (float*)mv.release();
   st.t_table = ev_seconds(ctx->ev[3], ctx->ev[4]);
   st.t_csr = ev_seconds(ctx->ev[4], ctx->ev[5]);
-  // the table is the largest buffer of the pipeline; give it back
-  ctx_release(ctx, "tab");
-  ctx_release(ctx, "wtab");
+  // sort scratch scales with nnz; later stages need the memory
+  ctx_release(ctx, "rs_keys");
+  ctx_release(ctx, "rs_vals");
+  ctx_release(ctx, "rs_hist");
+  ctx_release(ctx, "rs_offs");
+  ctx_release(ctx, "edge_src");
   *out = csr;
   if (stats) *stats = st;
   return LIGHTNE_OK;
