@@ -138,6 +138,7 @@ extern "C" int lightne_propagate_backward(lightne_ctx* ctx, const lightne_graph*
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate_backward: bad graph or dim");
   if (p->precision != LIGHTNE_FP32)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate_backward: the adjoint runs in FP32 only");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   Adj A;
   A.ctx = ctx;

@@ -1,6 +1,7 @@
 // Context, error reporting and scratch management for liblightne_amd.
 #include "lightne_internal.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -33,8 +34,42 @@ int global_fail(int code, const char* fmt, ...) {
   return code;
 }
 
+namespace {
+// frees the cached buffers not used by the call in progress; false if none
+bool evict_stale(lightne_ctx* ctx) {
+  if (ctx->depth == 0) return false;
+  bool any = false;
+  for (auto it = ctx->bufs.begin(); it != ctx->bufs.end();) {
+    if (it->second.epoch != ctx->epoch) {
+      if (it->second.p) (void)hipFree(it->second.p);
+      it = ctx->bufs.erase(it);
+      any = true;
+    } else {
+      ++it;
+    }
+  }
+  return any;
+}
+}  // namespace
+
+int ctx_malloc(lightne_ctx* ctx, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) return hipSuccess;
+  (void)hipGetLastError();
+  *p = nullptr;
+  if (!evict_stale(ctx)) return e;
+  e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+  }
+  return e;
+}
+
 void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes) {
   auto& b = ctx->bufs[name];
+  b.epoch = ctx->epoch;
   if (b.bytes >= bytes && b.p) return b.p;
   if (b.p) {
     (void)hipFree(b.p);
@@ -42,14 +77,18 @@ void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes) {
     b.bytes = 0;
   }
   size_t want = bytes ? bytes : 256;
-  if (hipMalloc(&b.p, want) != hipSuccess) {
-    (void)hipGetLastError();
-    b.p = nullptr;
+  void* q = nullptr;
+  if (ctx_malloc(ctx, &q, want) != hipSuccess) {
+    ctx->bufs.erase(name);
     ctx_fail(ctx, LIGHTNE_ERR_OOM, "out of device memory allocating %zu bytes for %s", want, name);
     return nullptr;
   }
-  b.bytes = want;
-  return b.p;
+  // evict_stale may have rehashed the map: look the entry up again
+  auto& b2 = ctx->bufs[name];
+  b2.p = q;
+  b2.bytes = want;
+  b2.epoch = ctx->epoch;
+  return q;
 }
 
 void ctx_release(lightne_ctx* ctx, const char* name) {
@@ -59,10 +98,16 @@ void ctx_release(lightne_ctx* ctx, const char* name) {
   ctx->bufs.erase(it);
 }
 
-void ctx_trim(lightne_ctx* ctx) {
-  for (auto& kv : ctx->bufs)
-    if (kv.second.p) (void)hipFree(kv.second.p);
-  ctx->bufs.clear();
+int WsCarve::commit(lightne_ctx* ctx, size_t reserve) {
+  const size_t need = std::max(end(), reserve);
+  char* w = (char*)ctx_buf(ctx, "ws", need);
+  if (!w) return LIGHTNE_ERR_OOM;
+  size_t o = base;
+  for (auto& r : req) {
+    *r.first = w + o;
+    o += (r.second + 255) & ~(size_t)255;
+  }
+  return LIGHTNE_OK;
 }
 
 int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what) {
@@ -70,8 +115,7 @@ int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what) {
     (void)hipFree(m.p);
     m.p = nullptr;
   }
-  if (hipMalloc(&m.p, bytes ? bytes : 256) != hipSuccess) {
-    (void)hipGetLastError();
+  if (ctx_malloc(ctx, &m.p, bytes) != hipSuccess) {
     m.p = nullptr;
     return ctx_fail(ctx, LIGHTNE_ERR_OOM, "out of device memory allocating %zu bytes for %s", bytes, what);
   }

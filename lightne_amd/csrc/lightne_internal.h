@@ -23,8 +23,11 @@ struct lightne_ctx {
   struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
+    uint64_t epoch = 0;  // call that last used it
   };
-  std::unordered_map<std::string, Buf> bufs;  // grow-only named scratch
+  std::unordered_map<std::string, Buf> bufs;  // grow-only named scratch, kept across calls
+  uint64_t epoch = 0;  // numbered top-level library calls
+  int depth = 0;       // nesting of library calls in progress
   hipEvent_t ev[16] = {};
   int num_cus = 256;
 };
@@ -43,6 +46,7 @@ struct lightne_csr {
   int64_t* row_ptr = nullptr;
   int32_t* col = nullptr;
   float* val = nullptr;
+  bool owns = true;          // false: the arrays are context buffers (internal use)
   SpmmPlan* plan = nullptr;  // nnz-balanced SpMM work split, built on first use
 };
 
@@ -50,7 +54,38 @@ int ctx_fail(lightne_ctx* ctx, int code, const char* fmt, ...);
 int global_fail(int code, const char* fmt, ...);
 void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes);  // nullptr on OOM
 void ctx_release(lightne_ctx* ctx, const char* name);
-void ctx_trim(lightne_ctx* ctx);  // frees every cached buffer (none may be in use)
+// Device memory kept across calls is reclaimed only under pressure: when an
+// allocation fails inside a scoped call, the cached buffers that call has not
+// touched are freed and the allocation is tried once more.
+struct CallScope {
+  lightne_ctx* c;
+  explicit CallScope(lightne_ctx* c_) : c(c_) {
+    if (c && c->depth++ == 0) ++c->epoch;
+  }
+  ~CallScope() {
+    if (c) --c->depth;
+  }
+};
+int ctx_malloc(lightne_ctx* ctx, void** p, size_t bytes);  // hipMalloc with reclaim-and-retry; hipError_t
+// Sub-buffers carved, 256-B aligned, from the context's shared workspace (the
+// buffer named "ws").  Stages whose large temporaries are never live at the
+// same time (the sampler's table and sort scratch, the randomized SVD's
+// panels, the propagation iterates) reuse the same device memory, so that a
+// repeated pipeline neither frees nor maps memory.
+struct WsCarve {
+  std::vector<std::pair<void**, size_t>> req;
+  size_t base = 0;  // first byte of ws this carve may use
+  explicit WsCarve(size_t base_ = 0) : base(base_) {}
+  void add(void** p, size_t bytes) { req.push_back({p, bytes}); }
+  size_t end() const {
+    size_t o = base;
+    for (auto& r : req) o += (r.second + 255) & ~(size_t)255;
+    return o;
+  }
+  // reserve ws (growing it if needed; contents are lost on growth) and fill
+  // the pointers; returns nonzero on OOM
+  int commit(lightne_ctx* ctx, size_t reserve = 0);
+};
 
 #define LN_HIP(ctx, expr)                                                                        \
   do {                                                                                           \
@@ -88,6 +123,12 @@ struct DevMem {
 };
 int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 
+// stage 1 (sparsify.hip): sampler, table, trunc-log filter, CSR.  ctx_owned:
+// the CSR arrays are context buffers (reused by the next call) and the handle
+// does not free them.
+int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
+                  lightne_sparsify_stats* stats, bool ctx_owned);
+
 // elapsed seconds between two events recorded on the context stream
 double ev_seconds(hipEvent_t a, hipEvent_t b);
 
@@ -99,8 +140,10 @@ double ev_seconds(hipEvent_t a, hipEvent_t b);
 int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n);
 int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n);
 // stable LSD radix sort of (key, val) pairs on the low `bits` bits of key.
-// keys/vals hold the result on return; tmp buffers come from the context.
-int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits);
+// keys/vals hold the result on return; the ping-pong buffers (n keys, n vals)
+// are k_tmp / v_tmp when given, else context buffers.
+int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp = nullptr,
+                     float* v_tmp = nullptr);
 int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits);
 
 // ---------------------------------------------------------------------------

@@ -229,11 +229,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
 }
 
 template <typename V>
-int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits) {
+int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits, uint64_t* k2, V* v2) {
   if (n <= 1 || bits <= 0) return LIGHTNE_OK;
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  auto* k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
-  auto* v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
+  if (!k2 || !v2) {
+    k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
+    v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
+  }
   auto* hist = (uint32_t*)ctx_buf(ctx, "rs_hist", 256 * ntiles * sizeof(uint32_t));
   auto* offs = (uint64_t*)ctx_buf(ctx, "rs_offs", (256 * ntiles + 1) * sizeof(uint64_t));
   if (!k2 || !v2 || !hist || !offs) return LIGHTNE_ERR_OOM;
@@ -265,9 +267,10 @@ int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t
 int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n) {
   return scan_impl<uint64_t>(ctx, in, out, n, 0);
 }
-int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits) {
-  return radix_sort_impl<float>(ctx, keys, vals, n, bits);
+int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp,
+                     float* v_tmp) {
+  return radix_sort_impl<float>(ctx, keys, vals, n, bits, k_tmp, v_tmp);
 }
 int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits) {
-  return radix_sort_impl<uint32_t>(ctx, keys, vals, n, bits);
+  return radix_sort_impl<uint32_t>(ctx, keys, vals, n, bits, nullptr, nullptr);
 }

@@ -68,6 +68,14 @@ void cheb_coeffs(int64_t order, double theta, std::vector<float>& c) {
 double lightne_bessel_i_impl(int k, double x) { return bessel_i(k, x); }
 extern "C" double lightne_bessel_i(int k, double x) { return bessel_i(k, x); }
 
+// workspace bytes propagate_forward carves (from offset 0)
+size_t prop_ws_bytes(int64_t n, int64_t dim, bool bf) {
+  const size_t pe = (size_t)n * dim;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t esz = bf ? 2 : 4;
+  return 4 * al(pe * esz) + al(pe * 4) + (bf ? 2 * al(pe * 4) : 0);
+}
+
 // forward recurrence; optional iterate checkpoints for the adjoint
 int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                       const lightne_prop_params* p, float* y, float* z, lightne_prop_stats* st, SpmmPlan& P) {
@@ -76,13 +84,25 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   const int xt = bf ? PANEL_BF16 : PANEL_F32;
   const size_t esz = bf ? 2 : 4;
   const size_t pe = (size_t)n * dim;
+  // iterates and accumulator at the start of the context workspace
   void* B[3];
-  B[0] = ctx_buf(ctx, "pr_b0", pe * esz);
-  B[1] = ctx_buf(ctx, "pr_b1", pe * esz);
-  B[2] = ctx_buf(ctx, "pr_b2", pe * esz);
-  void* tmp = ctx_buf(ctx, "pr_tmp", pe * esz);
-  auto* conv = (float*)ctx_buf(ctx, "pr_conv", pe * sizeof(float));
-  if (!B[0] || !B[1] || !B[2] || !tmp || !conv) return LIGHTNE_ERR_OOM;
+  void* tmp = nullptr;
+  float* conv = nullptr;
+  float* tmp32 = nullptr;
+  float* lx1_32 = nullptr;
+  WsCarve wc;
+  for (int j = 0; j < 3; ++j) wc.add(&B[j], pe * esz);
+  wc.add(&tmp, pe * esz);
+  wc.add((void**)&conv, pe * sizeof(float));
+  if (bf) {
+    wc.add((void**)&tmp32, pe * sizeof(float));
+    wc.add((void**)&lx1_32, pe * sizeof(float));
+  }
+  LN_CHECK(wc.commit(ctx));
+  if (!bf) {
+    tmp32 = (float*)tmp;
+    lx1_32 = (float*)B[1];
+  }
   std::vector<float> c;
   cheb_coeffs(p->order, (double)p->theta, c);
   hipStream_t s = ctx->stream;
@@ -108,9 +128,6 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   // The first step (T_0 -> T_1, the two largest weights c_0, c_1) always
   // gathers FP32: in the BF16 form it is the refinement pass; later iterates
   // are gathered in the panel type.
-  float* tmp32 = bf ? (float*)ctx_buf(ctx, "pr_tmp32", pe * sizeof(float)) : (float*)tmp;
-  float* lx1_32 = bf ? (float*)ctx_buf(ctx, "pr_lx1_32", pe * sizeof(float)) : (float*)B[1];
-  if (!tmp32 || !lx1_32) return LIGHTNE_ERR_OOM;
   // tmp = M a   (:101-111)
   {
     SpmmArgs t = base;
@@ -190,11 +207,14 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
 }
 
 int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
-                    float* S_host) {
+                    float* S_host, float* Q = nullptr) {
   if (rank % 16 || dim % 16 || dim > rank || rank > 256 || n < rank)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embed_dense: rank, dim must be multiples of 16, dim <= rank <= 256, n >= rank");
-  auto* Q = (float*)ctx_buf(ctx, "ed_q", (size_t)n * rank * sizeof(float));
-  if (!Q) return LIGHTNE_ERR_OOM;
+  if (!Q) {
+    WsCarve wc;
+    wc.add((void**)&Q, (size_t)n * rank * sizeof(float));
+    LN_CHECK(wc.commit(ctx));
+  }
   LN_HIP(ctx, hipMemcpyAsync(Q, mat, (size_t)n * rank * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
   std::vector<double> R(rank * rank), Ur(rank * rank), Sr(rank);
   int passes = 0;
@@ -220,14 +240,16 @@ extern "C" {
 int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                       const lightne_prop_params* p, float* y_out, float* z_out, lightne_prop_stats* stats) {
   LN_CHECK(check_prop(ctx, g, a, dim, p));
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   auto t0 = std::chrono::steady_clock::now();
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
   float* y = y_out;
   if (!y) {
-    y = (float*)ctx_buf(ctx, "pr_y", (size_t)g->n * dim * sizeof(float));
-    if (!y) return LIGHTNE_ERR_OOM;
+    WsCarve wc(prop_ws_bytes(g->n, dim, p->precision == LIGHTNE_BF16));
+    wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
+    LN_CHECK(wc.commit(ctx));
   }
   lightne_prop_stats st{};
   LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z_out, &st, gp.P));
@@ -240,6 +262,7 @@ int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
 int lightne_embed_dense(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
                         float* S_host) {
   if (!ctx || !mat || !emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_embed_dense: NULL argument");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   return embed_dense_dev(ctx, mat, n, rank, dim, emb_out, S_host);
 }
@@ -248,17 +271,22 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
                                 const lightne_prop_params* p, float* emb_out, lightne_prop_stats* stats) {
   LN_CHECK(check_prop(ctx, g, a, dim, p));
   if (!emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "chebyshev_expansion: NULL output");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   auto t0 = std::chrono::steady_clock::now();
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
-  auto* y = (float*)ctx_buf(ctx, "pr_y", (size_t)g->n * dim * sizeof(float));
-  auto* z = (float*)ctx_buf(ctx, "pr_z", (size_t)g->n * dim * sizeof(float));
-  if (!y || !z) return LIGHTNE_ERR_OOM;
+  // y, z and the tall-SVD panel after the recurrence's own workspace
+  float *y = nullptr, *z = nullptr, *q = nullptr;
+  WsCarve wc(prop_ws_bytes(g->n, dim, p->precision == LIGHTNE_BF16));
+  wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
+  wc.add((void**)&z, (size_t)g->n * dim * sizeof(float));
+  wc.add((void**)&q, (size_t)g->n * dim * sizeof(float));
+  LN_CHECK(wc.commit(ctx));
   lightne_prop_stats st{};
   LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z, &st, gp.P));
   auto t1 = std::chrono::steady_clock::now();
-  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr));
+  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr, q));
   auto t2 = std::chrono::steady_clock::now();
   st.t_svd = std::chrono::duration<double>(t2 - t1).count();
   st.t_total = std::chrono::duration<double>(t2 - t0).count();

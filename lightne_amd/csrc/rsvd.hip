@@ -110,6 +110,7 @@ int lightne_csr_create(lightne_ctx* ctx, int64_t n, int64_t nnz, const int64_t* 
                        const float* val, lightne_csr** out) {
   if (!ctx || !out || n <= 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!col || !val)))
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_csr_create: bad argument");
+  CallScope scope(ctx);
   *out = nullptr;
   LN_HIP(ctx, hipSetDevice(ctx->device));
   DevMem a, b, c;
@@ -164,9 +165,11 @@ int lightne_csr_free(lightne_csr* c) {
     spmm_plan_free(*c->plan);
     delete c->plan;
   }
-  if (c->row_ptr) (void)hipFree(c->row_ptr);
-  if (c->col) (void)hipFree(c->col);
-  if (c->val) (void)hipFree(c->val);
+  if (c->owns) {
+    if (c->row_ptr) (void)hipFree(c->row_ptr);
+    if (c->col) (void)hipFree(c->col);
+    if (c->val) (void)hipFree(c->val);
+  }
   delete c;
   return LIGHTNE_OK;
 }
@@ -184,20 +187,22 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
   if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
   const int q = std::max(1, (int)p->power_iters);
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   lightne_rsvd_stats st{};
   hipStream_t s = ctx->stream;
   const bool bf = p->precision == LIGHTNE_BF16;
   const size_t pf = (size_t)n * k * sizeof(float);
-  auto* P0 = (float*)ctx_buf(ctx, "rs_p0", pf);  // Omega, later Z
-  auto* P1 = (float*)ctx_buf(ctx, "rs_p1", pf);  // Y / Q
-  auto* P2 = (float*)ctx_buf(ctx, "rs_p2", pf);  // B
+  float *P0 = nullptr, *P1 = nullptr, *P2 = nullptr;  // Omega then Z; Y / Q; B
   __hip_bfloat16* PB = nullptr;
-  if (bf) {
-    PB = (__hip_bfloat16*)ctx_buf(ctx, "rs_pb", (size_t)n * k * 2);
-    if (!PB) return LIGHTNE_ERR_OOM;
+  {
+    WsCarve wc;
+    wc.add((void**)&P0, pf);
+    wc.add((void**)&P1, pf);
+    wc.add((void**)&P2, pf);
+    if (bf) wc.add((void**)&PB, (size_t)n * k * 2);
+    LN_CHECK(wc.commit(ctx));
   }
-  if (!P0 || !P1 || !P2) return LIGHTNE_ERR_OOM;
   hipEvent_t* e = ctx->ev;
   double t_spmm = 0, t_qr = 0, t_gemm = 0;
   auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
@@ -279,6 +284,7 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
 int lightne_dense_gram(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
                        int32_t symmetric, double* G_host) {
   if (!ctx || !X || !Y || !G_host || n < 0) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_gram: bad argument");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   return gram_f64(ctx, X, Y, n, k1, k2, symmetric, G_host);
 }
@@ -287,6 +293,7 @@ int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_
                              int64_t k2, float* Out, int32_t normalize_rows) {
   if (!ctx || !X || !W_host || !Out || n < 0)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_panel_gemm: bad argument");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   LN_CHECK(panel_times_small(ctx, X, n, k1, k1, W_host, k2, Out, normalize_rows, nullptr));
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -296,6 +303,7 @@ int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_
 int lightne_dense_cholqr(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_host, int32_t* passes_out) {
   if (!ctx || !Y || n < k || k <= 0 || k % 16 || k > 256)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_cholqr: bad argument");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   int passes = 0;
   LN_CHECK(cholqr2(ctx, Y, n, k, R_host, &passes));
@@ -308,6 +316,7 @@ int lightne_u_sigma_root(lightne_ctx* ctx, const float* U, const float* S_host, 
                          int32_t normalize, float* emb_out) {
   if (!ctx || !U || !emb_out || n <= 0 || dim <= 0 || dim > rank)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_u_sigma_root: bad argument");
+  CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   LN_CHECK(u_sigma_root_dev(ctx, U, S_host, n, rank, dim, normalize, emb_out));
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -319,10 +328,11 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
                    lightne_netsmf_stats* stats) {
   if (!ctx || !g || !sp || !rp || !emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: NULL argument");
   if (dim <= 0 || dim > rp->rank) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: dim must be in [1, rank]");
+  CallScope scope(ctx);
   lightne_netsmf_stats st{};
   auto t0 = std::chrono::steady_clock::now();
   lightne_csr* A = nullptr;
-  LN_CHECK(lightne_sparsify(ctx, g, sp, &A, &st.sparsify));
+  LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true));
   auto t1 = std::chrono::steady_clock::now();
   const int64_t n = g->n, k = rp->rank;
   auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));

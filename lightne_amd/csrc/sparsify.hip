@@ -395,8 +395,8 @@ int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
   return LIGHTNE_OK;
 }
 
-// Sampler + table.  On success the table (slots, optional weights) lives in
-// the context buffers "tab" / "wtab"; stats are filled.
+// Sampler + table.  On success the table (slots, optional weights) occupies
+// the start of the context workspace; stats are filled.
 int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
                 lightne_sparsify_stats* st, Slot** tab_out, double** wtab_out, uint64_t* cap_out) {
   LN_CHECK(check_graph(ctx, g));
@@ -405,12 +405,16 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   LN_HIP(ctx, hipSetDevice(ctx->device));
   *st = lightne_sparsify_stats{};
   LN_CHECK(lightne_sample_scalars(g->n, g->m, p, st));
-  const uint64_t per_edge = st->exp_rv_rounded + (st->fractional_bits > 0.0000000001 ? 1 : 0);
-  const uint64_t max_samples = (uint64_t)g->m * per_edge;
+  // Samples: exp_rv per directed edge plus one more with probability frac
+  // (PathEmbed.h:142-148), i.e. m*exp_rv + Binomial(m, frac).  Bound the
+  // binomial 8 standard deviations above its mean.
+  const double m = (double)(uint64_t)g->m;
+  const double fr = st->fractional_bits > 0.0000000001 ? st->fractional_bits : 0.0;
+  const double bound = m * (double)st->exp_rv_rounded + m * fr + 8.0 * std::sqrt(m * fr * (1.0 - fr)) + 64.0;
   // capacity: load factor <= 0.7 even if every sample is a distinct key (the
   // reference sizes by -tablesz / -mem_ratio, 2^ceil(log2(2 table_size)), and
   // exits when that is too small, par_table.h:104-107)
-  uint64_t cap = next_pow2(std::max<uint64_t>((max_samples / 7) * 10 + 10, 1024));
+  uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(bound / 0.7) + 1, 1024));
   if (p->max_table_bytes) {
     const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
     uint64_t lim = 1024;
@@ -419,25 +423,15 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   }
   st->table_capacity = cap;
 
-  // the table, the kept entries (at most two per sample) and their sort scratch
-  // are the pipeline's memory peak: drop buffers cached by earlier calls when
-  // they would not fit beside it
-  {
-    const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
-    const uint64_t need = cap * slot_bytes + (uint64_t)g->m * 4 + 24 * max_samples;
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < need) ctx_trim(ctx);
-    (void)hipGetLastError();
-  }
-  auto* tab = (Slot*)ctx_buf(ctx, "tab", cap * sizeof(Slot));
+  Slot* tab = nullptr;
   double* wtab = nullptr;
-  if (p->sample) {
-    wtab = (double*)ctx_buf(ctx, "wtab", cap * sizeof(double));
-    if (!wtab) return LIGHTNE_ERR_OOM;
-  }
+  WsCarve wc;
+  wc.add((void**)&tab, cap * sizeof(Slot));
+  if (p->sample) wc.add((void**)&wtab, cap * sizeof(double));
+  LN_CHECK(wc.commit(ctx));
   auto* src = (uint32_t*)ctx_buf(ctx, "edge_src", std::max<int64_t>(g->m, 1) * sizeof(uint32_t));
   auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
-  if (!tab || !src || !stats) return LIGHTNE_ERR_OOM;
+  if (!src || !stats) return LIGHTNE_ERR_OOM;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
@@ -519,6 +513,113 @@ int table_totals(lightne_ctx* ctx, const Slot* tab, const double* wtab, uint64_t
 
 }  // namespace
 
+// Sampler, table, trunc-log filter and CSR.  ctx_owned: the CSR arrays are
+// context buffers reused by the next call (the pipeline's own use); else the
+// handle owns them.
+int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
+                  lightne_sparsify_stats* stats, bool ctx_owned) {
+  *out = nullptr;
+  lightne_sparsify_stats st{};
+  Slot* tab;
+  double* wtab;
+  uint64_t cap;
+  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  hipStream_t s = ctx->stream;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+  uint64_t distinct, csum;
+  double wsum;
+  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
+  st.distinct_keys = distinct;
+  // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
+  st.M_exact = p->sample ? wsum : (double)csum;
+  const double vol = (double)(uint64_t)g->m;
+  st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+
+  const int colbits = std::max(1, bits_for((uint64_t)g->n));
+  TruncArgs ta;
+  ta.tab = tab;
+  ta.wtab = wtab;
+  ta.cap = cap;
+  ta.off = g->offsets;
+  ta.factor = st.factor;
+  ta.upper = p->upper;
+  ta.colbits = colbits;
+  // contiguous slot ranges, one per block; counts first, then a scan, then the
+  // entries in slot order (deterministic, no contended atomics)
+  const int nblk = 4096;
+  const uint64_t per_block = (cap + nblk - 1) / nblk;
+  auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
+  auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
+  auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
+  if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
+  LN_LAUNCH(ctx);
+  LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+  std::vector<uint32_t> hk(nblk);
+  uint64_t htot = 0;
+  LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  const int64_t nnz = (int64_t)htot;
+  st.kept_upper = 0;
+  for (uint32_t v : hk) st.kept_upper += v;
+  st.nnz = (uint64_t)nnz;
+  const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
+  auto* skey = (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
+  if (!skey) return LIGHTNE_ERR_OOM;
+  DevMem mrp, mcol, mval;
+  int64_t* rp = nullptr;
+  int32_t* col = nullptr;
+  float* val = nullptr;
+  if (ctx_owned) {
+    rp = (int64_t*)ctx_buf(ctx, "s1_rp", (g->n + 1) * 8);
+    col = (int32_t*)ctx_buf(ctx, "s1_col", nz * 4);
+    val = (float*)ctx_buf(ctx, "s1_val", nz * 4);
+    if (!rp || !col || !val) return LIGHTNE_ERR_OOM;
+  } else {
+    LN_CHECK(dev_alloc(ctx, mval, nz * 4, "sparsifier values"));
+    LN_CHECK(dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr"));
+    LN_CHECK(dev_alloc(ctx, mcol, nz * 4, "csr col"));
+    rp = (int64_t*)mrp.p;
+    col = (int32_t*)mcol.p;
+    val = (float*)mval.p;
+  }
+  ta.skey = skey;
+  ta.sval = val;
+  hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
+  LN_LAUNCH(ctx);
+  // the table is dead once the kept entries are out: the sort's ping-pong
+  // buffers reuse its memory when they fit there
+  const size_t ws_bytes = ctx->bufs["ws"].bytes;
+  const size_t kbytes = (nz * 8 + 255) & ~(size_t)255;
+  uint64_t* k_tmp = nullptr;
+  float* v_tmp = nullptr;
+  if (kbytes + nz * 4 <= ws_bytes) {
+    k_tmp = (uint64_t*)tab;
+    v_tmp = (float*)((char*)tab + kbytes);
+  }
+  LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp));
+  hipLaunchKernelGGL(k_csr_finish, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, s, (const uint64_t*)skey,
+                     nnz, g->n, colbits, rp, col);
+  LN_LAUNCH(ctx);
+  LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  auto* csr = new lightne_csr();
+  csr->n = g->n;
+  csr->nnz = nnz;
+  csr->owns = !ctx_owned;
+  csr->row_ptr = ctx_owned ? rp : (int64_t*)mrp.release();
+  csr->col = ctx_owned ? col : (int32_t*)mcol.release();
+  csr->val = ctx_owned ? val : (float*)mval.release();
+  st.t_table = ev_seconds(ctx->ev[3], ctx->ev[4]);
+  st.t_csr = ev_seconds(ctx->ev[4], ctx->ev[5]);
+  *out = csr;
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+
 extern "C" {
 
 int lightne_sample_scalars(int64_t n, int64_t m, const lightne_sparsify_params* p, lightne_sparsify_stats* out) {
@@ -543,6 +644,7 @@ int lightne_sample_scalars(int64_t n, int64_t m, const lightne_sparsify_params* 
 int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
                             lightne_table** out, lightne_sparsify_stats* stats) {
   if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify_counts: NULL argument");
+  CallScope scope(ctx);
   *out = nullptr;
   lightne_sparsify_stats st{};
   Slot* tab;
@@ -615,99 +717,8 @@ int lightne_table_free(lightne_table* t) {
 int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
                      lightne_sparsify_stats* stats) {
   if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify: NULL argument");
-  *out = nullptr;
-  lightne_sparsify_stats st{};
-  Slot* tab;
-  double* wtab;
-  uint64_t cap;
-  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
-  hipStream_t s = ctx->stream;
-  LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-  uint64_t distinct, csum;
-  double wsum;
-  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
-  st.distinct_keys = distinct;
-  // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
-  st.M_exact = p->sample ? wsum : (double)csum;
-  const double vol = (double)(uint64_t)g->m;
-  st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
-  LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-
-  const int colbits = std::max(1, bits_for((uint64_t)g->n));
-  TruncArgs ta;
-  ta.tab = tab;
-  ta.wtab = wtab;
-  ta.cap = cap;
-  ta.off = g->offsets;
-  ta.factor = st.factor;
-  ta.upper = p->upper;
-  ta.colbits = colbits;
-  // contiguous slot ranges, one per block; counts first, then a scan, then the
-  // entries in slot order (deterministic, no contended atomics)
-  const int nblk = 4096;
-  const uint64_t per_block = (cap + nblk - 1) / nblk;
-  auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
-  auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
-  auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
-  if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
-  hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
-  LN_LAUNCH(ctx);
-  LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
-  std::vector<uint32_t> hk(nblk);
-  uint64_t htot = 0;
-  LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  const int64_t nnz = (int64_t)htot;
-  st.kept_upper = 0;
-  for (uint32_t v : hk) st.kept_upper += v;
-  st.nnz = (uint64_t)nnz;
-  DevMem mk, mv;
-  LN_CHECK(dev_alloc(ctx, mk, std::max<int64_t>(nnz, 1) * 8, "sparsifier sort keys"));
-  LN_CHECK(dev_alloc(ctx, mv, std::max<int64_t>(nnz, 1) * 4, "sparsifier values"));
-  ta.skey = (uint64_t*)mk.p;
-  ta.sval = (float*)mv.p;
-  hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
-  LN_LAUNCH(ctx);
-  // the table is the largest buffer of the pipeline: give it back before the
-  // sort allocates its ping-pong buffers
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  ctx_release(ctx, "tab");
-  ctx_release(ctx, "wtab");
-  LN_CHECK(radix_sort_pairs(ctx, (uint64_t*)mk.p, (float*)mv.p, nnz, 2 * colbits));
-
-  auto* csr = new lightne_csr();
-  DevMem mrp, mcol, mval;
-  int rc = dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr");
-  if (rc == LIGHTNE_OK) rc = dev_alloc(ctx, mcol, std::max<int64_t>(nnz, 1) * 4, "csr col");
-  if (rc != LIGHTNE_OK) {
-    delete csr;
-    return rc;
-  }
-  hipLaunchKernelGGL(k_csr_finish, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, s, (const uint64_t*)mk.p,
-                     nnz, g->n, colbits, (int64_t*)mrp.p, (int32_t*)mcol.p);
-  if (hipGetLastError() != hipSuccess) {
-    delete csr;
-    return ctx_fail(ctx, LIGHTNE_ERR_HIP, "k_csr_finish launch failed");
-  }
-  LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  csr->n = g->n;
-  csr->nnz = nnz;
-  csr->row_ptr = (int64_t*)mrp.release();
-  csr->col = (int32_t*)mcol.release();
-  csr->val = (float*)mv.release();
-  st.t_table = ev_seconds(ctx->ev[3], ctx->ev[4]);
-  st.t_csr = ev_seconds(ctx->ev[4], ctx->ev[5]);
-  // sort scratch scales with nnz; later stages need the memory
-  ctx_release(ctx, "rs_keys");
-  ctx_release(ctx, "rs_vals");
-  ctx_release(ctx, "rs_hist");
-  ctx_release(ctx, "rs_offs");
-  ctx_release(ctx, "edge_src");
-  *out = csr;
-  if (stats) *stats = st;
-  return LIGHTNE_OK;
+  CallScope scope(ctx);
+  return sparsify_impl(ctx, g, p, out, stats, false);
 }
 
 }  // extern "C"

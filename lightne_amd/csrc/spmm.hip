@@ -306,10 +306,8 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
   P.nparts = (int64_t)tot[2];
   if (P.nparts >= ((int64_t)1 << 31)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: too many partial chunks");
   auto al = [&](void** dst, size_t bytes) -> int {
-    if (hipMalloc(dst, bytes ? bytes : 16) != hipSuccess) {
-      (void)hipGetLastError();
+    if (ctx_malloc(ctx, dst, bytes ? bytes : 16) != hipSuccess)
       return ctx_fail(ctx, LIGHTNE_ERR_OOM, "spmm plan allocation failed");
-    }
     return LIGHTNE_OK;
   };
   LN_CHECK(al((void**)&P.chunk_start, (2 * P.nchunks + 2) * sizeof(int64_t)));
