@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     # name: (n, beta, mean_deg, max_deg, graph_seed, T, samples_per_directed_edge, k, d, q, K)
     "c2": dict(n=1_140_000, beta=2.0, mean=5.25, maxdeg=2500.0, gseed=2, T=10, per_edge=10.0, k=144, d=128, q=1,
-               K=10),
+               K=10, adjoint=True),
     "c1": dict(n=10_312, beta=2.1, mean=65.0, maxdeg=3900.0, gseed=1, T=10, per_edge=50.0, k=144, d=128, q=1,
                K=10),
     # R-MAT scale 24, edge factor 16, (a, b, c) = (.57, .19, .19); T = 5, M = 10 m, q = 2
@@ -270,19 +270,66 @@ def main():
                 "bytes_per_launch": int(bytes_dom / launches), "t_per_launch_ms": round(1e3 * t_dom / launches, 3)}
     bf16 = None
     if not args.skip_bf16:
-        run_pipeline(L, g, cfg, ratio, L.BF16, ctx)  # warm
-        log("bf16 warmup done")
-        t1 = time.perf_counter()
-        embb, Sb, prob, stb, pstb = run_pipeline(L, g, cfg, ratio, L.BF16, ctx)
-        tb = time.perf_counter() - t1
-        log(f"bf16 step: {tb:.3f}s")
-        # top-d stage-2 subspace against the FP32 run
-        ang = principal_angle(emb, embb)
-        sv = float(np.max(np.abs(Sb[:d] - S[:d]) / S[:d]))
-        bf16 = {"seconds": round(tb, 4), "speedup_vs_fp32": round((ms_per_step / 1e3) / tb, 3),
-                "stage2_angle_vs_fp32_rad": ang, "sv_rel_err_vs_fp32": sv,
-                "spmm_ms_per_call": round(1e3 * stb["rsvd"]["t_spmm"] / max(stb["rsvd"]["spmm_calls"], 1), 3),
-                "prop_cheb_s": pstb["t_cheb"]}
+        # stage 2 against the FP64 form (same sparsifier, same sketch): top-d
+        # principal angle and singular values
+        log("fp64 reference ...")
+        emb64, S64, st64 = L.NetSMF(g, walk_len=cfg["T"], rank=cfg["k"], dim=cfg["d"], sample_ratio=ratio, seed=1,
+                                    svd_seed=777, power_iters=cfg["q"], precision=L.FP64, ctx=ctx)
+        torch.cuda.synchronize()
+        S64 = S64.astype(np.float64)
+
+        def sv_err(Sx):
+            return float(np.max(np.abs(Sx[:d].astype(np.float64) - S64[:d]) / S64[:d]))
+
+        # stage 3 on one input (the FP32 stage-2 embedding): z of the BF16
+        # iterates against z of the FP32 ones, relative Frobenius error
+        _, z32, _ = L.propagate(emb, g, order=cfg["K"], mu=0.2, theta=0.5, precision=L.FP32, ctx=ctx)
+        _, z16, _ = L.propagate(emb, g, order=cfg["K"], mu=0.2, theta=0.5, precision=L.BF16, ctx=ctx)
+        prop_err = float((torch.linalg.vector_norm(z16 - z32) / torch.linalg.vector_norm(z32)).item())
+        del z32, z16
+        bf16 = {"fp32_stage2_angle_vs_fp64_rad": principal_angle(emb, emb64), "fp32_sv_rel_err_vs_fp64": sv_err(S),
+                "spectral_gap_d": float(S64[d - 1] / S64[d]), "stage3_z_rel_err_vs_fp32": prop_err,
+                "fp64_rsvd_s": round(st64["rsvd"]["t_total"], 4)}
+        forms = [("bf16", L.BF16)] + ([("bf16_power", L.BF16_POWER)] if cfg["q"] > 1 else [])
+        for name, prec in forms:
+            run_pipeline(L, g, cfg, ratio, prec, ctx)  # warm
+            t1 = time.perf_counter()
+            embb, Sb, prob, stb, pstb = run_pipeline(L, g, cfg, ratio, prec, ctx)
+            tb = time.perf_counter() - t1
+            log(f"{name} step: {tb:.3f}s")
+            bf16[name] = {"seconds": round(tb, 4), "speedup_vs_fp32": round((ms_per_step / 1e3) / tb, 3),
+                          "stage2_angle_vs_fp64_rad": principal_angle(embb, emb64), "sv_rel_err_vs_fp64": sv_err(Sb),
+                          "spmm_ms_per_call": round(1e3 * stb["rsvd"]["t_spmm"] / max(stb["rsvd"]["spmm_calls"], 1),
+                                                    3),
+                          "prop_cheb_s": pstb["t_cheb"]}
+            del embb, prob
+        del emb64
+    adjoint = None
+    if cfg.get("adjoint"):
+        # differentiation form: forward z = (A + I)(a - sum c_i T_i a) and the
+        # hand-written adjoint (dL/da, dL/dmu, dL/dtheta, checkpoint every 3
+        # steps) for a ~ N(0, 1), dL/dz ~ N(0, 1)
+        gen = torch.Generator(device="cuda:0").manual_seed(4)
+        a = torch.randn(hg.n, d, generator=gen, device="cuda:0")
+        gz = torch.randn(hg.n, d, generator=gen, device="cuda:0")
+        tf, tbk = [], []
+        for it in range(3):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.propagate(a, g, order=cfg["K"], mu=0.2, theta=0.5, ctx=ctx)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            L.propagate_backward(a, gz, g, order=cfg["K"], mu=0.2, theta=0.5, aggregate=True, checkpoint_every=3,
+                                 ctx=ctx)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            if it:  # the first pair warms up
+                tf.append(t2 - t1)
+                tbk.append(t3 - t2)
+        adjoint = {"forward_s": round(float(np.median(tf)), 5), "backward_s": round(float(np.median(tbk)), 5),
+                   "checkpoint_every": 3, "d": d, "K": cfg["K"]}
+        log(f"adjoint: forward {adjoint['forward_s']}s backward {adjoint['backward_s']}s")
+        del a, gz
     cpu = None
     if not args.skip_cpu:
         log("cpu baseline ...")
@@ -300,7 +347,7 @@ def main():
                    "parallelism": "1 GPU"},
         "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
         "sparsifier_nnz": nnz, "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
-        "bf16": bf16, "roofline": roofline, "cpu_baseline": cpu,
+        "bf16": bf16, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -57,6 +57,8 @@ extern "C" {
 /* panel precision */
 #define LIGHTNE_FP32 0
 #define LIGHTNE_BF16 1
+#define LIGHTNE_FP64 2 /* randomized SVD only: FP64 panels, products and Grams (accuracy reference) */
+#define LIGHTNE_BF16_POWER 3 /* as LIGHTNE_BF16, and the randomized SVD's power iterations also gather BF16 */
 
 typedef struct lightne_ctx lightne_ctx;
 typedef struct lightne_table lightne_table;
@@ -148,7 +150,7 @@ int lightne_csr_free(lightne_csr* c);
 typedef struct {
   int64_t rank;             /* -rank: sketch width k (d + oversampling)       */
   int32_t power_iters;      /* q >= 1; 1 = the reference schedule             */
-  int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 SpMM panels        */
+  int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 / LIGHTNE_FP64 panels */
   int32_t random_project_only; /* -random_project_only (mklredsvd.cc:254-258) */
   int32_t upper;            /* A holds only the upper triangle (symmetric)    */
   uint64_t seed;            /* Gaussian sketch seed (reference: VSL seed 777) */

@@ -27,8 +27,8 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (BF16, FP32, Graph, LightNEError, NetsmfStats, PropParams, PropStats, RsvdParams, RsvdStats,
-                   SparsifyParams, SparsifyStats, check, load)
+from ._lib import (BF16, BF16_POWER, FP32, FP64, Graph, LightNEError, NetsmfStats, PropParams, PropStats,
+                   RsvdParams, RsvdStats, SparsifyParams, SparsifyStats, check, load)
 
 
 def _torch():

@@ -70,29 +70,39 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
 }
 
 // ---------------------------------------------------------------------------
-// Gram: rows are staged through LDS in slabs of GR_ROWS; every wave owns a
+// Gram: rows are staged through LDS in 32-KB slabs; every wave owns a
 // fixed subset of the 16x16 output tiles and walks the slab 4 rows at a time.
 // v_mfma_f64_16x16x4f64 operand layout: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; accumulator register v of lane l holds D[(l/16) + 4v][l%16]
 // (rows interleaved, unlike the f32 16x16x4 form whose register v holds row
 // 4(l/16) + v).
 // ---------------------------------------------------------------------------
-constexpr int GR_ROWS = 32;
 constexpr int GR_THREADS = 512;
 constexpr int GR_WAVES = GR_THREADS / 64;
 constexpr int GR_MAX_PAIRS_PER_WAVE = 12;
-constexpr int GR_MAX_F4 = 9;  // float4 per thread for one slab: 32 rows x (256 + 256) / 4 / 512 = 8, +1
+constexpr int GR_MAX_V = 9;  // 16-B vectors per thread for one slab: 2 x 32 KB / 16 B / 512 = 8, +1
 
-// One block owns a contiguous row range; the 32-row slab of X (and Y) is
-// staged in LDS while the next slab is already in flight in registers.
-__global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict__ X, const float* __restrict__ Y,
+// rows per LDS slab: 32 FP32 rows or 16 FP64 rows (the same bytes)
+template <typename TI>
+struct GramRows {
+  static constexpr int R = 128 / sizeof(TI);
+};
+
+// One block owns a contiguous row range; the slab of X (and Y) is staged in
+// LDS while the next slab is already in flight in registers.  TI = float for
+// the FP32 panels, double for the FP64 form.
+template <typename TI>
+__global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ X, const TI* __restrict__ Y,
                                                          int64_t n, int k1, int k2, int npairs,
                                                          const int2* __restrict__ pairs, int64_t rows_per_block,
                                                          double* __restrict__ part) {
-  extern __shared__ float s_mem[];
+  constexpr int ROWS = GramRows<TI>::R;
+  constexpr int E = 16 / sizeof(TI);  // elements per 16-B vector
+  extern __shared__ float4 s_raw[];
+  TI* s_mem = reinterpret_cast<TI*>(s_raw);
   const bool same = X == Y;
-  float* sX = s_mem;
-  float* sY = same ? s_mem : s_mem + GR_ROWS * k1;
+  TI* sX = s_mem;
+  TI* sY = same ? s_mem : s_mem + ROWS * k1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   int2 pr[GR_MAX_PAIRS_PER_WAVE];
@@ -106,39 +116,39 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const float* __restrict
   for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) acc[q] = doublex4{0.0, 0.0, 0.0, 0.0};
   const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r_end = min(n, r_begin + rows_per_block);
-  const int nx4 = GR_ROWS * k1 / 4, ny4 = same ? 0 : GR_ROWS * k2 / 4;
-  const int row4x = k1 / 4, row4y = k2 / 4;
-  float4 buf[GR_MAX_F4];
+  const int nxv = ROWS * k1 / E, nyv = same ? 0 : ROWS * k2 / E;
+  const int rowvx = k1 / E, rowvy = k2 / E;
+  float4 buf[GR_MAX_V];
   auto fetch = [&](int64_t r0) {
-    const int nrow = (int)min<int64_t>(GR_ROWS, r_end - r0);
+    const int nrow = (int)min<int64_t>(ROWS, r_end - r0);
     const float4* gx = reinterpret_cast<const float4*>(X + r0 * k1);
     const float4* gy = reinterpret_cast<const float4*>(Y + r0 * k2);
 #pragma unroll
-    for (int t = 0; t < GR_MAX_F4; ++t) {
+    for (int t = 0; t < GR_MAX_V; ++t) {
       const int f = threadIdx.x + t * GR_THREADS;
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (f < nx4) {
-        if (f / row4x < nrow) v = gx[f];
-      } else if (f < nx4 + ny4) {
-        const int g = f - nx4;
-        if (g / row4y < nrow) v = gy[g];
+      if (f < nxv) {
+        if (f / rowvx < nrow) v = gx[f];
+      } else if (f < nxv + nyv) {
+        const int g = f - nxv;
+        if (g / rowvy < nrow) v = gy[g];
       }
       buf[t] = v;
     }
   };
   if (r_begin < r_end) fetch(r_begin);
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += GR_ROWS) {
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += ROWS) {
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < GR_MAX_F4; ++t) {
+    for (int t = 0; t < GR_MAX_V; ++t) {
       const int f = threadIdx.x + t * GR_THREADS;
-      if (f < nx4 + ny4) reinterpret_cast<float4*>(s_mem)[f] = buf[t];
+      if (f < nxv + nyv) s_raw[f] = buf[t];
     }
     __syncthreads();
-    if (r0 + GR_ROWS < r_end) fetch(r0 + GR_ROWS);
-    for (int s = 0; s < GR_ROWS; s += 4) {
-      const float* xr = sX + (s + lk) * k1 + li;
-      const float* yr = sY + (s + lk) * k2 + li;
+    if (r0 + ROWS < r_end) fetch(r0 + ROWS);
+    for (int s = 0; s < ROWS; s += 4) {
+      const TI* xr = sX + (s + lk) * k1 + li;
+      const TI* yr = sY + (s + lk) * k2 + li;
 #pragma unroll
       for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
         if (wid + q * GR_WAVES < npairs) {
@@ -237,6 +247,64 @@ __global__ __launch_bounds__(PS_THREADS) void k_panel_small(const float* X, int6
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 form: Out(n x k2, double) = X(n x k1, double) W(k1 x k2), W's column
+// block [c0, c0 + 16 nt) in LDS, one wave per 16-row slab on
+// v_mfma_f64_16x16x4f64 (lane l: A = X[row0 + l%16][kk + l/16],
+// B = W[kk + l/16][c + l%16]; register v holds Out[row0 + l/16 + 4v][c + l%16]).
+// ---------------------------------------------------------------------------
+constexpr int P64_MAX_T = 4;
+
+__global__ __launch_bounds__(256) void k_panel64(const double* __restrict__ X, int64_t n, int k1,
+                                                 const double* __restrict__ W, int k2, int c0, int nt,
+                                                 double* __restrict__ Out) {
+  extern __shared__ double sW64[];  // k1 x (16 nt + 1)
+  const int ldw = 16 * nt + 1;
+  for (int e = threadIdx.x; e < k1 * 16 * nt; e += blockDim.x) {
+    const int r = e / (16 * nt), c = e - r * (16 * nt);
+    sW64[r * ldw + c] = W[(int64_t)r * k2 + c0 + c];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int64_t nslab = (n + 15) / 16;
+  const int64_t wave0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t slab = wave0; slab < nslab; slab += nwaves) {
+    const int64_t row = slab * 16 + li;
+    const bool rv = row < n;
+    doublex4 acc[P64_MAX_T];
+#pragma unroll
+    for (int j = 0; j < P64_MAX_T; ++j) acc[j] = doublex4{0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < k1; kk += 4) {
+      const double a = rv ? X[row * k1 + kk + lk] : 0.0;
+      const double* wr = sW64 + (kk + lk) * ldw + li;
+#pragma unroll
+      for (int j = 0; j < P64_MAX_T; ++j)
+        if (j < nt) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wr[16 * j], acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t orow = slab * 16 + lk + 4 * v;
+      if (orow < n) {
+#pragma unroll
+        for (int j = 0; j < P64_MAX_T; ++j)
+          if (j < nt) Out[orow * k2 + c0 + 16 * j + li] = acc[j][v];
+      }
+    }
+  }
+}
+
+__global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ out, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (double)in[i];
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (float)in[i];
+}
+
 __global__ void k_f32_to_bf16(const float* __restrict__ in, __hip_bfloat16* __restrict__ out, int64_t count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -290,8 +358,11 @@ int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t coun
   return LIGHTNE_OK;
 }
 
-int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
-             double* G_host) {
+namespace {
+template <typename TI>
+int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+              double* G_host) {
+  constexpr int ROWS = GramRows<TI>::R;
   if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 256)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: widths must be multiples of 16 in [16, 256]");
   if (symmetric && (X != Y || k1 != k2)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: symmetric needs X == Y");
@@ -304,8 +375,8 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
   const int per_launch = GR_MAX_PAIRS_PER_WAVE * GR_WAVES;
   int nblk = std::max(1, ctx->num_cus);
   int64_t rpb = (n + nblk - 1) / nblk;
-  rpb = ((rpb + GR_ROWS - 1) / GR_ROWS) * GR_ROWS;
-  if (rpb == 0) rpb = GR_ROWS;
+  rpb = ((rpb + ROWS - 1) / ROWS) * ROWS;
+  if (rpb == 0) rpb = ROWS;
   nblk = (int)std::max<int64_t>(1, (n + rpb - 1) / rpb);
   const int64_t len = k1 * k2;
   auto* dpairs = (int2*)ctx_buf(ctx, "gram_pairs", pairs.size() * sizeof(int2));
@@ -313,12 +384,12 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
   auto* gsum = (double*)ctx_buf(ctx, "gram_sum", len * sizeof(double));
   if (!dpairs || !part || !gsum) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
-  const size_t lds = (size_t)GR_ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(float);
+  const size_t lds = (size_t)ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(TI);
   if (n > 0) {
     // each launch owns a disjoint set of output tiles
     for (int p0 = 0; p0 < npairs_all; p0 += per_launch) {
       const int np = std::min(per_launch, npairs_all - p0);
-      hipLaunchKernelGGL(k_gram_f64, dim3(nblk), dim3(GR_THREADS), lds, ctx->stream, X, Y, n, (int)k1, (int)k2, np,
+      hipLaunchKernelGGL(k_gram_f64<TI>, dim3(nblk), dim3(GR_THREADS), lds, ctx->stream, X, Y, n, (int)k1, (int)k2, np,
                          (const int2*)(dpairs + p0), rpb, part);
       LN_LAUNCH(ctx);
     }
@@ -335,6 +406,16 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
     for (int64_t i = 0; i < k1; ++i)
       for (int64_t j = 0; j < i; ++j) G_host[i * k2 + j] = G_host[j * k2 + i];
   return LIGHTNE_OK;
+}
+}  // namespace
+
+int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+             double* G_host) {
+  return gram_impl<float>(ctx, X, Y, n, k1, k2, symmetric, G_host);
+}
+int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+             double* G_host) {
+  return gram_impl<double>(ctx, X, Y, n, k1, k2, symmetric, G_host);
 }
 
 int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,
@@ -386,6 +467,41 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
   }
   if (dst != Out)
     LN_HIP(ctx, hipMemcpyAsync(Out, dst, (size_t)n * k2 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  return LIGHTNE_OK;
+}
+
+int panel_times_small64(lightne_ctx* ctx, const double* X, int64_t n, int64_t k1, const double* W_host, int64_t k2,
+                        double* Out) {
+  if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 256 || Out == X)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small64: bad shape or in-place call");
+  auto* dW = (double*)ctx_buf(ctx, "small_w64", (size_t)k1 * k2 * sizeof(double));
+  if (!dW) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemcpyAsync(dW, W_host, (size_t)k1 * k2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));  // W_host may be a caller's temporary
+  if (n <= 0) return LIGHTNE_OK;
+  const int64_t nslab = (n + 15) / 16;
+  const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((nslab + 3) / 4, (int64_t)ctx->num_cus * 8));
+  for (int c0 = 0; c0 < k2; c0 += 16 * P64_MAX_T) {
+    const int nt = (int)std::min<int64_t>(P64_MAX_T, (k2 - c0) / 16);
+    const size_t lds = (size_t)k1 * (16 * nt + 1) * sizeof(double);
+    hipLaunchKernelGGL(k_panel64, dim3((unsigned)nblk), dim3(256), lds, ctx->stream, X, n, (int)k1,
+                       (const double*)dW, (int)k2, c0, nt, Out);
+    LN_LAUNCH(ctx);
+  }
+  return LIGHTNE_OK;
+}
+
+int f32_to_f64(lightne_ctx* ctx, const float* in, double* out, int64_t count) {
+  if (count <= 0) return LIGHTNE_OK;
+  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}
+
+int f64_to_f32(lightne_ctx* ctx, const double* in, float* out, int64_t count) {
+  if (count <= 0) return LIGHTNE_OK;
+  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
 

@@ -18,7 +18,8 @@
 //   -seed S        pbbs::random seed of the sampler (reference: time(0))
 //   -svd_seed S    Gaussian sketch seed (reference: VSL seed 777)
 //   -power_iters q randomized-SVD power iterations (1 = reference schedule)
-//   -precision fp32|bf16   SpMM panels / propagation iterates
+//   -precision fp32|bf16|bf16_power|fp64   SpMM panels / propagation iterates
+//                          (fp64: randomized SVD in FP64, propagation FP32)
 //   -mu X -theta X propagation parameters (reference hard-codes 0.2, 0.5)
 //   -ne_in FILE    read the stage-2 embedding (raw FP32 n x dim) instead of
 //                  computing it, so stage 3 can run alone
@@ -193,8 +194,8 @@ int main(int argc, char** argv) {
     fprintf(stderr, "lightne: -sparse_project is not available in this build\n");
     return 2;
   }
-  if (precision != "fp32" && precision != "bf16") {
-    fprintf(stderr, "lightne: -precision must be fp32 or bf16\n");
+  if (precision != "fp32" && precision != "bf16" && precision != "bf16_power" && precision != "fp64") {
+    fprintf(stderr, "lightne: -precision must be fp32, bf16, bf16_power or fp64\n");
     return 2;
   }
 
@@ -257,7 +258,10 @@ int main(int argc, char** argv) {
   double total = 0.0;
   for (size_t r = 0; r < rounds; ++r) {
     const double t0 = now();
-    const int prec = precision == "bf16" ? LIGHTNE_BF16 : LIGHTNE_FP32;
+    const int prec = precision == "bf16"         ? LIGHTNE_BF16
+                     : precision == "bf16_power" ? LIGHTNE_BF16_POWER
+                     : precision == "fp64"       ? LIGHTNE_FP64
+                                                 : LIGHTNE_FP32;
     if (!ne_in.empty()) {
       std::vector<float> h((size_t)n * dim);
       FILE* f = fopen(ne_in.c_str(), "rb");
@@ -319,7 +323,7 @@ int main(int argc, char** argv) {
       pp.order = (int64_t)order;
       pp.mu = mu;
       pp.theta = theta;
-      pp.precision = prec;
+      pp.precision = prec == LIGHTNE_FP64 ? LIGHTNE_FP32 : prec;
       lightne_prop_stats ps{};
       CK(lightne_chebyshev_expansion(ctx, &g, emb, (int64_t)dim, &pp, pro, &ps), ctx);
       printf("proX time: %.6f\n", now() - tp);

@@ -169,6 +169,14 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
 int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
                      int normalize, float* emb);
 int f32_to_bf16(lightne_ctx* ctx, const float* in, __hip_bfloat16* out, int64_t count);
+// FP64 form of the randomized SVD: Gram of FP64 panels, Out = X W (Out != X),
+// and the conversions at its ends
+int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
+             double* G_host);
+int panel_times_small64(lightne_ctx* ctx, const double* X, int64_t n, int64_t k1, const double* W_host, int64_t k2,
+                        double* Out);
+int f32_to_f64(lightne_ctx* ctx, const float* in, double* out, int64_t count);
+int f64_to_f32(lightne_ctx* ctx, const double* in, float* out, int64_t count);
 
 // ---------------------------------------------------------------------------
 // host FP64 small linear algebra (linalg_host.cpp)

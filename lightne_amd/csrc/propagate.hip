@@ -48,8 +48,8 @@ int check_prop(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t
   if (g->n <= 0 || !g->offsets || (g->m > 0 && !g->edges)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: bad graph");
   if (dim <= 0 || dim % 16 || dim > 1024)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: dim must be a multiple of 16 in [16, 1024]");
-  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: unknown precision");
+  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_BF16_POWER)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: FP32, BF16 or BF16_POWER iterates");
   return LIGHTNE_OK;
 }
 
@@ -80,7 +80,7 @@ size_t prop_ws_bytes(int64_t n, int64_t dim, bool bf) {
 int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                       const lightne_prop_params* p, float* y, float* z, lightne_prop_stats* st, SpmmPlan& P) {
   const int64_t n = g->n, K = std::max<int64_t>(p->order, 2);
-  const bool bf = p->precision == LIGHTNE_BF16;
+  const bool bf = p->precision != LIGHTNE_FP32;
   const int xt = bf ? PANEL_BF16 : PANEL_F32;
   const size_t esz = bf ? 2 : 4;
   const size_t pe = (size_t)n * dim;
@@ -247,7 +247,7 @@ int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
   float* y = y_out;
   if (!y) {
-    WsCarve wc(prop_ws_bytes(g->n, dim, p->precision == LIGHTNE_BF16));
+    WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
     wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
     LN_CHECK(wc.commit(ctx));
   }
@@ -278,7 +278,7 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
   // y, z and the tall-SVD panel after the recurrence's own workspace
   float *y = nullptr, *z = nullptr, *q = nullptr;
-  WsCarve wc(prop_ws_bytes(g->n, dim, p->precision == LIGHTNE_BF16));
+  WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
   wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
   wc.add((void**)&z, (size_t)g->n * dim * sizeof(float));
   wc.add((void**)&q, (size_t)g->n * dim * sizeof(float));

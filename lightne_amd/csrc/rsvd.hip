@@ -13,9 +13,14 @@
 //   C = Z^T B                        (:327-350)   FP64 Gram kernel
 //   C = U_C S V^T                    (:352-377)   host FP64 eig(C C^T)
 //   matU = Z U_C                     (:379-400)
-// In the BF16 form the sketch Omega and the power-iteration panels are
-// gathered as BF16 with FP32 accumulation; the last product B = A Q gathers
-// FP32 (refinement pass); every other panel stays FP32 and the Grams FP64.
+// In the BF16 form the sketch Omega, whose values are BF16-representable, is
+// gathered as BF16 with FP32 accumulation: the product is the FP32 form's,
+// bit for bit, at half the gathered bytes.  The power iterations gather the
+// FP32 Q: a BF16 rounding of Q (2^-9) is amplified by 1/(sigma_d/sigma_{d+1}
+// - 1) in the top-d subspace, ~5000 on an R-MAT scale-24 sparsifier.  The
+// BF16_POWER form also gathers the power-iteration panels as BF16 (all but the
+// last product B = A Q, the FP32 refinement pass).  Grams are FP64 in every
+// form.
 #include "lightne_internal.h"
 #include "spmm_epi.h"
 
@@ -55,11 +60,37 @@ int csr_mm(lightne_ctx* ctx, lightne_csr* A, const void* X, int xt, int64_t k, f
   return spmm_run(ctx, *A->plan, a, xt, EPI_STORE_F32, false);
 }
 
+// Y(double) = A X(double), the FP64 form
+int csr_mm64(lightne_ctx* ctx, lightne_csr* A, const double* X, int64_t k, double* Y) {
+  LN_CHECK(csr_plan(ctx, A));
+  SpmmArgs a;
+  a.n = A->n;
+  a.k = k;
+  a.rp = A->row_ptr;
+  a.col = A->col;
+  a.val = A->val;
+  a.X = X;
+  a.out = Y;
+  return spmm_run64(ctx, *A->plan, a);
+}
+
+// Y <- Y W for a square host W (in place for FP32 panels; FP64 panels go
+// through tmp)
+int apply_small(lightne_ctx* ctx, float* Y, float*, int64_t n, int64_t k, const double* W) {
+  return panel_times_small(ctx, Y, n, k, k, W, k, Y, 0, nullptr);
+}
+int apply_small(lightne_ctx* ctx, double* Y, double* tmp, int64_t n, int64_t k, const double* W) {
+  LN_CHECK(panel_times_small64(ctx, Y, n, k, W, k, tmp));
+  LN_HIP(ctx, hipMemcpyAsync(Y, tmp, (size_t)n * k * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  return LIGHTNE_OK;
+}
+
 }  // namespace
 
-int cholqr2(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_out_host, int* passes_out) {
+template <typename T>
+int cholqr2_t(lightne_ctx* ctx, T* Y, T* tmp, int64_t n, int64_t k, double* R_out_host, int* passes_out) {
   const int kk = (int)k;
-  std::vector<double> G(k * k), Gs(k * k), Rs(k * k), R(k * k), Rinv(k * k), Rtot(k * k, 0.0), tmp(k * k);
+  std::vector<double> G(k * k), Gs(k * k), Rs(k * k), R(k * k), Rinv(k * k), Rtot(k * k, 0.0), Rnext(k * k);
   for (int i = 0; i < kk; ++i) Rtot[i * k + i] = 1.0;
   int passes = 0;
   for (int pass = 0; pass < 4; ++pass) {
@@ -93,15 +124,19 @@ int cholqr2(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_out_host
     for (int i = 0; i < kk; ++i)
       for (int j = 0; j < kk; ++j) R[i * k + j] = Rs[i * k + j] / d[j];
     tri_upper_inverse(R.data(), kk, Rinv.data());
-    LN_CHECK(panel_times_small(ctx, Y, n, k, k, Rinv.data(), k, Y, 0, nullptr));
-    matmul_host(R.data(), Rtot.data(), kk, kk, kk, tmp.data());
-    Rtot.swap(tmp);
+    LN_CHECK(apply_small(ctx, Y, tmp, n, k, Rinv.data()));
+    matmul_host(R.data(), Rtot.data(), kk, kk, kk, Rnext.data());
+    Rtot.swap(Rnext);
     ++passes;
     if (pass >= 1 && dev < 1e-1) break;  // input was already close to orthonormal
   }
   if (R_out_host) std::copy(Rtot.begin(), Rtot.end(), R_out_host);
   if (passes_out) *passes_out = passes;
   return LIGHTNE_OK;
+}
+
+int cholqr2(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_out_host, int* passes_out) {
+  return cholqr2_t<float>(ctx, Y, nullptr, n, k, R_out_host, passes_out);
 }
 
 extern "C" {
@@ -174,6 +209,87 @@ int lightne_csr_free(lightne_csr* c) {
   return LIGHTNE_OK;
 }
 
+// FP64 form of the schedule above: every panel double, the same sketch (its
+// values are BF16-representable, hence exact in double), FP64 products and
+// Grams; U is returned rounded to FP32.  The accuracy reference for the FP32
+// and BF16 forms at sizes the host oracle cannot reach.
+static int rsvd64(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p, int q, float* U_out, float* S_host,
+                  lightne_rsvd_stats* stats) {
+  const int64_t n = A->n, k = p->rank;
+  hipStream_t s = ctx->stream;
+  const size_t pd = (size_t)n * k * sizeof(double);
+  double *P0 = nullptr, *P1 = nullptr, *P2 = nullptr, *T = nullptr;  // Omega then Z; Y / Q; B; scratch
+  {
+    WsCarve wc;
+    wc.add((void**)&P0, pd);
+    wc.add((void**)&P1, pd);
+    wc.add((void**)&P2, pd);
+    wc.add((void**)&T, pd);
+    LN_CHECK(wc.commit(ctx));
+  }
+  lightne_rsvd_stats st{};
+  hipEvent_t* e = ctx->ev;
+  auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
+  double t_spmm = 0, t_qr = 0;
+  LN_HIP(ctx, hipEventRecord(e[0], s));
+  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, (float*)T));
+  LN_CHECK(f32_to_f64(ctx, (const float*)T, P0, n * k));
+  LN_HIP(ctx, hipEventRecord(e[1], s));
+  LN_CHECK(csr_plan(ctx, A));
+  LN_HIP(ctx, hipEventRecord(e[2], s));
+  LN_CHECK(csr_mm64(ctx, A, P0, k, P1));
+  LN_HIP(ctx, hipEventRecord(e[3], s));
+  LN_HIP(ctx, hipEventSynchronize(e[3]));
+  st.t_sketch = lap(e[0], e[1]);
+  t_spmm += lap(e[2], e[3]);
+  st.spmm_calls = 1;
+  if (p->random_project_only) {
+    LN_CHECK(f64_to_f32(ctx, P1, U_out, n * k));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+    st.t_spmm = t_spmm;
+    st.t_total = lap(e[0], e[3]);
+    if (stats) *stats = st;
+    return LIGHTNE_OK;
+  }
+  int passes = 0;
+  for (int it = 0; it < q; ++it) {
+    LN_HIP(ctx, hipEventRecord(e[4], s));
+    LN_CHECK(cholqr2_t<double>(ctx, P1, T, n, k, nullptr, &passes));
+    st.qr_calls++;
+    LN_HIP(ctx, hipEventRecord(e[5], s));
+    LN_CHECK(csr_mm64(ctx, A, P1, k, P2));
+    st.spmm_calls++;
+    LN_HIP(ctx, hipEventRecord(e[6], s));
+    LN_HIP(ctx, hipEventSynchronize(e[6]));
+    t_qr += lap(e[4], e[5]);
+    t_spmm += lap(e[5], e[6]);
+    if (it + 1 < q) std::swap(P1, P2);
+  }
+  std::vector<float> Pf(k * k);
+  gaussian_fill_host(p->seed, 1u, k * k, Pf.data());
+  std::vector<double> Pd(Pf.begin(), Pf.end());
+  LN_CHECK(panel_times_small64(ctx, P2, n, k, Pd.data(), k, P0));  // Z = B P
+  LN_CHECK(cholqr2_t<double>(ctx, P0, T, n, k, nullptr, &passes));
+  st.qr_calls++;
+  std::vector<double> C(k * k), Uc(k * k), Sc(k);
+  LN_CHECK(gram_f64(ctx, P0, P2, n, k, k, 0, C.data()));  // C = Z^T B
+  auto t0 = std::chrono::steady_clock::now();
+  if (!svd_left_via_eig(C.data(), (int)k, Uc.data(), Sc.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "lightne_rsvd: small SVD did not converge");
+  st.t_small_svd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  LN_CHECK(panel_times_small64(ctx, P0, n, k, Uc.data(), k, T));  // matU = Z U_C
+  LN_CHECK(f64_to_f32(ctx, T, U_out, n * k));
+  LN_HIP(ctx, hipEventRecord(e[12], s));
+  LN_HIP(ctx, hipEventSynchronize(e[12]));
+  if (S_host)
+    for (int64_t j = 0; j < k; ++j) S_host[j] = (float)Sc[j];
+  st.t_spmm = t_spmm;
+  st.t_qr = t_qr;
+  st.t_total = lap(e[0], e[12]);
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
 int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_params* p, float* U_out, float* S_host,
                  lightne_rsvd_stats* stats) {
   if (!ctx || !Ac || !p || !U_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: NULL argument");
@@ -184,14 +300,17 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
                     (long long)k);
   if (k > n) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank exceeds n");
   if (p->upper) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: upper-triangle storage is not supported");
-  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16)
+  if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_FP64 &&
+      p->precision != LIGHTNE_BF16_POWER)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
   const int q = std::max(1, (int)p->power_iters);
   CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
+  if (p->precision == LIGHTNE_FP64) return rsvd64(ctx, A, p, q, U_out, S_host, stats);
   lightne_rsvd_stats st{};
   hipStream_t s = ctx->stream;
-  const bool bf = p->precision == LIGHTNE_BF16;
+  const bool bf = p->precision == LIGHTNE_BF16 || p->precision == LIGHTNE_BF16_POWER;
+  const bool bf_power = p->precision == LIGHTNE_BF16_POWER;
   const size_t pf = (size_t)n * k * sizeof(float);
   float *P0 = nullptr, *P1 = nullptr, *P2 = nullptr;  // Omega then Z; Y / Q; B
   __hip_bfloat16* PB = nullptr;
@@ -233,10 +352,10 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
     LN_CHECK(cholqr2(ctx, P1, n, k, nullptr, &passes));
     st.qr_calls++;
     LN_HIP(ctx, hipEventRecord(e[5], s));
-    // power iterations gather BF16 in the BF16 form; the last product B = A Q
-    // gathers the FP32 Q (the refinement pass that keeps C = Z^T A Q exact for
-    // the orthonormal basis Q)
-    const bool gather_bf = bf && (it + 1 < q);
+    // power iterations gather BF16 in the BF16_POWER form; the last product
+    // B = A Q gathers the FP32 Q (the refinement pass that keeps C = Z^T A Q
+    // exact for the orthonormal basis Q)
+    const bool gather_bf = bf_power && (it + 1 < q);
     if (gather_bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
     LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2));
     st.spmm_calls++;

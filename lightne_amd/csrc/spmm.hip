@@ -80,31 +80,40 @@ __device__ __forceinline__ void load16<__hip_bfloat16>(const __hip_bfloat16* __r
   }
 }
 
+// Rows are walked GATHER_U nonzeros at a time so that each lane has
+// GATHER_U independent row loads in flight; products are added one nonzero at
+// a time in index order whatever GATHER_U is, so the FP32 and BF16 panels
+// give bit-identical sums for BF16-representable inputs.
+constexpr int GATHER_U = 2;
+
 template <typename XT, bool IMPL>
 __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_t j1, int64_t stride, int sub,
                                            float* acc) {
   const XT* __restrict__ X = reinterpret_cast<const XT*>(a.X);
   const int G = a.G;
   int64_t j = j0;
-  for (; j + stride < j1; j += 2 * stride) {
-    int64_t c0, c1;
-    float w0 = 1.f, w1 = 1.f;
-    if (IMPL) {
-      c0 = a.gcol[j];
-      c1 = a.gcol[j + stride];
-    } else {
-      c0 = a.col[j];
-      c1 = a.col[j + stride];
-      w0 = a.val[j];
-      w1 = a.val[j + stride];
-    }
-    float x0[16], x1[16];
-    load16<XT>(X + c0 * a.k, sub, G, x0);
-    load16<XT>(X + c1 * a.k, sub, G, x1);
+  for (; j + (GATHER_U - 1) * stride < j1; j += GATHER_U * stride) {
+    int64_t c[GATHER_U];
+    float w[GATHER_U];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i] + w1 * x1[i];
+    for (int u = 0; u < GATHER_U; ++u) {
+      if (IMPL) {
+        c[u] = a.gcol[j + u * stride];
+        w[u] = 1.f;
+      } else {
+        c[u] = a.col[j + u * stride];
+        w[u] = a.val[j + u * stride];
+      }
+    }
+    float x[GATHER_U][16];
+#pragma unroll
+    for (int u = 0; u < GATHER_U; ++u) load16<XT>(X + c[u] * a.k, sub, G, x[u]);
+#pragma unroll
+    for (int u = 0; u < GATHER_U; ++u)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] += w[u] * x[u][i];
   }
-  if (j < j1) {
+  for (; j < j1; j += stride) {
     int64_t c0;
     float w0 = 1.f;
     if (IMPL) {
@@ -223,6 +232,87 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
     LN_LAUNCH(ctx);
   }
   return LIGHTNE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 form (explicit CSR, Y = A X with X, Y double): the same work split and
+// deterministic reduction; a lane owns 16 columns as 8 chunks of 2 doubles,
+// chunk q at column 2 sub + 2 G q.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void gather_row64(const SpmmArgs& a, int64_t j0, int64_t j1, int64_t stride, int sub,
+                                             double* acc) {
+  const double* __restrict__ X = reinterpret_cast<const double*>(a.X);
+  const int G = a.G;
+  for (int64_t j = j0; j < j1; j += stride) {
+    const double w = (double)a.val[j];
+    const double* xr = X + (int64_t)a.col[j] * a.k + 2 * sub;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double2 t = *reinterpret_cast<const double2*>(xr + 2 * G * q);
+      acc[2 * q] += w * t.x;
+      acc[2 * q + 1] += w * t.y;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_row64(double* __restrict__ out, int64_t k, int G, int sub, const double* acc) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    *reinterpret_cast<double2*>(out + 2 * sub + 2 * G * q) = make_double2(acc[2 * q], acc[2 * q + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_spmm64(SpmmArgs a, int64_t w_short) {
+  const int lane = threadIdx.x & 63;
+  const int G = a.G, NG = 64 / G;
+  const int g = lane / G, sub = lane - g * G;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  double acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+  double* out = reinterpret_cast<double*>(a.out);
+  auto* part = reinterpret_cast<double*>(a.partial);
+  if (wave < w_short) {
+    if (g >= NG) return;
+    const int64_t r = wave * NG + g;
+    if (r >= a.n) return;
+    const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
+    if (j1 - j0 > a.short_nnz) return;
+    gather_row64(a, j0, j1, 1, sub, acc);
+    store_row64(out + r * a.k, a.k, G, sub, acc);
+    return;
+  }
+  const int64_t c = wave - w_short;
+  if (c >= a.nchunks) return;  // uniform across the wave
+  if (g < NG) gather_row64(a, a.chunk_start[2 * c] + g, a.chunk_start[2 * c + 1], NG, sub, acc);
+  for (int q = 1; q < NG; ++q) {
+    const int src = (lane + q * G) & 63;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double t = __shfl(acc[i], src, 64);
+      if (lane < G) acc[i] += t;
+    }
+  }
+  if (lane >= G) return;
+  const int slot = a.chunk_slot[c];
+  if (slot < 0)
+    store_row64(out + (int64_t)a.chunk_row[c] * a.k, a.k, G, sub, acc);
+  else
+    store_row64(part + (int64_t)slot * a.k, a.k, G, sub, acc);
+}
+
+__global__ __launch_bounds__(256) void k_spmm64_fixup(SpmmArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (w >= a.nmulti) return;
+  const int64_t row = a.multi_row[w];
+  const int64_t s0 = a.multi_first[w], s1 = a.multi_first[w + 1];
+  const auto* part = reinterpret_cast<const double*>(a.partial);
+  double* out = reinterpret_cast<double*>(a.out) + row * a.k;
+  for (int c = lane; c < a.k; c += 64) {
+    double s = 0.0;
+    for (int64_t sl = s0; sl < s1; ++sl) s += part[sl * a.k + c];
+    out[c] = s;
+  }
 }
 
 }  // namespace
@@ -350,4 +440,33 @@ int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, b
     LN_DISPATCH(__hip_bfloat16)
   }
   return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "unknown SpMM epilogue %d", epi);
+}
+
+int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
+  if (a.k % 16 != 0 || a.k < 16 || a.k > 1024)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel width %lld must be a multiple of 16 in [16, 1024]",
+                    (long long)a.k);
+  a.short_nnz = P.short_nnz;
+  a.nchunks = P.nchunks;
+  a.chunk_start = P.chunk_start;
+  a.chunk_row = P.chunk_row;
+  a.chunk_slot = P.chunk_slot;
+  a.nmulti = P.nmulti;
+  a.multi_row = P.multi_row;
+  a.multi_first = P.multi_first;
+  a.G = (int)(a.k / 16);
+  const int NG = 64 / a.G;
+  if (P.nparts > 0) {
+    a.partial = (float*)ctx_buf(ctx, "spmm_partial64", (size_t)P.nparts * a.k * sizeof(double));
+    if (!a.partial) return LIGHTNE_ERR_OOM;
+  }
+  const int64_t waves_short = (a.n + NG - 1) / NG;
+  const int64_t waves = waves_short + P.nchunks;
+  hipLaunchKernelGGL(k_spmm64, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a, waves_short);
+  LN_LAUNCH(ctx);
+  if (P.nmulti > 0) {
+    hipLaunchKernelGGL(k_spmm64_fixup, dim3((unsigned)((P.nmulti + 3) / 4)), dim3(256), 0, ctx->stream, a);
+    LN_LAUNCH(ctx);
+  }
+  return LIGHTNE_OK;
 }

@@ -62,6 +62,8 @@ struct SpmmArgs {
 int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P);
 void spmm_plan_free(SpmmPlan& P);
 int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, bool impl);
+// FP64 form: out(double) = A X(double), explicit CSR values
+int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a);
 
 #ifdef __HIPCC__
 

@@ -122,26 +122,35 @@ def test_rsvd_c1_matches_fp64_oracle(gpu, q):
 
 
 @pytest.mark.parametrize("q", [1, 2])
-def test_rsvd_bf16_panels_error_vs_fp64(gpu, q):
-    """BF16 form: the sketch and power-iteration panels are gathered as BF16, the
-    last product B = A Q in FP32.  Same sketch as the FP64 run (drawn at BF16
-    precision).  Bar: top-d principal angle <= 1e-3 rad."""
+def test_rsvd_bf16_form_is_the_fp32_result(gpu, q):
+    """BF16 form: only the BF16-representable sketch is gathered as BF16, with
+    FP32 accumulation in the same order as the FP32 form, so the result is the
+    FP32 form's bit for bit (and within the FP64 bars)."""
+    hg, csr = _c1_sparsifier(gpu)
+    k = 144
+    svd = L.MKLRedSVD(csr, k, power_iters=q, precision=L.BF16, seed=777, ctx=gpu).run()
+    f32 = L.MKLRedSVD(csr, k, power_iters=q, seed=777, ctx=gpu).run()
+    np.testing.assert_array_equal(svd.S, f32.S)
+    np.testing.assert_array_equal(svd.matU.cpu().numpy(), f32.matU.cpu().numpy())
+
+
+@pytest.mark.parametrize("q", [1, 2])
+def test_rsvd_bf16_power_error_vs_fp64(gpu, q):
+    """BF16_POWER form: the power-iteration panels are gathered as BF16 too,
+    the last product B = A Q in FP32.  Same sketch as the FP64 run."""
     hg, csr = _c1_sparsifier(gpu)
     k, d = 144, 128
-    svd = L.MKLRedSVD(csr, k, power_iters=q, precision=L.BF16, seed=777, ctx=gpu).run()
+    svd = L.MKLRedSVD(csr, k, power_iters=q, precision=L.BF16_POWER, seed=777, ctx=gpu).run()
     rp, col, val = csr.to_numpy()
     U64, S64 = O.rsvd(rp, col, val, k, seed=777, power_iters=q)
     rel = np.abs(svd.S[:d] - S64[:d]) / S64[:d]
     ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d], U64[:, :d])
     gap = S64[d - 1] / S64[d]
-    print(f"BF16 q={q}: max rel sv err={rel.max():.2e} angle={ang:.2e} gap={gap:.4f}")
+    print(f"BF16_POWER q={q}: max rel sv err={rel.max():.2e} angle={ang:.2e} gap={gap:.4f}")
     assert rel.max() <= 1e-4
-    # q = 1 gathers BF16 only for the BF16-exact sketch (bar 1e-3 rad); a BF16
-    # power iteration perturbs Q by ~2^-9, amplified by 1/(gap - 1) ~ 800 here
+    # q = 1 has no power iteration (the bar is the FP32 one); a BF16 power
+    # iteration perturbs Q by ~2^-9, amplified by 1/(gap - 1) ~ 800 here
     assert ang <= (1e-3 if q == 1 else 1e-2)
-    if q == 1:  # one BF16 gather of a BF16-exact sketch: identical to FP32
-        f32 = L.MKLRedSVD(csr, k, power_iters=1, seed=777, ctx=gpu).run()
-        np.testing.assert_array_equal(svd.S, f32.S)
 
 
 def test_netsmf_end_to_end(gpu):
@@ -155,3 +164,38 @@ def test_netsmf_end_to_end(gpu):
     np.testing.assert_array_equal(emb.cpu().numpy(), ref.cpu().numpy())
     np.testing.assert_array_equal(S, svd.S)
     assert st["t_total"] > 0
+
+
+def test_spmm_fp64_sketch_matches_oracle(gpu):
+    """FP64 form: Y = A Omega with FP64 panels and accumulation (short, long
+    and multi-chunk rows) against the oracle's FP64 product."""
+    n, k = 6000, 144
+    rp, col, val = _csr_np(_power_law_sym(n, 3))
+    csr = L.CSR.from_arrays(rp, col, val, ctx=gpu)
+    Y = L.MKLRedSVD(csr, k, random_project_only=True, precision=L.FP64, seed=9, ctx=gpu).run().matU.cpu().numpy()
+    Om = O.gaussian(9, 0, n * k).reshape(n, k).astype(np.float64)
+    ref = O.spmm_f64(rp, col, val, Om)
+    # the FP64 result is returned rounded to FP32: within one FP32 ulp of the
+    # oracle's product rounded the same way (a lost FP64 digit would not be)
+    r32 = ref.astype(np.float32)
+    big = np.abs(ref) > 1e-6 * np.abs(ref).max()
+    ulp = np.abs(Y.view(np.int32).astype(np.int64) - r32.view(np.int32).astype(np.int64))[big]
+    assert ulp.max() <= 1, ulp.max()
+    assert np.abs(Y - ref)[~big].max(initial=0.0) <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("q", [1, 2])
+def test_rsvd_fp64_form_matches_oracle(gpu, q):
+    """The FP64 form is the accuracy reference at sizes the host oracle cannot
+    reach: on config 1 it agrees with the oracle's FP64 run (same sketch) to
+    FP64 rounding amplified by the spectral gap, and to FP32 rounding of U."""
+    hg, csr = _c1_sparsifier(gpu)
+    k, d = 144, 128
+    svd = L.MKLRedSVD(csr, k, power_iters=q, precision=L.FP64, seed=777, ctx=gpu).run()
+    rp, col, val = csr.to_numpy()
+    U64, S64 = O.rsvd(rp, col, val, k, seed=777, power_iters=q)
+    rel = np.abs(svd.S[:d].astype(np.float64) - S64[:d]) / S64[:d]
+    ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d].astype(np.float64), U64[:, :d])
+    print(f"FP64 form q={q}: max rel sv err={rel.max():.2e} angle={ang:.2e}")
+    assert rel.max() <= 1e-7  # S is returned as FP32
+    assert ang <= 1e-5
