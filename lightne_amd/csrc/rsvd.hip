@@ -290,6 +290,108 @@ static int rsvd64(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p
   return LIGHTNE_OK;
 }
 
+// FP32 / BF16 schedule up to B = A Q and the eigendecomposition of its FP64
+// Gram.  On return *B_out is B (n x k, in the workspace) with S (descending)
+// and V (k x k, row-major) such that B^T B = V diag(S^2) V^T; with
+// random_project_only *B_out is Y = A Omega and S, V are empty.
+static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p, int q, const float** B_out,
+                     std::vector<double>& S, std::vector<double>& V, lightne_rsvd_stats* stp) {
+  const int64_t n = A->n, k = p->rank;
+  lightne_rsvd_stats& st = *stp;
+  st = lightne_rsvd_stats{};
+  hipStream_t s = ctx->stream;
+  const bool bf = p->precision == LIGHTNE_BF16 || p->precision == LIGHTNE_BF16_POWER;
+  const bool bf_power = p->precision == LIGHTNE_BF16_POWER;
+  const size_t pf = (size_t)n * k * sizeof(float);
+  float *P1 = nullptr, *P2 = nullptr;  // Y / Q; Omega (FP32), then B
+  __hip_bfloat16* PB = nullptr;
+  {
+    WsCarve wc;
+    wc.add((void**)&P1, pf);
+    wc.add((void**)&P2, pf);
+    if (bf) wc.add((void**)&PB, (size_t)n * k * 2);
+    LN_CHECK(wc.commit(ctx));
+  }
+  hipEvent_t* e = ctx->ev;
+  double t_spmm = 0, t_qr = 0, t_gemm = 0;
+  auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
+
+  LN_HIP(ctx, hipEventRecord(e[0], s));
+  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P2));
+  if (bf) LN_CHECK(f32_to_bf16(ctx, P2, PB, n * k));
+  LN_HIP(ctx, hipEventRecord(e[1], s));
+  LN_CHECK(csr_plan(ctx, A));
+  LN_HIP(ctx, hipEventRecord(e[2], s));
+  LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1));
+  LN_HIP(ctx, hipEventRecord(e[3], s));
+  LN_HIP(ctx, hipEventSynchronize(e[3]));
+  st.t_sketch = lap(e[0], e[1]);
+  t_spmm += lap(e[2], e[3]);
+  st.spmm_calls = 1;
+  if (p->random_project_only) {
+    *B_out = P1;
+    S.clear();
+    V.clear();
+    st.t_spmm = t_spmm;
+    st.t_total = lap(e[0], e[3]);
+    return LIGHTNE_OK;
+  }
+  int passes = 0;
+  for (int it = 0; it < q; ++it) {
+    LN_HIP(ctx, hipEventRecord(e[4], s));
+    LN_CHECK(cholqr2(ctx, P1, n, k, nullptr, &passes));
+    st.qr_calls++;
+    LN_HIP(ctx, hipEventRecord(e[5], s));
+    // power iterations gather BF16 in the BF16_POWER form; the last product
+    // B = A Q gathers the FP32 Q (the refinement pass)
+    const bool gather_bf = bf_power && (it + 1 < q);
+    if (gather_bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
+    LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2));
+    st.spmm_calls++;
+    LN_HIP(ctx, hipEventRecord(e[6], s));
+    LN_HIP(ctx, hipEventSynchronize(e[6]));
+    t_qr += lap(e[4], e[5]);
+    t_spmm += lap(e[5], e[6]);
+    if (it + 1 < q) std::swap(P1, P2);  // next pass orthonormalises A Q
+  }
+  // Thin SVD of B: B^T B = V diag(S^2) V^T (FP64 Gram, host FP64 symmetric
+  // eigensolver), U = B V S^-1.  The reference orthonormalises Z = B P for a
+  // Gaussian k x k P and takes the SVD of C = Z^T B, U = Z U_C
+  // (mklredsvd.cc:289-400); range(Z) = range(B) for every invertible P, so U
+  // and S are B's left singular vectors and values in both formulations.
+  LN_HIP(ctx, hipEventRecord(e[7], s));
+  std::vector<double> G(k * k), w(k);
+  V.assign(k * k, 0.0);
+  LN_CHECK(gram_f64(ctx, P2, P2, n, k, k, 1, G.data()));
+  LN_HIP(ctx, hipEventRecord(e[8], s));
+  auto t0 = std::chrono::steady_clock::now();
+  if (!sym_eig(G.data(), (int)k, w.data(), V.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "lightne_rsvd: eigensolver did not converge");
+  st.t_small_svd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  S.resize(k);
+  for (int64_t j = 0; j < k; ++j) S[j] = std::sqrt(std::max(w[j], 0.0));
+  t_gemm += lap(e[7], e[8]);
+  *B_out = P2;
+  st.t_spmm = t_spmm;
+  st.t_qr = t_qr;
+  st.t_gemm = t_gemm;
+  st.t_total = lap(e[0], e[8]) + st.t_small_svd;
+  return LIGHTNE_OK;
+}
+
+// W = V[:, :cols] diag(S^-p) for the panel product B W; columns whose singular
+// value is below 1e-6 S_0 (numerically zero for an FP32 B) are set to zero.
+static std::vector<double> right_factor(const std::vector<double>& V, const std::vector<double>& S, int64_t k,
+                                        int64_t cols, double pw) {
+  std::vector<double> W(k * cols, 0.0);
+  for (int64_t j = 0; j < cols; ++j) {
+    if (!(S[j] > 1e-6 * S[0])) continue;
+    const double f = std::pow(S[j], -pw);
+    for (int64_t i = 0; i < k; ++i) W[i * cols + j] = V[i * k + j] * f;
+  }
+  return W;
+}
+
 int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_params* p, float* U_out, float* S_host,
                  lightne_rsvd_stats* stats) {
   if (!ctx || !Ac || !p || !U_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: NULL argument");
@@ -308,94 +410,25 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
   LN_HIP(ctx, hipSetDevice(ctx->device));
   if (p->precision == LIGHTNE_FP64) return rsvd64(ctx, A, p, q, U_out, S_host, stats);
   lightne_rsvd_stats st{};
+  const float* B = nullptr;
+  std::vector<double> S, V;
+  LN_CHECK(rsvd_core(ctx, A, p, q, &B, S, V, &st));
   hipStream_t s = ctx->stream;
-  const bool bf = p->precision == LIGHTNE_BF16 || p->precision == LIGHTNE_BF16_POWER;
-  const bool bf_power = p->precision == LIGHTNE_BF16_POWER;
-  const size_t pf = (size_t)n * k * sizeof(float);
-  float *P0 = nullptr, *P1 = nullptr, *P2 = nullptr;  // Omega then Z; Y / Q; B
-  __hip_bfloat16* PB = nullptr;
-  {
-    WsCarve wc;
-    wc.add((void**)&P0, pf);
-    wc.add((void**)&P1, pf);
-    wc.add((void**)&P2, pf);
-    if (bf) wc.add((void**)&PB, (size_t)n * k * 2);
-    LN_CHECK(wc.commit(ctx));
-  }
-  hipEvent_t* e = ctx->ev;
-  double t_spmm = 0, t_qr = 0, t_gemm = 0;
-  auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
-
-  LN_HIP(ctx, hipEventRecord(e[0], s));
-  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P0));
-  if (bf) LN_CHECK(f32_to_bf16(ctx, P0, PB, n * k));
-  LN_HIP(ctx, hipEventRecord(e[1], s));
-  LN_CHECK(csr_plan(ctx, A));
-  LN_HIP(ctx, hipEventRecord(e[2], s));
-  LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P0, bf ? PANEL_BF16 : PANEL_F32, k, P1));
-  LN_HIP(ctx, hipEventRecord(e[3], s));
-  LN_HIP(ctx, hipEventSynchronize(e[3]));
-  st.t_sketch = lap(e[0], e[1]);
-  t_spmm += lap(e[2], e[3]);
-  st.spmm_calls = 1;
   if (p->random_project_only) {
-    LN_HIP(ctx, hipMemcpyAsync(U_out, P1, pf, hipMemcpyDeviceToDevice, s));
-    LN_HIP(ctx, hipStreamSynchronize(s));
-    st.t_spmm = t_spmm;
-    st.t_total = lap(e[0], e[3]);
-    if (stats) *stats = st;
-    return LIGHTNE_OK;
+    LN_HIP(ctx, hipMemcpyAsync(U_out, B, (size_t)n * k * sizeof(float), hipMemcpyDeviceToDevice, s));
+  } else {
+    LN_HIP(ctx, hipEventRecord(ctx->ev[9], s));
+    const std::vector<double> W = right_factor(V, S, k, k, 1.0);  // U = B V S^-1
+    LN_CHECK(panel_times_small(ctx, B, n, k, k, W.data(), k, U_out, 0, nullptr));
+    LN_HIP(ctx, hipEventRecord(ctx->ev[10], s));
+    LN_HIP(ctx, hipEventSynchronize(ctx->ev[10]));
+    const double tg = ev_seconds(ctx->ev[9], ctx->ev[10]);
+    st.t_gemm += tg;
+    st.t_total += tg;
+    if (S_host)
+      for (int64_t j = 0; j < k; ++j) S_host[j] = (float)S[j];
   }
-  int passes = 0;
-  for (int it = 0; it < q; ++it) {
-    LN_HIP(ctx, hipEventRecord(e[4], s));
-    LN_CHECK(cholqr2(ctx, P1, n, k, nullptr, &passes));
-    st.qr_calls++;
-    LN_HIP(ctx, hipEventRecord(e[5], s));
-    // power iterations gather BF16 in the BF16_POWER form; the last product
-    // B = A Q gathers the FP32 Q (the refinement pass that keeps C = Z^T A Q
-    // exact for the orthonormal basis Q)
-    const bool gather_bf = bf_power && (it + 1 < q);
-    if (gather_bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
-    LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2));
-    st.spmm_calls++;
-    LN_HIP(ctx, hipEventRecord(e[6], s));
-    LN_HIP(ctx, hipEventSynchronize(e[6]));
-    t_qr += lap(e[4], e[5]);
-    t_spmm += lap(e[5], e[6]);
-    if (it + 1 < q) std::swap(P1, P2);  // next pass orthonormalises A Q
-  }
-  // B = P2.  P ~ N(0,1) k x k from stream 1 (mklredsvd.cc:290-291)
-  std::vector<float> Pf(k * k);
-  gaussian_fill_host(p->seed, 1u, k * k, Pf.data());
-  std::vector<double> Pd(Pf.begin(), Pf.end());
-  LN_HIP(ctx, hipEventRecord(e[7], s));
-  LN_CHECK(panel_times_small(ctx, P2, n, k, k, Pd.data(), k, P0, 0, nullptr));  // Z = B P
-  LN_HIP(ctx, hipEventRecord(e[8], s));
-  LN_CHECK(cholqr2(ctx, P0, n, k, nullptr, &passes));
-  st.qr_calls++;
-  LN_HIP(ctx, hipEventRecord(e[9], s));
-  std::vector<double> C(k * k), Uc(k * k), Sc(k);
-  LN_CHECK(gram_f64(ctx, P0, P2, n, k, k, 0, C.data()));  // C = Z^T B
-  LN_HIP(ctx, hipEventRecord(e[10], s));
-  LN_HIP(ctx, hipEventSynchronize(e[10]));
-  auto t0 = std::chrono::steady_clock::now();
-  if (!svd_left_via_eig(C.data(), (int)k, Uc.data(), Sc.data()))
-    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "lightne_rsvd: small SVD did not converge");
-  auto t1 = std::chrono::steady_clock::now();
-  st.t_small_svd = std::chrono::duration<double>(t1 - t0).count();
-  LN_HIP(ctx, hipEventRecord(e[11], s));
-  LN_CHECK(panel_times_small(ctx, P0, n, k, k, Uc.data(), k, U_out, 0, nullptr));  // matU = Z U_C
-  LN_HIP(ctx, hipEventRecord(e[12], s));
-  LN_HIP(ctx, hipEventSynchronize(e[12]));
-  t_gemm += lap(e[7], e[8]) + lap(e[9], e[10]) + lap(e[11], e[12]);
-  t_qr += lap(e[8], e[9]);
-  if (S_host)
-    for (int64_t j = 0; j < k; ++j) S_host[j] = (float)Sc[j];
-  st.t_spmm = t_spmm;
-  st.t_qr = t_qr;
-  st.t_gemm = t_gemm;
-  st.t_total = lap(e[0], e[12]);
+  LN_HIP(ctx, hipStreamSynchronize(s));
   if (stats) *stats = st;
   return LIGHTNE_OK;
 }
@@ -454,19 +487,42 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true));
   auto t1 = std::chrono::steady_clock::now();
   const int64_t n = g->n, k = rp->rank;
-  auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
-  if (!U) {
+  std::vector<float> S(k, 0.f);
+  int rc = LIGHTNE_OK;
+  if (rp->precision == LIGHTNE_FP64) {
+    // accuracy reference: explicit U, then U sqrt(S)
+    auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
+    rc = U ? lightne_rsvd(ctx, A, rp, U, S.data(), &st.rsvd) : LIGHTNE_ERR_OOM;
     lightne_csr_free(A);
-    return LIGHTNE_ERR_OOM;
-  }
-  std::vector<float> S(k);
-  int rc = lightne_rsvd(ctx, A, rp, U, S.data(), &st.rsvd);
-  lightne_csr_free(A);
-  if (rc != LIGHTNE_OK) return rc;
-  if (rp->random_project_only) {
-    LN_CHECK(u_sigma_root_dev(ctx, U, nullptr, n, k, dim, normalize, emb_out));
+    if (rc != LIGHTNE_OK) return rc;
+    LN_CHECK(u_sigma_root_dev(ctx, U, rp->random_project_only ? nullptr : S.data(), n, k, dim, normalize, emb_out));
   } else {
-    LN_CHECK(u_sigma_root_dev(ctx, U, S.data(), n, k, dim, normalize, emb_out));
+    // emb = U[:, :dim] sqrt(S) = B (V S^-1/2)[:, :dim] (PathEmbed.h:371-408),
+    // row-normalised in the same pass when asked
+    if (k > n || k % 16 || k <= 0 || k > 256 || rp->upper ||
+        (rp->precision != LIGHTNE_FP32 && rp->precision != LIGHTNE_BF16 && rp->precision != LIGHTNE_BF16_POWER)) {
+      lightne_csr_free(A);
+      return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: bad rank, precision or upper storage");
+    }
+    const float* B = nullptr;
+    std::vector<double> Sd, V;
+    rc = rsvd_core(ctx, A, rp, std::max(1, (int)rp->power_iters), &B, Sd, V, &st.rsvd);
+    lightne_csr_free(A);
+    if (rc != LIGHTNE_OK) return rc;
+    hipStream_t s = ctx->stream;
+    LN_HIP(ctx, hipEventRecord(ctx->ev[9], s));
+    if (rp->random_project_only) {
+      LN_CHECK(u_sigma_root_dev(ctx, B, nullptr, n, k, dim, normalize, emb_out));
+    } else {
+      const std::vector<double> W = right_factor(V, Sd, k, dim, 0.5);
+      LN_CHECK(panel_times_small(ctx, B, n, k, k, W.data(), dim, emb_out, normalize, nullptr));
+      for (int64_t j = 0; j < k; ++j) S[j] = (float)Sd[j];
+    }
+    LN_HIP(ctx, hipEventRecord(ctx->ev[10], s));
+    LN_HIP(ctx, hipEventSynchronize(ctx->ev[10]));
+    const double tg = ev_seconds(ctx->ev[9], ctx->ev[10]);
+    st.rsvd.t_gemm += tg;
+    st.rsvd.t_total += tg;
   }
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
   auto t2 = std::chrono::steady_clock::now();

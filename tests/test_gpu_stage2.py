@@ -160,8 +160,11 @@ def test_netsmf_end_to_end(gpu):
     emb, S, st = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, ctx=gpu)
     csr, _ = L.generate_trunc_log_matrix(g, 1, walk_len=10, sample_ratio=ratio, ctx=gpu)
     svd = L.MKLRedSVD(csr, 144, seed=777, ctx=gpu).run()
-    ref = L.compute_u_sigma_root(svd.matU, svd.S, hg.n, 144, 128, False, ctx=gpu)
-    np.testing.assert_array_equal(emb.cpu().numpy(), ref.cpu().numpy())
+    ref = L.compute_u_sigma_root(svd.matU, svd.S, hg.n, 144, 128, False, ctx=gpu).cpu().numpy()
+    # NetSMF forms U sqrt(S) as one product B (V S^-1/2); MKLRedSVD + compute_u_sigma_root
+    # as (B V S^-1) sqrt(S): the same values up to FP32 rounding
+    e = emb.cpu().numpy()
+    assert np.abs(e - ref).max() <= 1e-5 * np.abs(ref).max()
     np.testing.assert_array_equal(S, svd.S)
     assert st["t_total"] > 0
 
