@@ -346,7 +346,8 @@ def main():
                                f"K={cfg['K']} mu=0.2 theta=0.5 FP32",
                    "parallelism": "1 GPU"},
         "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
-        "sparsifier_nnz": nnz, "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
+        "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
+        "table_capacity": sp["table_capacity"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

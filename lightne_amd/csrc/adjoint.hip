@@ -53,14 +53,14 @@ __global__ __launch_bounds__(256) void k_dot(const float* __restrict__ x, const 
 // out = coef * D~^-1 g  (mode 0)   or   out = g + D~ m  (mode 1)
 __global__ void k_deg_scale(const float* __restrict__ g, const float* __restrict__ m, const int64_t* __restrict__ rp,
                             int64_t n, int64_t d, float coef, int mode, float* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * d) return;
-  const int64_t r = i / d;
-  const double deg = (double)(rp[r + 1] - rp[r] + 1);
-  if (mode == 0)
-    out[i] = coef * (float)((double)g[i] / deg);
-  else
-    out[i] = g[i] + (float)(deg * (double)m[i]);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * d; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / d;
+    const double deg = (double)(rp[r + 1] - rp[r] + 1);
+    if (mode == 0)
+      out[i] = coef * (float)((double)g[i] / deg);
+    else
+      out[i] = g[i] + (float)(deg * (double)m[i]);
+  }
 }
 
 struct Adj {
@@ -227,7 +227,7 @@ extern "C" int lightne_propagate_backward(lightne_ctx* ctx, const lightne_graph*
   float* spare = rot[2];
   {
     const int64_t tot = g->n * dim;
-    hipLaunchKernelGGL(k_deg_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, gbar,
+    hipLaunchKernelGGL(k_deg_scale, dim3(grid_blocks(tot, 256)), dim3(256), 0, ctx->stream, gbar,
                        (const float*)nullptr, g->offsets, g->n, dim, -c[K - 1], 0, m_cur);
     LN_LAUNCH(ctx);
   }
@@ -268,7 +268,7 @@ extern "C" int lightne_propagate_backward(lightne_ctx* ctx, const lightne_graph*
   // dL/da = g + D~ m_0
   {
     const int64_t tot = g->n * dim;
-    hipLaunchKernelGGL(k_deg_scale, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, gbar,
+    hipLaunchKernelGGL(k_deg_scale, dim3(grid_blocks(tot, 256)), dim3(256), 0, ctx->stream, gbar,
                        (const float*)m_cur, g->offsets, g->n, dim, 0.f, 1, grad_a);
     LN_LAUNCH(ctx);
   }

@@ -57,8 +57,8 @@ __host__ __device__ __forceinline__ float round_bf16(float f) {
 }
 
 __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float* __restrict__ out) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (2 * p >= count) return;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * p < count;
+       p += (int64_t)gridDim.x * blockDim.x) {
   uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
   philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
   const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
@@ -67,6 +67,7 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
   const double t = 6.283185307179586476925286766559 * u2;
   out[2 * p] = round_bf16((float)(r * cos(t)));
   if (2 * p + 1 < count) out[2 * p + 1] = round_bf16((float)(r * sin(t)));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -296,31 +297,31 @@ __global__ __launch_bounds__(256) void k_panel64(const double* __restrict__ X, i
 }
 
 __global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ out, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) out[i] = (double)in[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (double)in[i];
 }
 
 __global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) out[i] = (float)in[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
 }
 
 __global__ void k_f32_to_bf16(const float* __restrict__ in, __hip_bfloat16* __restrict__ out, int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
   uint32_t u = __float_as_uint(in[i]);
   u += 0x7fffu + ((u >> 16) & 1u);
   const unsigned short b = (unsigned short)(u >> 16);
   out[i] = *reinterpret_cast<const __hip_bfloat16*>(&b);
+  }
 }
 
 // emb[i, j] = sqrt(S_j) U[i, j] (j < dim, U row stride rank), optional row
 // normalisation; one wave per row.
 __global__ void k_u_sigma_root(const float* __restrict__ U, const float* __restrict__ sq, int64_t n, int64_t rank,
                                int64_t dim, int normalize, float* __restrict__ emb) {
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < n;
+       row += ((int64_t)gridDim.x * blockDim.x) >> 6) {
   float ss = 0.f;
   for (int64_t j = lane; j < dim; j += 64) {
     const float v = sq[j] * U[row * rank + j];
@@ -331,6 +332,7 @@ __global__ void k_u_sigma_root(const float* __restrict__ U, const float* __restr
   const float nrm = sqrtf(ss);
   const float inv = (normalize && nrm > 1e-5f) ? 1.0f / nrm : 1.0f;
   for (int64_t j = lane; j < dim; j += 64) emb[row * dim + j] = sq[j] * U[row * rank + j] * inv;
+  }
 }
 
 }  // namespace
@@ -352,7 +354,7 @@ void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* ou
 int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out) {
   if (count <= 0) return LIGHTNE_OK;
   const int64_t pairs = (count + 1) / 2;
-  hipLaunchKernelGGL(k_gaussian, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, ctx->stream, seed, stream,
+  hipLaunchKernelGGL(k_gaussian, dim3(grid_blocks(pairs, 256)), dim3(256), 0, ctx->stream, seed, stream,
                      count, out);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;
@@ -493,21 +495,21 @@ int panel_times_small64(lightne_ctx* ctx, const double* X, int64_t n, int64_t k1
 
 int f32_to_f64(lightne_ctx* ctx, const float* in, double* out, int64_t count) {
   if (count <= 0) return LIGHTNE_OK;
-  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_blocks(count, 256)), dim3(256), 0, ctx->stream, in, out, count);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
 
 int f64_to_f32(lightne_ctx* ctx, const double* in, float* out, int64_t count) {
   if (count <= 0) return LIGHTNE_OK;
-  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_blocks(count, 256)), dim3(256), 0, ctx->stream, in, out, count);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
 
 int f32_to_bf16(lightne_ctx* ctx, const float* in, __hip_bfloat16* out, int64_t count) {
   if (count <= 0) return LIGHTNE_OK;
-  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, in, out, count);
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(grid_blocks(count, 256)), dim3(256), 0, ctx->stream, in, out, count);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
@@ -522,7 +524,7 @@ int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int6
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));  // sq is a local
   if (n <= 0) return LIGHTNE_OK;
   const int64_t blocks = (n * 64 + 255) / 256;
-  hipLaunchKernelGGL(k_u_sigma_root, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, U, (const float*)dsq, n, rank,
+  hipLaunchKernelGGL(k_u_sigma_root, dim3(grid_blocks(blocks, 1)), dim3(256), 0, ctx->stream, U, (const float*)dsq, n, rank,
                      dim, normalize, emb);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;

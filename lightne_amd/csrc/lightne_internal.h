@@ -129,6 +129,14 @@ int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
                   lightne_sparsify_stats* stats, bool ctx_owned);
 
+// Blocks for a grid-stride loop over `count` items: one grid dimension may
+// hold at most 2^32 - 1 work-items, so large element counts loop.
+inline unsigned grid_blocks(int64_t count, int block, int64_t max_blocks = (int64_t)1 << 20) {
+  int64_t b = (count + block - 1) / block;
+  if (b > max_blocks) b = max_blocks;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
 // elapsed seconds between two events recorded on the context stream
 double ev_seconds(hipEvent_t a, hipEvent_t b);
 

@@ -39,8 +39,8 @@ struct GraphPlan {
 
 __global__ void k_y_from_conv(const float* __restrict__ a, const float* __restrict__ conv, float* __restrict__ y,
                               int64_t count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) y[i] = a[i] + (-1.0f) * conv[i];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a[i] + (-1.0f) * conv[i];
 }
 
 int check_prop(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim, const lightne_prop_params* p) {
@@ -152,7 +152,7 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   if (bf) LN_CHECK(f32_to_bf16(ctx, lx1_32, (__hip_bfloat16*)lx1, (int64_t)pe));
   float* yout = y;
   if (K <= 2) {
-    hipLaunchKernelGGL(k_y_from_conv, dim3((unsigned)((pe + 255) / 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
+    hipLaunchKernelGGL(k_y_from_conv, dim3(grid_blocks((int64_t)pe, 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
     LN_LAUNCH(ctx);
   }
   // free buffer for Lx2 (never the caller's a)

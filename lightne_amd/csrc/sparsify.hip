@@ -234,7 +234,8 @@ __global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ 
                                                        uint64_t cap, uint64_t* __restrict__ keys,
                                                        uint32_t* __restrict__ idx,
                                                        unsigned long long* __restrict__ counter) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < cap; i0 += (uint64_t)gridDim.x * blockDim.x) {
+  const uint64_t i = i0 + threadIdx.x;
   const bool has = i < cap && tab[i].key != EMPTY_KEY;
   const uint64_t bal = __ballot(has);
   const int lane = threadIdx.x & 63;
@@ -245,6 +246,7 @@ __global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ 
     const uint64_t pos = base + __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
     keys[pos] = tab[i].key;
     idx[pos] = (uint32_t)i;  // slot index (cap <= 2^32 enforced by the host)
+  }
   }
 }
 
@@ -669,7 +671,7 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   auto* counter = (unsigned long long*)ctx_buf(ctx, "compact_ctr", 8);
   if (!counter) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemsetAsync(counter, 0, 8, ctx->stream));
-  hipLaunchKernelGGL(k_table_compact, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, ctx->stream, tab, wtab, cap,
+  hipLaunchKernelGGL(k_table_compact, dim3(grid_blocks((int64_t)cap, 256)), dim3(256), 0, ctx->stream, tab, wtab, cap,
                      (uint64_t*)mk.p, (uint32_t*)mi.p, counter);
   LN_LAUNCH(ctx);
   LN_CHECK(radix_sort_pairs_u32(ctx, (uint64_t*)mk.p, (uint32_t*)mi.p, (int64_t)nk, 32 + bits_for((uint64_t)g->n)));

@@ -223,6 +223,8 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   }
   const int64_t waves_short = (a.n + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
+  if (waves >= ((int64_t)1 << 26))  // one grid dimension holds < 2^32 work-items
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: %lld waves exceed one launch", (long long)waves);
   hipLaunchKernelGGL((k_spmm<XT, EPI, IMPL>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a,
                      waves_short);
   LN_LAUNCH(ctx);
@@ -462,6 +464,8 @@ int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   }
   const int64_t waves_short = (a.n + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
+  if (waves >= ((int64_t)1 << 26))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: %lld waves exceed one launch", (long long)waves);
   hipLaunchKernelGGL(k_spmm64, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a, waves_short);
   LN_LAUNCH(ctx);
   if (P.nmulti > 0) {
