@@ -36,7 +36,7 @@ CONFIGS = {
                K=10),
     # R-MAT scale 24, edge factor 16, (a, b, c) = (.57, .19, .19); T = 5, M = 10 m, q = 2
     "c3": dict(rmat=(24, 16.0, 0.57, 0.19, 0.19), gseed=3, T=5, per_edge=5.0, k=144, d=128, q=2, K=10,
-               cpu_frac=1 / 64),
+               cpu_frac=1 / 64, cpu_prop_one=True),
 }
 
 
@@ -107,9 +107,11 @@ def cpu_baseline(hg, cfg, ratio, csr_host, frac=0.25):
                                   cap_log2=int(np.ceil(np.log2(2 * e_done * per_edge))),
                                   vertex_range=(0, n, stride))
     t_s1 = time.perf_counter() - t0
+    log(f"cpu: sampler {t_s1:.2f}s")
     t0 = time.perf_counter()
     O.trunc_log_csr(hg.offsets, keys, wgh)
     t_csr = time.perf_counter() - t0
+    log(f"cpu: trunc-log csr {t_csr:.2f}s")
     s1 = m / e_done
     del keys, cnt, wgh
     rp, col, val = csr_host
@@ -121,7 +123,9 @@ def cpu_baseline(hg, cfg, ratio, csr_host, frac=0.25):
     t0 = time.perf_counter()
     Y = O.spmm_f32(*sub, Om, threads)
     B = O.spmm_f32(*sub, Om, threads)
-    t_spmm = time.perf_counter() - t0
+    # 1 + q products: time two, scale to all
+    t_spmm = (time.perf_counter() - t0) * (1 + cfg["q"]) / 2
+    log(f"cpu: spmm {t_spmm:.2f}s")
     r3 = int(frac * n)
     Yr, Br = np.ascontiguousarray(Om[:r3]), np.ascontiguousarray(Om[r3:2 * r3])
     t0 = time.perf_counter()
@@ -138,6 +142,7 @@ def cpu_baseline(hg, cfg, ratio, csr_host, frac=0.25):
     t_rows += time.perf_counter() - t0
     del Y, B, Q, Z, U, Yr, Br
     t_svd = t_spmm * nnz / max(nnz2, 1) + t_rows * n / max(r3, 1) + t_small
+    log(f"cpu: rsvd scaled {t_svd:.2f}s")
     # stage 3
     off64 = np.ascontiguousarray(hg.offsets, np.uint64)
     edg = np.ascontiguousarray(hg.edges, np.uint32)
@@ -150,22 +155,38 @@ def cpu_baseline(hg, cfg, ratio, csr_host, frac=0.25):
                                Yb.ctypes.data, threads)
 
     tmp = np.empty_like(a)
-    P_(a, 1.0, 0.0, tmp)
-    Lx1 = -a
-    P_(tmp, 0.5, 1.0, Lx1)
-    conv = c[0] * a + c[1] * Lx1
-    Lx0 = a
-    for i in range(2, cfg["K"]):
-        P_(Lx1, 1.0, 0.0, tmp)
-        Lx2 = -Lx0
-        P_(tmp, 1.0, 1.0, Lx2)
-        Lx2 -= 2.0 * Lx1
-        conv += c[i] * Lx2
-        Lx0, Lx1 = Lx1, Lx2
-    y = np.ascontiguousarray((a - conv).astype(np.float64))
-    z = np.empty_like(y)
-    lib.orc_aggregate_f64(n, off64.ctypes.data, edg.ctypes.data, y.ctypes.data, d, z.ctypes.data, threads)
-    t_prop = time.perf_counter() - t0
+    if cfg.get("cpu_prop_one"):
+        # large graphs: one sparse product and one recurrence update on the
+        # full graph, scaled to the 2K - 1 products and K - 1 updates
+        P_(a, 1.0, 0.0, tmp)
+        t_p = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        Lx2 = -a
+        Lx2 -= 2.0 * tmp
+        conv = c[0] * a + c[1] * Lx2
+        t_u = time.perf_counter() - t1
+        t_prop = t_p * (2 * cfg["K"]) + t_u * (cfg["K"] - 1)
+        z = np.asarray(tmp, np.float64)
+        prop_note = f"propagation: 1 of {2 * cfg['K'] - 1} products timed on the full graph ({t_p:.2f} s)"
+    else:
+        P_(a, 1.0, 0.0, tmp)
+        Lx1 = -a
+        P_(tmp, 0.5, 1.0, Lx1)
+        conv = c[0] * a + c[1] * Lx1
+        Lx0 = a
+        for i in range(2, cfg["K"]):
+            P_(Lx1, 1.0, 0.0, tmp)
+            Lx2 = -Lx0
+            P_(tmp, 1.0, 1.0, Lx2)
+            Lx2 -= 2.0 * Lx1
+            conv += c[i] * Lx2
+            Lx0, Lx1 = Lx1, Lx2
+        y = np.ascontiguousarray((a - conv).astype(np.float64))
+        z = np.empty_like(y)
+        lib.orc_aggregate_f64(n, off64.ctypes.data, edg.ctypes.data, y.ctypes.data, d, z.ctypes.data, threads)
+        t_prop = time.perf_counter() - t0
+        prop_note = f"propagation on the full graph ({t_prop:.2f} s)"
+    log(f"cpu: propagation {t_prop:.2f}s")
     t0 = time.perf_counter()
     np.linalg.svd(z[:r3].astype(np.float32), full_matrices=False)
     t_tall = (time.perf_counter() - t0) * n / r3
@@ -177,7 +198,7 @@ def cpu_baseline(hg, cfg, ratio, csr_host, frac=0.25):
         "sample": (f"oracle restatement, FP32, {threads} threads; sampler+CSR on {e_done}/{m} directed edges "
                    f"({t_s1:.2f}+{t_csr:.2f} s, x{s1:.2f}); rSVD SpMMs on {nnz2}/{nnz} nnz ({t_spmm:.2f} s, "
                    f"x{nnz / max(nnz2, 1):.2f}), QR/GEMM on {r3}/{n} rows ({t_rows:.2f} s, x{n / max(r3, 1):.2f}); "
-                   f"propagation on the full graph ({t_prop:.2f} s); tall SVD on {r3}/{n} rows"),
+                   f"{prop_note}; tall SVD on {r3}/{n} rows"),
         "sampled_edges_per_s": st["samples"] / t_s1,
         "stages_s": {"sample": round(t_sample, 3), "csr": round(t_csr_full, 3), "svd": round(t_svd, 3),
                      "propagation": round(t_prop + t_tall, 3)},
@@ -336,11 +357,13 @@ def main():
         csr, _ = L.generate_trunc_log_matrix(g, 1, walk_len=cfg["T"], sample_ratio=ratio, ctx=ctx)
         csr_host = csr.to_numpy()
         csr.free()
+        log("cpu: sparsifier copied to the host")
         cpu = cpu_baseline(hg, cfg, ratio, csr_host, frac=cfg.get("cpu_frac", 0.25))
     out = {
         "metric": METRIC, "value": round(ms_per_step / 1e3, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Chung-Lu graph)",
+        "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic (seeded {'R-MAT' if 'rmat' in cfg else 'Chung-Lu'} graph)",
         "config": {"workload": f"{args.config}: {'R-MAT' if 'rmat' in cfg else 'Chung-Lu'} n={hg.n} "
                                f"m_undirected={hg.m // 2} T={cfg['T']} M={sp['samples']} k={k} d={d} q={cfg['q']} "
                                f"K={cfg['K']} mu=0.2 theta=0.5 FP32",

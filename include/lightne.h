@@ -222,6 +222,17 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
                    const lightne_rsvd_params* rp, int64_t dim, int32_t normalize, float* emb_out,
                    float* S_host, lightne_netsmf_stats* stats);
 
+/* ---- the ne_zhang_et_al method (PathEmbed.h:410-463, LightNE.cc:106) -------
+ * Matrix on the graph's own pattern: A_uv = log(1/d_u) - log(neg_v / sum neg),
+ * neg_j = (sum_{i in N(j)} 1/d_i)^0.75, then the randomized SVD (rp) and
+ * emb = rownorm(U[:, :dim] sqrt(S)) (normalisation always on, as :459).
+ * n must be below 2^31 (the neighbour ids are the CSR columns). */
+int lightne_ne_zhang(lightne_ctx* ctx, const lightne_graph* g, const lightne_rsvd_params* rp, int64_t dim,
+                     float* emb_out, float* S_host, lightne_rsvd_stats* stats);
+/* the matrix alone (PathEmbed.h:412-444), as a CSR handle: row_ptr = the graph's
+ * offsets, columns in adjacency order */
+int lightne_ne_zhang_matrix(lightne_ctx* ctx, const lightne_graph* g, lightne_csr** out);
+
 /* ---- dense panel kernels (building blocks of stages 2 and 3) ---------------
  * No reference counterpart as separate calls: they are the pieces that replace
  * the reference's cblas_?gemm (mklredsvd.cc:297,330,382) and the

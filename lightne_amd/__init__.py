@@ -6,15 +6,16 @@ include/lightne.h; this package is the host-side mirror of the reference's
 stage API (see ``lightne_amd.api``).
 """
 from ._lib import BF16, BF16_POWER, FP32, FP64, LightNEError, LIB_PATH  # noqa: F401
-from .api import (CSR, Context, DeviceGraph, HostGraph, MKLRedSVD, NetSMF, SparseTable, bessel_i,  # noqa: F401
-                  chebyshev_expansion, chung_lu, compute_u_sigma_root, context, dense_cholqr, dense_gram,
-                  dense_panel_gemm, generate_trunc_log_matrix, get_embedding_dense, propagate, propagate_backward,
-                  read_unweighted_symmetric_graph, rmat, sample_scalars, sparsify_counts, write_adjacency_graph)
+from .api import (CSR, Context, DeviceGraph, HostGraph, MKLRedSVD, NE_Zhang_et_al, NetSMF, SparseTable,  # noqa: F401
+                  bessel_i, chebyshev_expansion, chung_lu, compute_u_sigma_root, context, dense_cholqr, dense_gram,
+                  dense_panel_gemm, generate_trunc_log_matrix, get_embedding_dense, ne_zhang_matrix, propagate,
+                  propagate_backward, read_unweighted_symmetric_graph, rmat, sample_scalars, sparsify_counts,
+                  write_adjacency_graph)
 
 __all__ = [
-    "BF16", "BF16_POWER", "FP32", "FP64", "LightNEError", "CSR", "Context", "DeviceGraph", "HostGraph", "MKLRedSVD", "NetSMF",
-    "SparseTable", "bessel_i", "chebyshev_expansion", "chung_lu", "compute_u_sigma_root", "context",
-    "dense_cholqr", "dense_gram", "dense_panel_gemm", "generate_trunc_log_matrix", "get_embedding_dense",
-    "propagate", "propagate_backward", "read_unweighted_symmetric_graph", "rmat", "sample_scalars",
-    "sparsify_counts", "write_adjacency_graph",
+    "BF16", "BF16_POWER", "FP32", "FP64", "LightNEError", "CSR", "Context", "DeviceGraph", "HostGraph",
+    "MKLRedSVD", "NE_Zhang_et_al", "NetSMF", "SparseTable", "bessel_i", "chebyshev_expansion", "chung_lu",
+    "compute_u_sigma_root", "context", "dense_cholqr", "dense_gram", "dense_panel_gemm",
+    "generate_trunc_log_matrix", "get_embedding_dense", "ne_zhang_matrix", "propagate", "propagate_backward",
+    "read_unweighted_symmetric_graph", "rmat", "sample_scalars", "sparsify_counts", "write_adjacency_graph",
 ]

@@ -131,6 +131,9 @@ _SIGS = {
                                              _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
     "lightne_netsmf": (C.c_int, [_P, C.POINTER(Graph), C.POINTER(SparsifyParams), C.POINTER(RsvdParams), _I64,
                                  C.c_int32, _P, _P, C.POINTER(NetsmfStats)]),
+    "lightne_ne_zhang": (C.c_int, [_P, C.POINTER(Graph), C.POINTER(RsvdParams), _I64, _P, _P,
+                                   C.POINTER(RsvdStats)]),
+    "lightne_ne_zhang_matrix": (C.c_int, [_P, C.POINTER(Graph), _PP]),
     "lightne_dense_gram": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, C.c_int32, _P]),
     "lightne_dense_panel_gemm": (C.c_int, [_P, _P, _I64, _I64, _P, _I64, _P, C.c_int32]),
     "lightne_dense_cholqr": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int32)]),

@@ -363,6 +363,32 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
     return emb, S, _stats_dict(st)
 
 
+def ne_zhang_matrix(graph: DeviceGraph, ctx: Optional[Context] = None) -> "CSR":
+    """The ne_zhang_et_al log matrix (PathEmbed.h:412-444) on the graph's pattern."""
+    ctx = ctx or context()
+    h = C.c_void_p()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_ne_zhang_matrix(ctx.ptr, C.byref(g), C.byref(h)))
+    return CSR(ctx, h)
+
+
+def NE_Zhang_et_al(graph: DeviceGraph, rank: int = 256, dim: int = 128, svd_seed: int = 777, power_iters: int = 1,
+                   precision: int = FP32, ctx: Optional[Context] = None):
+    """path_embed::NE_Zhang_et_al (PathEmbed.h:410-463): returns (emb n x dim,
+    S, stats); rows of emb are L2-normalised as in the reference."""
+    torch = _torch()
+    ctx = ctx or context()
+    rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision, random_project_only=0, upper=0,
+                    seed=svd_seed)
+    emb = torch.empty((graph.n, dim), dtype=torch.float32, device=f"cuda:{ctx.device}")
+    S = np.zeros(rank, np.float32)
+    st = RsvdStats()
+    g = graph.desc()
+    ctx.check(ctx.lib.lightne_ne_zhang(ctx.ptr, C.byref(g), C.byref(rp), dim, _dev(emb), S.ctypes.data,
+                                       C.byref(st)))
+    return emb, S, _stats_dict(st)
+
+
 # ---------------------------------------------------------------------------
 # stage 3
 # ---------------------------------------------------------------------------

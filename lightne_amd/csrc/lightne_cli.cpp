@@ -186,10 +186,6 @@ int main(int argc, char** argv) {
     fprintf(stderr, "lightne: -ne_method must be netsmf or ne_zhang_et_al\n");
     return 2;
   }
-  if (ne_method == "ne_zhang_et_al") {
-    fprintf(stderr, "lightne: -ne_method ne_zhang_et_al is not available in this build\n");
-    return 2;
-  }
   if (sparse_project) {
     fprintf(stderr, "lightne: -sparse_project is not available in this build\n");
     return 2;
@@ -271,6 +267,18 @@ int main(int argc, char** argv) {
       }
       fclose(f);
       HK(hipMemcpy(emb, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else if (ne_method == "ne_zhang_et_al") {
+      // PathEmbed.h:410-463 (LightNE.cc:106)
+      lightne_rsvd_params rp{};
+      rp.rank = (int64_t)rank;
+      rp.power_iters = power_iters;
+      rp.precision = prec;
+      rp.seed = svd_seed;
+      lightne_rsvd_stats st{};
+      std::vector<float> S(rank);
+      const double ts = now();
+      CK(lightne_ne_zhang(ctx, &g, &rp, (int64_t)dim, emb, S.data(), &st), ctx);
+      printf("# svd time: %.6f\n", now() - ts);
     } else {
       lightne_sparsify_params sp{};
       sp.walk_len = walk_len;

@@ -128,6 +128,8 @@ int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 // does not free them.
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
                   lightne_sparsify_stats* stats, bool ctx_owned);
+// values of the ne_zhang_et_al matrix on the graph's pattern (nezhang.hip)
+int ne_zhang_values(lightne_ctx* ctx, const lightne_graph* g, float* val);
 
 // Blocks for a grid-stride loop over `count` items: one grid dimension may
 // hold at most 2^32 - 1 work-items, so large element counts loop.

@@ -68,15 +68,18 @@ void cheb_coeffs(int64_t order, double theta, std::vector<float>& c) {
 double lightne_bessel_i_impl(int k, double x) { return bessel_i(k, x); }
 extern "C" double lightne_bessel_i(int k, double x) { return bessel_i(k, x); }
 
-// workspace bytes propagate_forward carves (from offset 0)
+// Workspace propagate_forward carves (from offset 0).  FP32: two iterates,
+// the intermediate M T_i and the accumulator (the new iterate overwrites
+// T_{i-1} element by element in the epilogue that reads it).  BF16: the same
+// in BF16 plus an FP32 accumulator and one FP32 panel for the FP32 first step,
+// whose intermediate spans the second iterate and the intermediate.
 size_t prop_ws_bytes(int64_t n, int64_t dim, bool bf) {
   const size_t pe = (size_t)n * dim;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t esz = bf ? 2 : 4;
-  return 4 * al(pe * esz) + al(pe * 4) + (bf ? 2 * al(pe * 4) : 0);
+  return bf ? 3 * al(pe * 2) + 2 * al(pe * 4) : 4 * al(pe * 4);
 }
 
-// forward recurrence; optional iterate checkpoints for the adjoint
+// forward recurrence; y == NULL writes y over the accumulator (internal use)
 int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                       const lightne_prop_params* p, float* y, float* z, lightne_prop_stats* st, SpmmPlan& P) {
   const int64_t n = g->n, K = std::max<int64_t>(p->order, 2);
@@ -84,25 +87,21 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   const int xt = bf ? PANEL_BF16 : PANEL_F32;
   const size_t esz = bf ? 2 : 4;
   const size_t pe = (size_t)n * dim;
-  // iterates and accumulator at the start of the context workspace
-  void* B[3];
+  void* B[2];
   void* tmp = nullptr;
   float* conv = nullptr;
-  float* tmp32 = nullptr;
-  float* lx1_32 = nullptr;
+  float* x32 = nullptr;
   WsCarve wc;
-  for (int j = 0; j < 3; ++j) wc.add(&B[j], pe * esz);
+  wc.add(&B[0], pe * esz);
+  wc.add(&B[1], pe * esz);
   wc.add(&tmp, pe * esz);
   wc.add((void**)&conv, pe * sizeof(float));
-  if (bf) {
-    wc.add((void**)&tmp32, pe * sizeof(float));
-    wc.add((void**)&lx1_32, pe * sizeof(float));
-  }
+  if (bf) wc.add((void**)&x32, pe * sizeof(float));
   LN_CHECK(wc.commit(ctx));
-  if (!bf) {
-    tmp32 = (float*)tmp;
-    lx1_32 = (float*)B[1];
-  }
+  // FP32 first step: M a into tmp32, T_1 into lx1_32
+  float* tmp32 = bf ? (float*)B[1] : (float*)tmp;  // BF16: spans B[1] and tmp
+  float* lx1_32 = bf ? x32 : (float*)B[1];
+  if (!y) y = conv;
   std::vector<float> c;
   cheb_coeffs(p->order, (double)p->theta, c);
   hipStream_t s = ctx->stream;
@@ -155,9 +154,9 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
     hipLaunchKernelGGL(k_y_from_conv, dim3(grid_blocks((int64_t)pe, 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
     LN_LAUNCH(ctx);
   }
-  // free buffer for Lx2 (never the caller's a)
-  void* lx2 = bf ? B[2] : B[0];
   for (int64_t i = 2; i < K; ++i) {
+    // T_{i} overwrites T_{i-2} (never the caller's a)
+    void* lx2 = (lx0 == (const void*)a) ? B[0] : const_cast<void*>(lx0);
     {
       SpmmArgs t = base;  // tmp = M Lx1  (:153-163)
       t.X = lx1;
@@ -180,11 +179,8 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
       LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB_LAST, true));
     }
     ++calls;
-    // rotate: Lx0 <- Lx1, Lx1 <- Lx2, next Lx2 <- a free buffer (not a)
-    void* old0 = const_cast<void*>(lx0);
     lx0 = lx1;
     lx1 = lx2;
-    lx2 = (old0 == (const void*)a) ? B[2] : old0;
   }
   LN_HIP(ctx, hipEventRecord(ctx->ev[14], s));
   if (z) {
@@ -215,7 +211,8 @@ int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank,
     wc.add((void**)&Q, (size_t)n * rank * sizeof(float));
     LN_CHECK(wc.commit(ctx));
   }
-  LN_HIP(ctx, hipMemcpyAsync(Q, mat, (size_t)n * rank * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+  if (Q != mat)
+    LN_HIP(ctx, hipMemcpyAsync(Q, mat, (size_t)n * rank * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
   std::vector<double> R(rank * rank), Ur(rank * rank), Sr(rank);
   int passes = 0;
   LN_CHECK(cholqr2(ctx, Q, n, rank, R.data(), &passes));
@@ -245,14 +242,8 @@ int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   auto t0 = std::chrono::steady_clock::now();
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
-  float* y = y_out;
-  if (!y) {
-    WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
-    wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
-    LN_CHECK(wc.commit(ctx));
-  }
   lightne_prop_stats st{};
-  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z_out, &st, gp.P));
+  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y_out, z_out, &st, gp.P));
   auto t1 = std::chrono::steady_clock::now();
   st.t_total = std::chrono::duration<double>(t1 - t0).count();
   if (stats) *stats = st;
@@ -276,17 +267,16 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
   auto t0 = std::chrono::steady_clock::now();
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
-  // y, z and the tall-SVD panel after the recurrence's own workspace
-  float *y = nullptr, *z = nullptr, *q = nullptr;
+  // z after the recurrence's own workspace (y is written over its
+  // accumulator); the tall SVD orthonormalises z in place
+  float* z = nullptr;
   WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
-  wc.add((void**)&y, (size_t)g->n * dim * sizeof(float));
   wc.add((void**)&z, (size_t)g->n * dim * sizeof(float));
-  wc.add((void**)&q, (size_t)g->n * dim * sizeof(float));
   LN_CHECK(wc.commit(ctx));
   lightne_prop_stats st{};
-  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y, z, &st, gp.P));
+  LN_CHECK(propagate_forward(ctx, g, a, dim, p, nullptr, z, &st, gp.P));
   auto t1 = std::chrono::steady_clock::now();
-  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr, q));
+  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr, z));
   auto t2 = std::chrono::steady_clock::now();
   st.t_svd = std::chrono::duration<double>(t2 - t1).count();
   st.t_total = std::chrono::duration<double>(t2 - t0).count();

@@ -475,6 +475,49 @@ int lightne_u_sigma_root(lightne_ctx* ctx, const float* U, const float* S_host, 
   return LIGHTNE_OK;
 }
 
+// Stage 2 and the embedding on a CSR: emb = U[:, :dim] sqrt(S), row-normalised
+// when asked (PathEmbed.h:371-408); A is freed here.
+static int svd_embed(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* rp, int64_t dim, int32_t normalize,
+                     float* emb_out, std::vector<float>& S, lightne_rsvd_stats* rst) {
+  const int64_t n = A->n, k = rp->rank;
+  S.assign(k, 0.f);
+  int rc = LIGHTNE_OK;
+  if (rp->precision == LIGHTNE_FP64) {
+    // accuracy reference: explicit U, then U sqrt(S)
+    auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
+    rc = U ? lightne_rsvd(ctx, A, rp, U, S.data(), rst) : LIGHTNE_ERR_OOM;
+    lightne_csr_free(A);
+    if (rc != LIGHTNE_OK) return rc;
+    return u_sigma_root_dev(ctx, U, rp->random_project_only ? nullptr : S.data(), n, k, dim, normalize, emb_out);
+  }
+  // emb = B (V S^-1/2)[:, :dim], row-normalised in the same pass
+  if (k > n || k % 16 || k <= 0 || k > 256 || rp->upper ||
+      (rp->precision != LIGHTNE_FP32 && rp->precision != LIGHTNE_BF16 && rp->precision != LIGHTNE_BF16_POWER)) {
+    lightne_csr_free(A);
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embedding: bad rank, precision or upper storage");
+  }
+  const float* B = nullptr;
+  std::vector<double> Sd, V;
+  rc = rsvd_core(ctx, A, rp, std::max(1, (int)rp->power_iters), &B, Sd, V, rst);
+  lightne_csr_free(A);
+  if (rc != LIGHTNE_OK) return rc;
+  hipStream_t s = ctx->stream;
+  LN_HIP(ctx, hipEventRecord(ctx->ev[9], s));
+  if (rp->random_project_only) {
+    LN_CHECK(u_sigma_root_dev(ctx, B, nullptr, n, k, dim, normalize, emb_out));
+  } else {
+    const std::vector<double> W = right_factor(V, Sd, k, dim, 0.5);
+    LN_CHECK(panel_times_small(ctx, B, n, k, k, W.data(), dim, emb_out, normalize, nullptr));
+    for (int64_t j = 0; j < k; ++j) S[j] = (float)Sd[j];
+  }
+  LN_HIP(ctx, hipEventRecord(ctx->ev[10], s));
+  LN_HIP(ctx, hipEventSynchronize(ctx->ev[10]));
+  const double tg = ev_seconds(ctx->ev[9], ctx->ev[10]);
+  rst->t_gemm += tg;
+  rst->t_total += tg;
+  return LIGHTNE_OK;
+}
+
 int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* sp,
                    const lightne_rsvd_params* rp, int64_t dim, int32_t normalize, float* emb_out, float* S_host,
                    lightne_netsmf_stats* stats) {
@@ -486,50 +529,42 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   lightne_csr* A = nullptr;
   LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true));
   auto t1 = std::chrono::steady_clock::now();
-  const int64_t n = g->n, k = rp->rank;
-  std::vector<float> S(k, 0.f);
-  int rc = LIGHTNE_OK;
-  if (rp->precision == LIGHTNE_FP64) {
-    // accuracy reference: explicit U, then U sqrt(S)
-    auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
-    rc = U ? lightne_rsvd(ctx, A, rp, U, S.data(), &st.rsvd) : LIGHTNE_ERR_OOM;
-    lightne_csr_free(A);
-    if (rc != LIGHTNE_OK) return rc;
-    LN_CHECK(u_sigma_root_dev(ctx, U, rp->random_project_only ? nullptr : S.data(), n, k, dim, normalize, emb_out));
-  } else {
-    // emb = U[:, :dim] sqrt(S) = B (V S^-1/2)[:, :dim] (PathEmbed.h:371-408),
-    // row-normalised in the same pass when asked
-    if (k > n || k % 16 || k <= 0 || k > 256 || rp->upper ||
-        (rp->precision != LIGHTNE_FP32 && rp->precision != LIGHTNE_BF16 && rp->precision != LIGHTNE_BF16_POWER)) {
-      lightne_csr_free(A);
-      return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_netsmf: bad rank, precision or upper storage");
-    }
-    const float* B = nullptr;
-    std::vector<double> Sd, V;
-    rc = rsvd_core(ctx, A, rp, std::max(1, (int)rp->power_iters), &B, Sd, V, &st.rsvd);
-    lightne_csr_free(A);
-    if (rc != LIGHTNE_OK) return rc;
-    hipStream_t s = ctx->stream;
-    LN_HIP(ctx, hipEventRecord(ctx->ev[9], s));
-    if (rp->random_project_only) {
-      LN_CHECK(u_sigma_root_dev(ctx, B, nullptr, n, k, dim, normalize, emb_out));
-    } else {
-      const std::vector<double> W = right_factor(V, Sd, k, dim, 0.5);
-      LN_CHECK(panel_times_small(ctx, B, n, k, k, W.data(), dim, emb_out, normalize, nullptr));
-      for (int64_t j = 0; j < k; ++j) S[j] = (float)Sd[j];
-    }
-    LN_HIP(ctx, hipEventRecord(ctx->ev[10], s));
-    LN_HIP(ctx, hipEventSynchronize(ctx->ev[10]));
-    const double tg = ev_seconds(ctx->ev[9], ctx->ev[10]);
-    st.rsvd.t_gemm += tg;
-    st.rsvd.t_total += tg;
-  }
+  std::vector<float> S;
+  LN_CHECK(svd_embed(ctx, A, rp, dim, normalize, emb_out, S, &st.rsvd));
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
   auto t2 = std::chrono::steady_clock::now();
   if (S_host) std::copy(S.begin(), S.end(), S_host);
   st.t_sparsify = std::chrono::duration<double>(t1 - t0).count();
   st.t_svd = std::chrono::duration<double>(t2 - t1).count();
   st.t_total = std::chrono::duration<double>(t2 - t0).count();
+  if (stats) *stats = st;
+  return LIGHTNE_OK;
+}
+
+int lightne_ne_zhang(lightne_ctx* ctx, const lightne_graph* g, const lightne_rsvd_params* rp, int64_t dim,
+                     float* emb_out, float* S_host, lightne_rsvd_stats* stats) {
+  if (!ctx || !g || !rp || !emb_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_ne_zhang: NULL argument");
+  if (dim <= 0 || dim > rp->rank) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_ne_zhang: dim must be in [1, rank]");
+  if (g->n <= 0 || !g->offsets || (g->m > 0 && !g->edges) || g->n >= ((int64_t)1 << 31))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_ne_zhang: bad graph (n must be < 2^31)");
+  CallScope scope(ctx);
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  // the matrix keeps the graph's row pointers and neighbour order
+  auto* val = (float*)ctx_buf(ctx, "nz_val", (size_t)std::max<int64_t>(g->m, 1) * sizeof(float));
+  if (!val) return LIGHTNE_ERR_OOM;
+  LN_CHECK(ne_zhang_values(ctx, g, val));
+  auto* A = new lightne_csr();
+  A->n = g->n;
+  A->nnz = g->m;
+  A->owns = false;
+  A->row_ptr = const_cast<int64_t*>(g->offsets);
+  A->col = reinterpret_cast<int32_t*>(const_cast<uint32_t*>(g->edges));
+  A->val = val;
+  lightne_rsvd_stats st{};
+  std::vector<float> S;
+  LN_CHECK(svd_embed(ctx, A, rp, dim, 1, emb_out, S, &st));  // PathEmbed.h:459 normalises
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (S_host) std::copy(S.begin(), S.end(), S_host);
   if (stats) *stats = st;
   return LIGHTNE_OK;
 }

@@ -185,6 +185,29 @@ def spmm_f32(rp, col, val, X: np.ndarray, threads: int = 0) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# the ne_zhang_et_al matrix (PathEmbed.h:410-444)
+# ---------------------------------------------------------------------------
+def ne_zhang_matrix(offsets: np.ndarray, edges: np.ndarray):
+    """(rp, col, val) of PathEmbed.h:412-444: rows and columns are the graph's
+    (adjacency order), val_uv = float(log(1/d_u) - log(neg_v / sum(neg))) with
+    neg_j = (sum_{i in N(j)} 1/d_i)^0.75 (:418-427), all in FP64 until the
+    final cast (:441)."""
+    off = np.asarray(offsets, np.int64)
+    adj = np.asarray(edges, np.int64)
+    n = off.shape[0] - 1
+    deg = np.diff(off).astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(off))
+    inv = np.zeros(n)
+    inv[deg > 0] = 1.0 / deg[deg > 0]
+    neg = np.zeros(n)
+    np.add.at(neg, rows, inv[adj])  # sum over the neighbours i of j of 1/d_i
+    neg = neg ** 0.75
+    neg_sum = neg.sum()
+    val = (np.log(1.0 / deg[rows]) - np.log(neg[adj] / neg_sum)).astype(np.float32)
+    return off.copy(), adj.astype(np.int32), val
+
+
+# ---------------------------------------------------------------------------
 # stage 2: randomized SVD, FP64 (mklredsvd.cc:93-403)
 # ---------------------------------------------------------------------------
 class _Op:

@@ -41,3 +41,18 @@ def test_cli_matches_api(gpu, tmp_path):
                          capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, out.stderr
     np.testing.assert_array_equal(np.fromfile(pro2, np.float32), p1.ravel())
+
+
+def test_cli_ne_zhang_matches_api(gpu, tmp_path):
+    """-ne_method ne_zhang_et_al (LightNE.cc:106) writes the NE_Zhang_et_al embedding."""
+    hg = L.chung_lu(3000, 2.1, 12.0, 300.0, 5)
+    adj = str(tmp_path / "g.adj")
+    L.write_adjacency_graph(adj, hg)
+    ne = str(tmp_path / "ne.emb")
+    out = subprocess.run([CLI, "-ne_method", "ne_zhang_et_al", "-rank", "64", "-dim", "32", "-rounds", "1",
+                          "-ne_out", ne, "-s", adj], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    assert "### Running Time" in out.stdout
+    e1 = np.fromfile(ne, np.float32).reshape(-1, 32)
+    emb, S, _ = L.NE_Zhang_et_al(hg.to_device(), rank=64, dim=32)
+    np.testing.assert_array_equal(e1, emb.cpu().numpy())

@@ -210,3 +210,36 @@ def test_rsvd_oracle_exact_rank():
     np.testing.assert_allclose(S[:r], np.sort(np.abs(lam))[::-1], rtol=1e-5)
     assert S[r:].max() < 1e-5
     assert H.principal_angles(U[:, :r], Q) < 1e-5
+
+
+def _sym_graph(n, pairs):
+    adj = [[] for _ in range(n)]
+    for u, v in pairs:
+        adj[u].append(v)
+        adj[v].append(u)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum([len(a) for a in adj])
+    return off, np.array([v for a in adj for v in a], np.uint32)
+
+
+def test_ne_zhang_matrix_regular_graph_closed_form():
+    """d-regular graph: neg_j = (d * 1/d)^0.75 = 1, sum = n, every entry
+    log(1/d) - log(1/n) = log(n/d) (PathEmbed.h:418-441)."""
+    n = 12
+    off, edg = _sym_graph(n, [(i, (i + 1) % n) for i in range(n)] + [(i, (i + 6) % n) for i in range(6)])
+    assert np.all(np.diff(off) == 3)
+    rp, col, val = O.ne_zhang_matrix(off, edg)
+    assert np.array_equal(rp, off) and np.array_equal(col, edg.astype(np.int32))
+    np.testing.assert_allclose(val, np.float32(np.log(n / 3.0)), rtol=1e-7)
+
+
+def test_ne_zhang_matrix_star_closed_form():
+    """Star K_{1,m}: neg_0 = m^0.75, neg_leaf = m^-0.75, S = m^0.75 + m^0.25;
+    centre->leaf = log S - 0.25 log m, leaf->centre = log S - 0.75 log m (a
+    swapped d_u / d_v or a missing exponent changes both)."""
+    m = 7
+    off, edg = _sym_graph(m + 1, [(0, i) for i in range(1, m + 1)])
+    rp, col, val = O.ne_zhang_matrix(off, edg)
+    S = m ** 0.75 + m ** 0.25
+    np.testing.assert_allclose(val[:m], np.log(S) - 0.25 * np.log(m), rtol=1e-6)
+    np.testing.assert_allclose(val[m:], np.log(S) - 0.75 * np.log(m), rtol=1e-6)
