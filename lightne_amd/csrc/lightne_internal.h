@@ -124,10 +124,11 @@ struct DevMem {
 int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 
 // stage 1 (sparsify.hip): sampler, table, trunc-log filter, CSR.  ctx_owned:
-// the CSR arrays are context buffers (reused by the next call) and the handle
-// does not free them.
+// the CSR arrays are context memory (reused by the next call) and the handle
+// does not free them; they then stay clear of the first front_reserve bytes
+// of the workspace, which the caller's next stage may carve.
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
-                  lightne_sparsify_stats* stats, bool ctx_owned);
+                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve);
 // values of the ne_zhang_et_al matrix on the graph's pattern (nezhang.hip)
 int ne_zhang_values(lightne_ctx* ctx, const lightne_graph* g, float* val);
 

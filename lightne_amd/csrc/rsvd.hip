@@ -527,7 +527,11 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   lightne_netsmf_stats st{};
   auto t0 = std::chrono::steady_clock::now();
   lightne_csr* A = nullptr;
-  LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true));
+  // stage 2 carves its panels from the workspace front (rsvd_core)
+  const size_t pnl = ((size_t)g->n * rp->rank * sizeof(float) + 255) & ~(size_t)255;
+  const bool bf = rp->precision == LIGHTNE_BF16 || rp->precision == LIGHTNE_BF16_POWER;
+  const size_t front = rp->precision == LIGHTNE_FP64 ? 4 * 2 * pnl : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
+  LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front));
   auto t1 = std::chrono::steady_clock::now();
   std::vector<float> S;
   LN_CHECK(svd_embed(ctx, A, rp, dim, normalize, emb_out, S, &st.rsvd));

@@ -253,11 +253,11 @@ __global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ 
 __global__ void k_gather_counts(const Slot* __restrict__ tab, const double* __restrict__ wtab,
                                 const uint32_t* __restrict__ idx, int64_t nk, uint32_t* __restrict__ cnt,
                                 double* __restrict__ w) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
-  const uint32_t s = idx[i];
-  cnt[i] = tab[s].cnt;
-  if (w) w[i] = wtab ? wtab[s] : (double)tab[s].cnt;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = idx[i];
+    cnt[i] = tab[s].cnt;
+    if (w) w[i] = wtab ? wtab[s] : (double)tab[s].cnt;
+  }
 }
 
 struct TruncArgs {
@@ -369,13 +369,13 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_
 // row_ptr by boundary detection over the sorted keys; col extraction
 __global__ void k_csr_finish(const uint64_t* __restrict__ skey, int64_t nnz, int64_t n, int colbits,
                              int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nnz) return;
   const uint64_t cmask = (colbits >= 64) ? ~0ull : ((1ull << colbits) - 1);
-  const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
-  const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
-  for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = i;
-  if (i < nnz) col[i] = (int32_t)(skey[i] & cmask);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nnz; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
+    const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
+    for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = i;
+    if (i < nnz) col[i] = (int32_t)(skey[i] & cmask);
+  }
 }
 
 int bits_for(uint64_t n) {
@@ -394,6 +394,9 @@ int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
   if (!g || g->n <= 0 || g->m < 0 || !g->offsets || (g->m > 0 && !g->edges))
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "invalid graph descriptor");
   if (g->n >= (int64_t)1 << 32) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 vertices");
+  // the sampler runs one lane per directed edge in one launch
+  if (g->m >= ((int64_t)1 << 32) - 256)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 directed edges");
   return LIGHTNE_OK;
 }
 
@@ -427,13 +430,16 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
 
   Slot* tab = nullptr;
   double* wtab = nullptr;
+  // workspace front: the table, then each edge's source vertex (dead after
+  // sampling)
+  uint32_t* src = nullptr;
   WsCarve wc;
   wc.add((void**)&tab, cap * sizeof(Slot));
   if (p->sample) wc.add((void**)&wtab, cap * sizeof(double));
+  wc.add((void**)&src, std::max<int64_t>(g->m, 1) * sizeof(uint32_t));
   LN_CHECK(wc.commit(ctx));
-  auto* src = (uint32_t*)ctx_buf(ctx, "edge_src", std::max<int64_t>(g->m, 1) * sizeof(uint32_t));
   auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
-  if (!src || !stats) return LIGHTNE_ERR_OOM;
+  if (!stats) return LIGHTNE_ERR_OOM;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
@@ -519,7 +525,7 @@ int table_totals(lightne_ctx* ctx, const Slot* tab, const double* wtab, uint64_t
 // context buffers reused by the next call (the pipeline's own use); else the
 // handle owns them.
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
-                  lightne_sparsify_stats* stats, bool ctx_owned) {
+                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve) {
   *out = nullptr;
   lightne_sparsify_stats st{};
   Slot* tab;
@@ -568,13 +574,30 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   for (uint32_t v : hk) st.kept_upper += v;
   st.nnz = (uint64_t)nnz;
   const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
-  auto* skey = (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
+  // Workspace layout when it is large enough: the table and then the sort
+  // keys at the front, the CSR at the back, clear of the front region the
+  // next stage carves (front_reserve), so that a repeated pipeline holds one
+  // allocation.  Otherwise the keys (and a context-owned CSR) are buffers of
+  // their own.
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t ws_bytes = ctx->bufs["ws"].bytes;
+  const size_t tab_bytes = al(cap * sizeof(Slot)) + (wtab ? al(cap * sizeof(double)) : 0);
+  const size_t csr_bytes = al((g->n + 1) * 8) + 2 * al(nz * 4);
+  const size_t front = std::max(tab_bytes + al(nz * 8), front_reserve);
+  const bool in_ws = ctx_owned && front + csr_bytes <= ws_bytes;
+  char* ws = (char*)tab;  // the table starts the workspace
+  uint64_t* skey = in_ws ? (uint64_t*)(ws + tab_bytes) : (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
   if (!skey) return LIGHTNE_ERR_OOM;
   DevMem mrp, mcol, mval;
   int64_t* rp = nullptr;
   int32_t* col = nullptr;
   float* val = nullptr;
-  if (ctx_owned) {
+  if (in_ws) {
+    char* back = ws + (ws_bytes - csr_bytes);
+    rp = (int64_t*)back;
+    col = (int32_t*)(back + al((g->n + 1) * 8));
+    val = (float*)(back + al((g->n + 1) * 8) + al(nz * 4));
+  } else if (ctx_owned) {
     rp = (int64_t*)ctx_buf(ctx, "s1_rp", (g->n + 1) * 8);
     col = (int32_t*)ctx_buf(ctx, "s1_col", nz * 4);
     val = (float*)ctx_buf(ctx, "s1_val", nz * 4);
@@ -593,16 +616,15 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   LN_LAUNCH(ctx);
   // the table is dead once the kept entries are out: the sort's ping-pong
   // buffers reuse its memory when they fit there
-  const size_t ws_bytes = ctx->bufs["ws"].bytes;
-  const size_t kbytes = (nz * 8 + 255) & ~(size_t)255;
+  const size_t kbytes = al(nz * 8);
   uint64_t* k_tmp = nullptr;
   float* v_tmp = nullptr;
-  if (kbytes + nz * 4 <= ws_bytes) {
+  if (kbytes + nz * 4 <= tab_bytes) {
     k_tmp = (uint64_t*)tab;
     v_tmp = (float*)((char*)tab + kbytes);
   }
   LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp));
-  hipLaunchKernelGGL(k_csr_finish, dim3((unsigned)((nnz + 1 + 255) / 256)), dim3(256), 0, s, (const uint64_t*)skey,
+  hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)skey,
                      nnz, g->n, colbits, rp, col);
   LN_LAUNCH(ctx);
   LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
@@ -676,7 +698,7 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   LN_LAUNCH(ctx);
   LN_CHECK(radix_sort_pairs_u32(ctx, (uint64_t*)mk.p, (uint32_t*)mi.p, (int64_t)nk, 32 + bits_for((uint64_t)g->n)));
   if (nk) {
-    hipLaunchKernelGGL(k_gather_counts, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, ctx->stream, tab, wtab,
+    hipLaunchKernelGGL(k_gather_counts, dim3(grid_blocks((int64_t)nk, 256)), dim3(256), 0, ctx->stream, tab, wtab,
                        (const uint32_t*)mi.p, (int64_t)nk, (uint32_t*)mc.p, (double*)mw.p);
     LN_LAUNCH(ctx);
   }
@@ -720,7 +742,7 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
                      lightne_sparsify_stats* stats) {
   if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify: NULL argument");
   CallScope scope(ctx);
-  return sparsify_impl(ctx, g, p, out, stats, false);
+  return sparsify_impl(ctx, g, p, out, stats, false, 0);
 }
 
 }  // extern "C"
