@@ -153,7 +153,10 @@ typedef struct {
   int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 / LIGHTNE_FP64 panels */
   int32_t random_project_only; /* -random_project_only (mklredsvd.cc:254-258) */
   int32_t upper;            /* A holds only the upper triangle (symmetric)    */
-  uint64_t seed;            /* Gaussian sketch seed (reference: VSL seed 777) */
+  uint64_t seed;            /* sketch seed (reference: VSL seed 777)          */
+  int32_t sparse_project;   /* -sparse_project: very sparse random projection
+                               instead of the Gaussian sketch (mklredsvd.cc:148-226) */
+  float sparse_project_s;   /* -sparse_project_s: 1/s density; < 1 selects sqrt(n) */
 } lightne_rsvd_params;
 
 typedef struct {

@@ -66,6 +66,7 @@ class RsvdParams(C.Structure):
     _fields_ = [
         ("rank", C.c_int64), ("power_iters", C.c_int32), ("precision", C.c_int32),
         ("random_project_only", C.c_int32), ("upper", C.c_int32), ("seed", C.c_uint64),
+        ("sparse_project", C.c_int32), ("sparse_project_s", C.c_float),
     ]
 
 

@@ -306,12 +306,13 @@ class MKLRedSVD:
 
     def __init__(self, csr: CSR, rank: int, power_iters: int = 1, precision: int = FP32,
                  random_project_only: bool = False, upper: bool = False, seed: int = 777,
-                 ctx: Optional[Context] = None):
+                 sparse_project: bool = False, sparse_project_s: float = 0.0, ctx: Optional[Context] = None):
         self.csr = csr
         self.ctx = ctx or csr.ctx
         self.params = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
                                  random_project_only=int(bool(random_project_only)), upper=int(bool(upper)),
-                                 seed=seed)
+                                 seed=seed, sparse_project=int(bool(sparse_project)),
+                                 sparse_project_s=sparse_project_s)
         self.matU = None
         self.S = None
         self.stats = None
@@ -345,15 +346,18 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
            sample: bool = False, rank: int = 256, dim: int = 128, analyze: bool = False, table_size: int = 0,
            negative: int = 1, normalize: bool = False, sample_ratio: float = 1.0, mem_ratio: float = 1.0,
            random_project_only: bool = False, seed: int = 0, svd_seed: int = 777, power_iters: int = 1,
-           precision: int = FP32, max_table_bytes: int = 0, ctx: Optional[Context] = None):
-    """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats)."""
+           precision: int = FP32, max_table_bytes: int = 0, sparse_project: bool = False,
+           sparse_project_s: float = 100.0, ctx: Optional[Context] = None):
+    """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats).
+    ``sparse_project_s`` defaults to 100 as LightNE.cc:50 does."""
     del walks_per_edge, analyze
     torch = _torch()
     ctx = ctx or context()
     sp = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
                           max_table_bytes)
     rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
-                    random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed)
+                    random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed,
+                    sparse_project=int(bool(sparse_project)), sparse_project_s=sparse_project_s)
     emb = torch.empty((graph.n, dim), dtype=torch.float32, device=f"cuda:{ctx.device}")
     S = np.zeros(rank, np.float32)
     st = NetsmfStats()

@@ -186,10 +186,6 @@ int main(int argc, char** argv) {
     fprintf(stderr, "lightne: -ne_method must be netsmf or ne_zhang_et_al\n");
     return 2;
   }
-  if (sparse_project) {
-    fprintf(stderr, "lightne: -sparse_project is not available in this build\n");
-    return 2;
-  }
   if (precision != "fp32" && precision != "bf16" && precision != "bf16_power" && precision != "fp64") {
     fprintf(stderr, "lightne: -precision must be fp32, bf16, bf16_power or fp64\n");
     return 2;
@@ -296,6 +292,8 @@ int main(int argc, char** argv) {
       rp.random_project_only = random_project_only;
       rp.upper = upper;
       rp.seed = svd_seed;
+      rp.sparse_project = sparse_project;
+      rp.sparse_project_s = sparse_project_s;
       lightne_netsmf_stats st{};
       std::vector<float> S(rank);
       printf("# seed = %llu\n", (unsigned long long)seed);

@@ -129,6 +129,10 @@ int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 // of the workspace, which the caller's next stage may carve.
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
                   lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve);
+// Y = A Omega for the very sparse random projection Omega (sparseproj.hip);
+// Y is n x k FP32, or FP64 when f64
+int sparse_project_sketch(lightne_ctx* ctx, const int64_t* rp, const int32_t* col, const float* val, int64_t n,
+                          int64_t k, uint64_t seed, double s_param, void* Y, bool f64);
 // values of the ne_zhang_et_al matrix on the graph's pattern (nezhang.hip)
 int ne_zhang_values(lightne_ctx* ctx, const lightne_graph* g, float* val);
 

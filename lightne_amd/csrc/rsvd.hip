@@ -232,12 +232,17 @@ static int rsvd64(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p
   auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
   double t_spmm = 0, t_qr = 0;
   LN_HIP(ctx, hipEventRecord(e[0], s));
-  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, (float*)T));
-  LN_CHECK(f32_to_f64(ctx, (const float*)T, P0, n * k));
+  if (!p->sparse_project) {
+    LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, (float*)T));
+    LN_CHECK(f32_to_f64(ctx, (const float*)T, P0, n * k));
+  }
   LN_HIP(ctx, hipEventRecord(e[1], s));
   LN_CHECK(csr_plan(ctx, A));
   LN_HIP(ctx, hipEventRecord(e[2], s));
-  LN_CHECK(csr_mm64(ctx, A, P0, k, P1));
+  if (p->sparse_project)
+    LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, true));
+  else
+    LN_CHECK(csr_mm64(ctx, A, P0, k, P1));
   LN_HIP(ctx, hipEventRecord(e[3], s));
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
@@ -317,12 +322,17 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
 
   LN_HIP(ctx, hipEventRecord(e[0], s));
-  LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P2));
-  if (bf) LN_CHECK(f32_to_bf16(ctx, P2, PB, n * k));
+  if (!p->sparse_project) {
+    LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P2));
+    if (bf) LN_CHECK(f32_to_bf16(ctx, P2, PB, n * k));
+  }
   LN_HIP(ctx, hipEventRecord(e[1], s));
   LN_CHECK(csr_plan(ctx, A));
   LN_HIP(ctx, hipEventRecord(e[2], s));
-  LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1));
+  if (p->sparse_project)
+    LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, false));
+  else
+    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1));
   LN_HIP(ctx, hipEventRecord(e[3], s));
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);

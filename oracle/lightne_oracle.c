@@ -518,4 +518,25 @@ void orc_gaussian(uint64_t seed, uint32_t stream, uint64_t count, float* out) {
   }
 }
 
+/* Very sparse random projection matrix (mklredsvd.cc:148-189), dense n x k
+ * FP64: entry (i, c) is present when the first Philox word of counter
+ * (i, c, stream 2) is below thresh = floor(2^32 / s), with value -sqrt(s) when
+ * the second word is odd, else +sqrt(s); s (float) = s_param if >= 1, else
+ * sqrt(n).  Same distribution as the reference's binomial row counts plus a
+ * Fisher-Yates column choice. */
+void orc_sparse_projection(uint64_t n, uint64_t k, uint64_t seed, double s_param, double* out) {
+  const float s = s_param >= 1.0 ? (float)s_param : (float)sqrt((double)n);
+  const double density = 1.0 / (double)s;
+  const uint32_t thresh = density >= 1.0 ? 0xffffffffu : (uint32_t)(density * 4294967296.0);
+  const double v = (double)sqrtf(s);
+  uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#pragma omp parallel for schedule(static)
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t c = 0; c < k; ++c) {
+      uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), (uint32_t)c, 2u}, w[4];
+      orc_philox4x32(ctr, key, w);
+      out[i * k + c] = (w[0] < thresh) ? ((w[1] & 1u) ? -v : v) : 0.0;
+    }
+}
+
 int orc_max_threads(void) { return omp_get_max_threads(); }

@@ -58,6 +58,7 @@ def load():
         "orc_aggregate_f64": (None, [u64, P, P, P, u64, P, C.c_int]),
         "orc_philox4x32": (None, [P, P, P]),
         "orc_gaussian": (None, [u64, C.c_uint32, u64, P]),
+        "orc_sparse_projection": (None, [u64, u64, u64, f64, P]),
         "orc_max_threads": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
@@ -160,6 +161,14 @@ def gaussian(seed: int, stream: int, count: int) -> np.ndarray:
     return out
 
 
+def sparse_projection(n: int, k: int, seed: int, s_param: float = 0.0) -> np.ndarray:
+    """Dense n x k FP64 very sparse random projection (mklredsvd.cc:148-189):
+    Bernoulli(1/s) entries of value +-sqrt(s) from Philox counters (row, column)."""
+    out = np.empty((n, k), np.float64)
+    load().orc_sparse_projection(n, k, seed & 0xFFFFFFFFFFFFFFFF, float(s_param), _p(out))
+    return out
+
+
 def spmm_f64(rp, col, val, X: np.ndarray, threads: int = 0) -> np.ndarray:
     lib = load()
     rp = np.ascontiguousarray(rp, np.int64)
@@ -221,12 +230,15 @@ class _Op:
 
 
 def rsvd(rp, col, val, rank: int, seed: int = 777, power_iters: int = 1, random_project_only: bool = False,
-         threads: int = 0, dtype=np.float64):
+         threads: int = 0, dtype=np.float64, sparse_project: bool = False, sparse_project_s: float = 0.0):
     """Returns (U n x rank, S rank) following mklredsvd.cc:141-400; S is None
     with random_project_only."""
     n = rp.shape[0] - 1
     A = _Op(rp, col, val, threads)
-    Om = gaussian(seed, 0, n * rank).reshape(n, rank).astype(dtype)
+    if sparse_project:
+        Om = sparse_projection(n, rank, seed, sparse_project_s).astype(dtype)
+    else:
+        Om = gaussian(seed, 0, n * rank).reshape(n, rank).astype(dtype)
     Y = A @ Om
     if random_project_only:
         return Y, None

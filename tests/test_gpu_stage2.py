@@ -202,3 +202,35 @@ def test_rsvd_fp64_form_matches_oracle(gpu, q):
     print(f"FP64 form q={q}: max rel sv err={rel.max():.2e} angle={ang:.2e}")
     assert rel.max() <= 1e-7  # S is returned as FP32
     assert ang <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_sparse_projection_sketch_matches_oracle(gpu, prec):
+    """-sparse_project: Y = A Omega with the very sparse Omega formed without a
+    dense panel (short, long and hub rows) against the oracle's FP64 product."""
+    n, k = 6000, 144
+    rp, col, val = _csr_np(_power_law_sym(n, 4))
+    csr = L.CSR.from_arrays(rp, col, val, ctx=gpu)
+    p = L.FP32 if prec == "fp32" else L.FP64
+    Y = L.MKLRedSVD(csr, k, random_project_only=True, precision=p, seed=31, sparse_project=True,
+                    sparse_project_s=16.0, ctx=gpu).run().matU.cpu().numpy()
+    ref = O.spmm_f64(rp, col, val, O.sparse_projection(n, k, 31, 16.0))
+    assert np.count_nonzero(ref) > 0.5 * ref.size
+    # FP64 accumulation, one rounding to FP32 at the end
+    scale = np.abs(ref).max()
+    assert np.abs(Y - ref).max() <= 2e-7 * scale
+
+
+def test_rsvd_sparse_projection_c1(gpu):
+    """Config-1 sparsifier with -sparse_project (s = sqrt(n)): singular values and
+    top-d subspace against the oracle's FP64 run with the same Omega."""
+    hg, csr = _c1_sparsifier(gpu)
+    k, d = 144, 128
+    svd = L.MKLRedSVD(csr, k, power_iters=1, seed=5, sparse_project=True, ctx=gpu).run()
+    rp, col, val = csr.to_numpy()
+    U64, S64 = O.rsvd(rp, col, val, k, seed=5, power_iters=1, sparse_project=True)
+    rel = np.abs(svd.S[:d] - S64[:d]) / S64[:d]
+    ang = H.principal_angles(svd.matU.cpu().numpy()[:, :d], U64[:, :d])
+    print(f"sparse_project: max rel sv err={rel.max():.2e} angle={ang:.2e}")
+    assert rel.max() <= 1e-4
+    assert ang <= 1e-3

@@ -243,3 +243,23 @@ def test_ne_zhang_matrix_star_closed_form():
     S = m ** 0.75 + m ** 0.25
     np.testing.assert_allclose(val[:m], np.log(S) - 0.25 * np.log(m), rtol=1e-6)
     np.testing.assert_allclose(val[m:], np.log(S) - 0.75 * np.log(m), rtol=1e-6)
+
+
+def test_sparse_projection_distribution():
+    """Very sparse random projection (mklredsvd.cc:148-189): entries +-sqrt(s)
+    with probability 1/s (s = sqrt(n) by default, -sparse_project_s when >= 1),
+    signs balanced, E[Omega_ic^2] = 1 (the Johnson-Lindenstrauss scaling)."""
+    for n, k, s_param in [(20000, 144, 0.0), (4000, 64, 16.0)]:
+        M = O.sparse_projection(n, k, 777, s_param)
+        s = float(np.float32(s_param if s_param >= 1 else np.sqrt(n)))
+        p = 1.0 / s
+        frac = (M != 0).mean()
+        sd = np.sqrt(p * (1 - p) / (n * k))
+        assert abs(frac - p) < 5 * sd, (frac, p)
+        nz = M[M != 0]
+        np.testing.assert_allclose(np.abs(nz), np.sqrt(np.float32(s)), rtol=1e-6)
+        assert abs((nz > 0).mean() - 0.5) < 5 * 0.5 / np.sqrt(nz.size)
+        assert abs((M ** 2).mean() - 1.0) < 5 * np.sqrt((s - 1) / (n * k))
+    # different seeds draw different matrices; the same seed the same one
+    assert not np.array_equal(O.sparse_projection(500, 32, 1, 4.0), O.sparse_projection(500, 32, 2, 4.0))
+    assert np.array_equal(O.sparse_projection(500, 32, 1, 4.0), O.sparse_projection(500, 32, 1, 4.0))
