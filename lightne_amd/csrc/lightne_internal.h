@@ -133,6 +133,9 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
 // Y is n x k FP32, or FP64 when f64
 int sparse_project_sketch(lightne_ctx* ctx, const int64_t* rp, const int32_t* col, const float* val, int64_t n,
                           int64_t k, uint64_t seed, double s_param, void* Y, bool f64);
+// full symmetric CSR from upper-triangle storage (sparsify.hip); F's arrays are
+// context buffers, F owns nothing
+int csr_expand_upper(lightne_ctx* ctx, const lightne_csr* U, lightne_csr* F);
 // values of the ne_zhang_et_al matrix on the graph's pattern (nezhang.hip)
 int ne_zhang_values(lightne_ctx* ctx, const lightne_graph* g, float* val);
 

@@ -411,13 +411,25 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank must be a multiple of 16 in [16, 256] (got %lld)",
                     (long long)k);
   if (k > n) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank exceeds n");
-  if (p->upper) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: upper-triangle storage is not supported");
   if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_FP64 &&
       p->precision != LIGHTNE_BF16_POWER)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
   const int q = std::max(1, (int)p->power_iters);
   CallScope scope(ctx);
   LN_HIP(ctx, hipSetDevice(ctx->device));
+  if (p->upper) {
+    // symmetric-upper storage (mklredsvd.cc:95-98): run on the expanded matrix
+    lightne_csr F;
+    LN_CHECK(csr_expand_upper(ctx, A, &F));
+    lightne_rsvd_params pf = *p;
+    pf.upper = 0;
+    const int rc = lightne_rsvd(ctx, &F, &pf, U_out, S_host, stats);
+    if (F.plan) {
+      spmm_plan_free(*F.plan);
+      delete F.plan;
+    }
+    return rc;
+  }
   if (p->precision == LIGHTNE_FP64) return rsvd64(ctx, A, p, q, U_out, S_host, stats);
   lightne_rsvd_stats st{};
   const float* B = nullptr;
@@ -537,11 +549,22 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   lightne_netsmf_stats st{};
   auto t0 = std::chrono::steady_clock::now();
   lightne_csr* A = nullptr;
+  // -upper 1 (PathEmbed.h:284-309 with mklredsvd.cc:95-98) denotes the same
+  // symmetric matrix: stage 1 writes both triangles at once and stage 2 runs
+  // on it; the reported nnz is the upper storage's, as the reference's
+  lightne_sparsify_params spf = *sp;
+  lightne_rsvd_params rpf = *rp;
+  const bool upper = sp->upper || rp->upper;
+  spf.upper = 0;
+  rpf.upper = 0;
+  sp = &spf;
+  rp = &rpf;
   // stage 2 carves its panels from the workspace front (rsvd_core)
   const size_t pnl = ((size_t)g->n * rp->rank * sizeof(float) + 255) & ~(size_t)255;
   const bool bf = rp->precision == LIGHTNE_BF16 || rp->precision == LIGHTNE_BF16_POWER;
   const size_t front = rp->precision == LIGHTNE_FP64 ? 4 * 2 * pnl : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
   LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front));
+  if (upper) st.sparsify.nnz = st.sparsify.kept_upper;
   auto t1 = std::chrono::steady_clock::now();
   std::vector<float> S;
   LN_CHECK(svd_embed(ctx, A, rp, dim, normalize, emb_out, S, &st.rsvd));

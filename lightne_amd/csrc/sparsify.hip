@@ -519,6 +519,37 @@ int table_totals(lightne_ctx* ctx, const Slot* tab, const double* wtab, uint64_t
   return LIGHTNE_OK;
 }
 
+// symmetric expansion of upper-triangle storage: entry e = (i, j, v) emits
+// (i, j) and, off the diagonal, (j, i); a diagonal entry's second key is the
+// sentinel ~0, which sorts last
+__global__ void k_up_emit(const int64_t* __restrict__ rp, const int32_t* __restrict__ col,
+                          const float* __restrict__ val, int64_t n, int64_t nnz, int colbits,
+                          uint64_t* __restrict__ key, float* __restrict__ kv, unsigned long long* __restrict__ ndiag) {
+  __shared__ unsigned long long s_d;
+  if (threadIdx.x == 0) s_d = 0;
+  __syncthreads();
+  unsigned long long d = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = n - 1;  // row of e: largest i with rp[i] <= e
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= e)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const uint64_t i = (uint64_t)lo, j = (uint64_t)(uint32_t)col[e];
+    key[2 * e] = (i << colbits) | j;
+    kv[2 * e] = val[e];
+    key[2 * e + 1] = (i == j) ? ~0ull : ((j << colbits) | i);
+    kv[2 * e + 1] = val[e];
+    d += (i == j);
+  }
+  atomicAdd(&s_d, d);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_d) atomicAdd(ndiag, s_d);
+}
+
 }  // namespace
 
 // Sampler, table, trunc-log filter and CSR.  ctx_owned: the CSR arrays are
@@ -643,6 +674,47 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   return LIGHTNE_OK;
 }
 
+
+// Full symmetric CSR from upper-triangle storage (the matrix the reference's
+// SPARSE_MATRIX_TYPE_SYMMETRIC / SPARSE_FILL_MODE_UPPER descriptor denotes,
+// mklredsvd.cc:95-98): A = U + U^T - diag(U).  The arrays are context
+// buffers ("up_*"), valid until the next expansion.
+int csr_expand_upper(lightne_ctx* ctx, const lightne_csr* U, lightne_csr* F) {
+  const int64_t n = U->n, nnz = U->nnz;
+  const int colbits = std::max(1, bits_for((uint64_t)n + 1));  // 2^colbits > n: no key equals the sentinel
+  if (2 * colbits > 64) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "upper expansion: n too large");
+  const size_t m2 = (size_t)std::max<int64_t>(2 * nnz, 1);
+  auto* key = (uint64_t*)ctx_buf(ctx, "up_key", m2 * 8);
+  auto* kv = (float*)ctx_buf(ctx, "up_val", m2 * 4);
+  auto* rp = (int64_t*)ctx_buf(ctx, "up_rp", (n + 1) * 8);
+  auto* col = (int32_t*)ctx_buf(ctx, "up_col", m2 * 4);
+  auto* nd = (unsigned long long*)ctx_buf(ctx, "up_ndiag", 8);
+  if (!key || !kv || !rp || !col || !nd) return LIGHTNE_ERR_OOM;
+  hipStream_t s = ctx->stream;
+  LN_HIP(ctx, hipMemsetAsync(nd, 0, 8, s));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_up_emit, dim3(grid_blocks(nnz, 256)), dim3(256), 0, s, U->row_ptr, U->col, U->val, n, nnz,
+                       colbits, key, kv, nd);
+    LN_LAUNCH(ctx);
+    LN_CHECK(radix_sort_pairs(ctx, key, kv, 2 * nnz, 2 * colbits));
+  }
+  unsigned long long ndiag = 0;
+  LN_HIP(ctx, hipMemcpyAsync(&ndiag, nd, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  const int64_t nf = 2 * nnz - (int64_t)ndiag;  // the sentinels sort past position nf
+  hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nf + 1, 256)), dim3(256), 0, s, (const uint64_t*)key, nf, n,
+                     colbits, rp, col);
+  LN_LAUNCH(ctx);
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  F->n = n;
+  F->nnz = nf;
+  F->row_ptr = rp;
+  F->col = col;
+  F->val = kv;  // the first nf sorted values
+  F->owns = false;
+  F->plan = nullptr;
+  return LIGHTNE_OK;
+}
 
 extern "C" {
 

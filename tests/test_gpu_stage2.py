@@ -234,3 +234,31 @@ def test_rsvd_sparse_projection_c1(gpu):
     print(f"sparse_project: max rel sv err={rel.max():.2e} angle={ang:.2e}")
     assert rel.max() <= 1e-4
     assert ang <= 1e-3
+
+
+def test_upper_storage_is_the_same_matrix(gpu):
+    """-upper 1 (PathEmbed.h:284-309, mklredsvd.cc:95-98): upper-triangle storage
+    denotes U + U^T - diag(U).  The randomized SVD of the upper CSR equals the
+    one of the full CSR bit for bit, and NetSMF gives the same embedding with the
+    upper storage's nnz reported."""
+    hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
+    g = hg.to_device()
+    up, su = L.generate_trunc_log_matrix(g, 1, walk_len=10, sample_ratio=ratio, upper=True, ctx=gpu)
+    full, sf = L.generate_trunc_log_matrix(g, 1, walk_len=10, sample_ratio=ratio, ctx=gpu)
+    assert su["nnz"] == sf["kept_upper"]
+    assert sf["nnz"] == 2 * su["nnz"] - _diag_count(up)
+    a = L.MKLRedSVD(up, 144, upper=True, seed=3, ctx=gpu).run()
+    b = L.MKLRedSVD(full, 144, seed=3, ctx=gpu).run()
+    np.testing.assert_array_equal(a.S, b.S)
+    np.testing.assert_array_equal(a.matU.cpu().numpy(), b.matU.cpu().numpy())
+    e1, S1, st1 = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, upper=True, ctx=gpu)
+    e0, S0, st0 = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, ctx=gpu)
+    np.testing.assert_array_equal(e1.cpu().numpy(), e0.cpu().numpy())
+    assert st1["sparsify"]["nnz"] == su["nnz"]
+
+
+def _diag_count(csr):
+    rp, col, _ = csr.to_numpy()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return int(np.count_nonzero(rows == col))
