@@ -158,10 +158,12 @@ double ev_seconds(hipEvent_t a, hipEvent_t b);
 int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n);
 int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n);
 // stable LSD radix sort of (key, val) pairs on the low `bits` bits of key.
-// keys/vals hold the result on return; the ping-pong buffers (n keys, n vals)
-// are k_tmp / v_tmp when given, else context buffers.
+// Sorts on key bits [lo_bit, lo_bit + bits).  The ping-pong buffers (n keys,
+// n vals) are k_tmp / v_tmp when given, else context buffers.  keys/vals hold
+// the result on return, unless k_res / v_res are given: then they receive
+// wherever the last pass left it (no copy back).
 int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp = nullptr,
-                     float* v_tmp = nullptr);
+                     float* v_tmp = nullptr, int lo_bit = 0, uint64_t** k_res = nullptr, float** v_res = nullptr);
 int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits);
 
 // ---------------------------------------------------------------------------

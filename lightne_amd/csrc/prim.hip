@@ -229,7 +229,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
 }
 
 template <typename V>
-int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits, uint64_t* k2, V* v2) {
+int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits, uint64_t* k2, V* v2, int lo,
+                    uint64_t** k_res, V** v_res) {
+  if (k_res) *k_res = keys;
+  if (v_res) *v_res = vals;
   if (n <= 1 || bits <= 0) return LIGHTNE_OK;
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   if (!k2 || !v2) {
@@ -241,7 +244,7 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
   if (!k2 || !v2 || !hist || !offs) return LIGHTNE_ERR_OOM;
   uint64_t *ka = keys, *kb = k2;
   V *va = vals, *vb = v2;
-  for (int shift = 0; shift < bits; shift += 8) {
+  for (int shift = lo; shift < lo + bits; shift += 8) {
     hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, n, shift, hist,
                        ntiles);
     LN_LAUNCH(ctx);
@@ -252,7 +255,10 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
     std::swap(ka, kb);
     std::swap(va, vb);
   }
-  if (ka != keys) {
+  if (k_res) {  // the caller reads the result where the last pass left it
+    *k_res = ka;
+    *v_res = va;
+  } else if (ka != keys) {
     LN_HIP(ctx, hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
     LN_HIP(ctx, hipMemcpyAsync(vals, va, n * sizeof(V), hipMemcpyDeviceToDevice, ctx->stream));
   }
@@ -268,9 +274,9 @@ int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n) {
   return scan_impl<uint64_t>(ctx, in, out, n, 0);
 }
 int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp,
-                     float* v_tmp) {
-  return radix_sort_impl<float>(ctx, keys, vals, n, bits, k_tmp, v_tmp);
+                     float* v_tmp, int lo_bit, uint64_t** k_res, float** v_res) {
+  return radix_sort_impl<float>(ctx, keys, vals, n, bits, k_tmp, v_tmp, lo_bit, k_res, v_res);
 }
 int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits) {
-  return radix_sort_impl<uint32_t>(ctx, keys, vals, n, bits, nullptr, nullptr);
+  return radix_sort_impl<uint32_t>(ctx, keys, vals, n, bits, nullptr, nullptr, 0, nullptr, nullptr);
 }

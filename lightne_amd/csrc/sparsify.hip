@@ -15,8 +15,8 @@
 //                    the symmetric expansion (PathEmbed.h:278-311): kept keys emit
 //                    (row, col) sort keys and float values; per-block counts, a
 //                    scan, then writes in slot order (no contended atomics).
-//  radix sort + k_csr_finish : rows sorted by column, row_ptr by boundary
-//                    detection (no atomics, deterministic).
+//  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
+//                    row_ptr by boundary detection (no atomics, deterministic).
 #include "lightne_internal.h"
 
 #include <algorithm>
@@ -367,14 +367,19 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_
 }
 
 // row_ptr by boundary detection over the sorted keys; col extraction
+// vin / vout: the values when the sort left them elsewhere (else NULL)
 __global__ void k_csr_finish(const uint64_t* __restrict__ skey, int64_t nnz, int64_t n, int colbits,
-                             int64_t* __restrict__ row_ptr, int32_t* __restrict__ col) {
+                             int64_t* __restrict__ row_ptr, int32_t* __restrict__ col,
+                             const float* __restrict__ vin = nullptr, float* __restrict__ vout = nullptr) {
   const uint64_t cmask = (colbits >= 64) ? ~0ull : ((1ull << colbits) - 1);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nnz; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
     const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
     for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = i;
-    if (i < nnz) col[i] = (int32_t)(skey[i] & cmask);
+    if (i < nnz) {
+      col[i] = (int32_t)(skey[i] & cmask);
+      if (vin) vout[i] = vin[i];
+    }
   }
 }
 
@@ -654,9 +659,14 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     k_tmp = (uint64_t*)tab;
     v_tmp = (float*)((char*)tab + kbytes);
   }
-  LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp));
-  hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)skey,
-                     nnz, g->n, colbits, rp, col);
+  // rows sorted by column: a key's table slot depends on the order concurrent
+  // inserts claimed slots, so the column order is what makes the matrix (and
+  // every summation over a row) deterministic
+  uint64_t* kres = nullptr;
+  float* vres = nullptr;
+  LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
+  hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, nnz,
+                     g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val);
   LN_LAUNCH(ctx);
   LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
   LN_HIP(ctx, hipStreamSynchronize(s));
@@ -703,7 +713,7 @@ int csr_expand_upper(lightne_ctx* ctx, const lightne_csr* U, lightne_csr* F) {
   LN_HIP(ctx, hipStreamSynchronize(s));
   const int64_t nf = 2 * nnz - (int64_t)ndiag;  // the sentinels sort past position nf
   hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nf + 1, 256)), dim3(256), 0, s, (const uint64_t*)key, nf, n,
-                     colbits, rp, col);
+                     colbits, rp, col, nullptr, nullptr);
   LN_LAUNCH(ctx);
   LN_HIP(ctx, hipStreamSynchronize(s));
   F->n = n;

@@ -262,3 +262,16 @@ def _diag_count(csr):
     rp, col, _ = csr.to_numpy()
     rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
     return int(np.count_nonzero(rows == col))
+
+
+def test_netsmf_is_deterministic(gpu):
+    """Two runs with the same seeds give the same embedding bit for bit (the
+    table's slot order varies with the order of concurrent inserts; rows sorted
+    by column and fixed-order reductions keep every sum in one order)."""
+    hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
+    g = hg.to_device()
+    e1, S1, _ = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=4, ctx=gpu)
+    e2, S2, _ = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=4, ctx=gpu)
+    np.testing.assert_array_equal(S1, S2)
+    np.testing.assert_array_equal(e1.cpu().numpy(), e2.cpu().numpy())
