@@ -125,8 +125,10 @@ int cholqr2_t(lightne_ctx* ctx, T* Y, T* tmp, int64_t n, int64_t k, double* R_ou
       for (int j = 0; j < kk; ++j) R[i * k + j] = Rs[i * k + j] / d[j];
     tri_upper_inverse(R.data(), kk, Rinv.data());
     LN_CHECK(apply_small(ctx, Y, tmp, n, k, Rinv.data()));
-    matmul_host(R.data(), Rtot.data(), kk, kk, kk, Rnext.data());
-    Rtot.swap(Rnext);
+    if (R_out_host) {  // accumulated R only when the caller wants it
+      matmul_host(R.data(), Rtot.data(), kk, kk, kk, Rnext.data());
+      Rtot.swap(Rnext);
+    }
     ++passes;
     if (pass >= 1 && dev < 1e-1) break;  // input was already close to orthonormal
   }

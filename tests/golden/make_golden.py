@@ -12,6 +12,7 @@ read with scipy.io.loadmat, which executes nothing from the file).  Outputs:
     golden_<case>.npz         scalars and either the full key->value table or a
                               SHA-256 digest plus a strided subset
     manifest.json             the case list
+    blogcatalog_labels.npz    BlogCatalog's node labels
 """
 from __future__ import annotations
 
@@ -83,8 +84,19 @@ def digest(*arrays) -> str:
     return h.hexdigest()
 
 
+def write_blog_labels():
+    """BlogCatalog's node labels (variable ``group``, 10,312 x 39) as a scipy
+    sparse .npz, for the node-classification check (LightNE/predict.py)."""
+    import scipy.io
+    import scipy.sparse as sp
+
+    y = sp.csr_matrix(scipy.io.loadmat(BLOG)["group"]).astype(np.bool_)
+    sp.save_npz(os.path.join(HERE, "blogcatalog_labels.npz"), y, compressed=True)
+
+
 def main():
     assert os.path.exists(HARNESS), "build the harness first: make -C oracle ref"
+    write_blog_labels()
     graphs = {
         "toy": toy_graph(),
         "cl200": api.chung_lu(200, 2.1, 6.0, 40.0, 3),

@@ -13,6 +13,10 @@ def manifest():
         return json.load(f)["cases"]
 
 
+def golden_path(name):
+    return os.path.join(GOLDEN, name)
+
+
 def golden_graph(name):
     d = np.load(os.path.join(GOLDEN, f"graph_{name}.npz"))
     return d["offsets"].astype(np.int64), d["edges"].astype(np.uint32)
