@@ -389,9 +389,10 @@ int bits_for(uint64_t n) {
   return b;
 }
 
+// smallest power of two >= x, saturating at 2^63
 uint64_t next_pow2(uint64_t x) {
   uint64_t c = 1;
-  while (c < x) c <<= 1;
+  while (c < x && c < (1ull << 63)) c <<= 1;
   return c;
 }
 
@@ -424,13 +425,17 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   // capacity: load factor <= 0.7 even if every sample is a distinct key (the
   // reference sizes by -tablesz / -mem_ratio, 2^ceil(log2(2 table_size)), and
   // exits when that is too small, par_table.h:104-107)
-  uint64_t cap = next_pow2(std::max<uint64_t>((uint64_t)(bound / 0.7) + 1, 1024));
+  const double want = bound / 0.7 + 1.0;
+  uint64_t cap = want > 1.0e12 ? (1ull << 41) : next_pow2(std::max<uint64_t>((uint64_t)want, 1024));
   if (p->max_table_bytes) {
     const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
     uint64_t lim = 1024;
-    while (lim * 2 * slot_bytes <= p->max_table_bytes) lim *= 2;
+    while (lim < (1ull << 40) && lim * 2 * slot_bytes <= p->max_table_bytes) lim *= 2;
     if (cap > lim) cap = lim;
   }
+  if (cap > (1ull << 40))
+    return ctx_fail(ctx, LIGHTNE_ERR_OOM, "sampling budget of %.3g samples needs a table of more than 2^40 slots",
+                    bound);
   st->table_capacity = cap;
 
   Slot* tab = nullptr;
@@ -736,7 +741,10 @@ int lightne_sample_scalars(int64_t n, int64_t m, const lightne_sparsify_params* 
   uint64_t ts = (uint64_t)((double)tn * logn * (double)p->mem_ratio);
   if (p->tablesz > ts) ts = p->tablesz;
   const double M = (double)tn * logn * (double)p->sample_ratio;
-  const double inv_lambda = M / (double)(uint64_t)m;
+  // a graph without edges draws nothing (the reference divides by m = 0)
+  const double inv_lambda = m > 0 ? M / (double)(uint64_t)m : 0.0;
+  if (!(inv_lambda < 1.8e19))
+    return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sample_scalars: %g samples per edge is out of range", inv_lambda);
   const uint64_t e = (uint64_t)inv_lambda;
   out->logn = logn;
   out->M = M;

@@ -132,3 +132,12 @@ def test_cli_usage_and_errors(tmp_path):
     assert out.returncode == 2 and "compressed" in out.stderr
     out = subprocess.run([CLI, "-ne_method", "nope", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
     assert out.returncode == 2
+
+
+def test_sample_scalars_without_edges_and_out_of_range():
+    """m = 0 draws nothing (the reference divides by zero); an absurd budget is an
+    error, not a runaway table size."""
+    st = L.sample_scalars(300, 0, walk_len=10, sample_ratio=1.0)
+    assert st["one_over_lambda"] == 0.0 and st["exp_rv_rounded"] == 0 and st["fractional_bits"] == 0.0
+    with pytest.raises(L.LightNEError):
+        L.sample_scalars(10, 2, walk_len=10, sample_ratio=1e30)
