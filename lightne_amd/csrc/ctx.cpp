@@ -91,13 +91,6 @@ void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes) {
   return q;
 }
 
-void ctx_release(lightne_ctx* ctx, const char* name) {
-  auto it = ctx->bufs.find(name);
-  if (it == ctx->bufs.end()) return;
-  if (it->second.p) (void)hipFree(it->second.p);
-  ctx->bufs.erase(it);
-}
-
 int WsCarve::commit(lightne_ctx* ctx, size_t reserve) {
   const size_t need = std::max(end(), reserve);
   char* w = (char*)ctx_buf(ctx, "ws", need);

@@ -53,7 +53,6 @@ struct lightne_csr {
 int ctx_fail(lightne_ctx* ctx, int code, const char* fmt, ...);
 int global_fail(int code, const char* fmt, ...);
 void* ctx_buf(lightne_ctx* ctx, const char* name, size_t bytes);  // nullptr on OOM
-void ctx_release(lightne_ctx* ctx, const char* name);
 // Device memory kept across calls is reclaimed only under pressure: when an
 // allocation fails inside a scoped call, the cached buffers that call has not
 // touched are freed and the allocation is tried once more.
@@ -156,7 +155,6 @@ double ev_seconds(hipEvent_t a, hipEvent_t b);
 // exclusive prefix sum of u32 or u64 counts into u64 offsets; out has n+1
 // entries (out[n] = total).  Deterministic.
 int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n);
-int scan_u64(lightne_ctx* ctx, const uint64_t* in, uint64_t* out, int64_t n);
 // stable LSD radix sort of (key, val) pairs on the low `bits` bits of key.
 // Sorts on key bits [lo_bit, lo_bit + bits).  The ping-pong buffers (n keys,
 // n vals) are k_tmp / v_tmp when given, else context buffers.  keys/vals hold
@@ -206,9 +204,6 @@ int f64_to_f32(lightne_ctx* ctx, const double* in, float* out, int64_t count);
 bool chol_upper(const double* G, int k, double* R);
 // inverse of an upper-triangular matrix (row-major)
 void tri_upper_inverse(const double* R, int k, double* Rinv);
-// one-sided Jacobi SVD of A (m x n row-major, m >= n): A = U diag(S) V^T,
-// singular values descending; U m x n, V n x n (either may be NULL).
-void jacobi_svd(const double* A, int m, int n, double* U, double* S, double* V);
 // symmetric eigendecomposition, eigenvalues descending, Z row-major n x n
 bool sym_eig(const double* A, int n, double* w, double* Z);
 // left singular vectors / values of a square matrix via eig(A A^T)

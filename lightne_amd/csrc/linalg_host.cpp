@@ -6,10 +6,8 @@
 //     SVDs that replace LAPACKE_?gesvd on C (mklredsvd.cc:352-377) and on the
 //     R factor of the tall panel of get_embedding_dense
 //     (SpectralPropagation.h:16-43).
-//   * jacobi_svd: one-sided (Hestenes) Jacobi SVD, kept as an independent
-//     cross-check of sym_eig.
-// Written from the textbook algorithms (Golub & Van Loan, 4th ed., 4.2.5,
-// 8.3 and 8.6.3); singular vectors are determined up to sign.
+// Written from the textbook algorithms (Golub & Van Loan, 4th ed., 4.2.5 and
+// 8.3); singular vectors are determined up to sign.
 #include "lightne_internal.h"
 
 #include <algorithm>
@@ -56,69 +54,6 @@ void matmul_host(const double* A, const double* B, int m, int k, int n, double* 
       for (int j = 0; j < n; ++j) tmp[(size_t)i * n + j] += a * B[(size_t)p * n + j];
     }
   std::copy(tmp.begin(), tmp.end(), C);
-}
-
-void jacobi_svd(const double* A, int m, int n, double* U, double* S, double* V) {
-  // work on columns: W = A (column-major copy for cache-friendly rotations)
-  std::vector<double> W((size_t)m * n), Vw((size_t)n * n, 0.0);
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j < n; ++j) W[(size_t)j * m + i] = A[(size_t)i * n + j];
-  for (int j = 0; j < n; ++j) Vw[(size_t)j * n + j] = 1.0;
-  const double eps = 1e-15;
-  for (int sweep = 0; sweep < 80; ++sweep) {
-    double off = 0.0;
-    for (int p = 0; p < n - 1; ++p) {
-      for (int q = p + 1; q < n; ++q) {
-        double* wp = &W[(size_t)p * m];
-        double* wq = &W[(size_t)q * m];
-        double alpha = 0.0, beta = 0.0, gamma = 0.0;
-        for (int i = 0; i < m; ++i) {
-          alpha += wp[i] * wp[i];
-          beta += wq[i] * wq[i];
-          gamma += wp[i] * wq[i];
-        }
-        if (gamma == 0.0) continue;
-        const double rel = std::fabs(gamma) / std::sqrt(alpha * beta);
-        if (!(rel > eps)) continue;
-        off = std::max(off, rel);
-        const double zeta = (beta - alpha) / (2.0 * gamma);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
-        for (int i = 0; i < m; ++i) {
-          const double x = wp[i], y = wq[i];
-          wp[i] = c * x - s * y;
-          wq[i] = s * x + c * y;
-        }
-        double* vp = &Vw[(size_t)p * n];
-        double* vq = &Vw[(size_t)q * n];
-        for (int i = 0; i < n; ++i) {
-          const double x = vp[i], y = vq[i];
-          vp[i] = c * x - s * y;
-          vq[i] = s * x + c * y;
-        }
-      }
-    }
-    if (off <= eps) break;
-  }
-  std::vector<double> sv(n);
-  for (int j = 0; j < n; ++j) {
-    double s = 0.0;
-    for (int i = 0; i < m; ++i) s += W[(size_t)j * m + i] * W[(size_t)j * m + i];
-    sv[j] = std::sqrt(s);
-  }
-  std::vector<int> ord(n);
-  std::iota(ord.begin(), ord.end(), 0);
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return sv[a] > sv[b]; });
-  for (int jj = 0; jj < n; ++jj) {
-    const int j = ord[jj];
-    if (S) S[jj] = sv[j];
-    if (U) {
-      const double inv = sv[j] > 0.0 ? 1.0 / sv[j] : 0.0;
-      for (int i = 0; i < m; ++i) U[(size_t)i * n + jj] = W[(size_t)j * m + i] * inv;
-    }
-    if (V)
-      for (int i = 0; i < n; ++i) V[(size_t)i * n + jj] = Vw[(size_t)j * n + i];
-  }
 }
 
 // ---------------------------------------------------------------------------

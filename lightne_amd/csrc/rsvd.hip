@@ -1,18 +1,20 @@
 // Stage 2: randomized SVD of the sparsifier (LightNE/mklredsvd/mklredsvd.cc:93-403)
 // and the U sqrt(Sigma) embedding (LightNE/PathEmbed.h:371-408), plus the CSR
-// handle API and the NetSMF driver (PathEmbed.h:494-563).
+// handle API, the NetSMF driver (PathEmbed.h:494-563) and ne_zhang_et_al's
+// stage 2 (PathEmbed.h:445-460).
 //
-// Schedule (q = power_iters; q = 1 is the reference schedule):
-//   Omega ~ N(0,1) n x k            (mklredsvd.cc:229-231, own counter-based RNG)
+// Reference schedule (q = power_iters; q = 1 is the reference's):
+//   Omega ~ N(0,1) n x k            (mklredsvd.cc:229-231, own counter-based RNG;
+//                                    or the very sparse projection, :148-226)
 //   Y = A Omega                      (:233-250)
 //   Q = orth(Y)                      (:262)   CholeskyQR2, FP64 Gram + Cholesky
 //   repeat q-1 times: Q = orth(A Q)
 //   B = A Q                          (:268-287)
-//   P ~ N(0,1) k x k; Z = B P        (:289-317)
-//   Z = orth(Z)                      (:321)
-//   C = Z^T B                        (:327-350)   FP64 Gram kernel
-//   C = U_C S V^T                    (:352-377)   host FP64 eig(C C^T)
-//   matU = Z U_C                     (:379-400)
+//   P ~ N(0,1) k x k; Z = orth(B P); C = Z^T B; C = U_C S V^T; matU = Z U_C
+//                                    (:289-400)
+// The FP32 and BF16 forms replace the last line by the thin SVD of B from its
+// FP64 Gram, B^T B = V S^2 V^T, U = B V S^-1 (range(Z) = range(B) for any
+// invertible P, so U and S are the same); the FP64 form keeps it as written.
 // In the BF16 form the sketch Omega, whose values are BF16-representable, is
 // gathered as BF16 with FP32 accumulation: the product is the FP32 form's,
 // bit for bit, at half the gathered bytes.  The power iterations gather the
