@@ -202,33 +202,31 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   return LIGHTNE_OK;
 }
 
+// get_embedding_dense (SpectralPropagation.h:16-43): thin SVD of mat (n x rank)
+// from its FP64 Gram, mat^T mat = V S^2 V^T (U = mat V S^-1), then
+// emb = rownorm(U[:, :dim] sqrt(S)) = rownorm(mat V[:, :dim] S^-1/2) in one
+// panel product with the row-normalisation epilogue.  mat is not modified.
 int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
-                    float* S_host, float* Q = nullptr) {
+                    float* S_host) {
   if (rank % 16 || dim % 16 || dim > rank || rank > 256 || n < rank)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embed_dense: rank, dim must be multiples of 16, dim <= rank <= 256, n >= rank");
-  if (!Q) {
-    WsCarve wc;
-    wc.add((void**)&Q, (size_t)n * rank * sizeof(float));
-    LN_CHECK(wc.commit(ctx));
+  std::vector<double> G(rank * rank), V(rank * rank), w(rank), S(rank);
+  LN_CHECK(gram_f64(ctx, mat, mat, n, rank, rank, 1, G.data()));
+  if (!sym_eig(G.data(), (int)rank, w.data(), V.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "embed_dense: eigensolver did not converge");
+  for (int64_t j = 0; j < rank; ++j) S[j] = std::sqrt(std::max(w[j], 0.0));
+  // columns whose singular value is numerically zero for an FP32 panel give
+  // zero columns (their U sqrt(S) is zero to that precision)
+  std::vector<double> W(rank * dim, 0.0);
+  for (int64_t j = 0; j < dim; ++j) {
+    if (!(S[j] > 1e-6 * S[0])) continue;
+    const double f = 1.0 / std::sqrt(S[j]);
+    for (int64_t i = 0; i < rank; ++i) W[i * dim + j] = V[i * rank + j] * f;
   }
-  if (Q != mat)
-    LN_HIP(ctx, hipMemcpyAsync(Q, mat, (size_t)n * rank * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-  std::vector<double> R(rank * rank), Ur(rank * rank), Sr(rank);
-  int passes = 0;
-  LN_CHECK(cholqr2(ctx, Q, n, rank, R.data(), &passes));
-  if (!svd_left_via_eig(R.data(), (int)rank, Ur.data(), Sr.data()))
-    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "embed_dense: small SVD did not converge");
-  // emb = rownorm(Q U_R[:, :dim] diag(sqrt(S)))  (compute_u_sigma_root, normalize = true)
-  std::vector<double> W(rank * dim);
-  for (int64_t i = 0; i < rank; ++i)
-    for (int64_t j = 0; j < dim; ++j) {
-      const float sq = std::sqrt((float)Sr[j]);
-      W[i * dim + j] = (double)((float)Ur[i * rank + j] * sq);
-    }
-  LN_CHECK(panel_times_small(ctx, Q, n, rank, rank, W.data(), dim, emb_out, 1, nullptr));
+  LN_CHECK(panel_times_small(ctx, mat, n, rank, rank, W.data(), dim, emb_out, 1, nullptr));
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (S_host)
-    for (int64_t j = 0; j < rank; ++j) S_host[j] = (float)Sr[j];
+    for (int64_t j = 0; j < rank; ++j) S_host[j] = (float)S[j];
   return LIGHTNE_OK;
 }
 
@@ -268,7 +266,7 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
   // z after the recurrence's own workspace (y is written over its
-  // accumulator); the tall SVD orthonormalises z in place
+  // accumulator)
   float* z = nullptr;
   WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
   wc.add((void**)&z, (size_t)g->n * dim * sizeof(float));
@@ -276,7 +274,7 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
   lightne_prop_stats st{};
   LN_CHECK(propagate_forward(ctx, g, a, dim, p, nullptr, z, &st, gp.P));
   auto t1 = std::chrono::steady_clock::now();
-  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr, z));
+  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr));
   auto t2 = std::chrono::steady_clock::now();
   st.t_svd = std::chrono::duration<double>(t2 - t1).count();
   st.t_total = std::chrono::duration<double>(t2 - t0).count();
