@@ -19,6 +19,9 @@
  *       path_embed::NetSMF                        (LightNE/PathEmbed.h:494-563)
  *   lightne_graph_read_adj
  *       gbbs_io::read_unweighted_symmetric_graph  (ligra/graph_io.h:185-252)
+ *   lightne_graph_decode_compressed
+ *       gbbs_io::read_compressed_symmetric_graph  (ligra/graph_io.h:344-377)
+ *       + bytepd_amortized::decode                (ligra/encodings/byte_pd_amortized.h:330-386)
  *
  * Conventions (differences from the reference are deliberate):
  *   - every call returns an int status (LIGHTNE_OK = 0); nothing calls exit()
@@ -86,7 +89,20 @@ typedef struct {
   int64_t m;
   const int64_t* offsets; /* n+1 */
   const uint32_t* edges;  /* m   */
+  int32_t ith_rule;       /* LIGHTNE_ITH_*: how a walk's draw i maps to a neighbour */
+  int32_t reserved;       /* 0 */
 } lightne_graph;
+
+/* The walk (PathEmbed.h:37-49) takes get_ith_out_neighbor(u, i), whose result
+ * depends on how the reference stored the graph:
+ *   PLAIN  - neighbour i in input order (ligra/vertex.h:296-300): text and
+ *            binary CSR inputs;
+ *   BYTEPD - graphs read from the compressed format (-c): the byte-pd-amortized
+ *            lookup (ligra/encodings/byte_pd_amortized.h:653-688) returns
+ *            neighbour i - 1 unless i is a multiple of 64.
+ * Only the sampler reads it; every other stage sees the same CSR. */
+#define LIGHTNE_ITH_PLAIN 0
+#define LIGHTNE_ITH_BYTEPD 1
 
 /* ---- stage 1: sparsifier (PathEmbed.h:81-369) ------------------------------ */
 typedef struct {
@@ -253,12 +269,31 @@ int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_
  * Y_in = Q R; passes_out (may be NULL) receives the number of passes */
 int lightne_dense_cholqr(lightne_ctx* ctx, float* Y, int64_t n, int64_t k, double* R_host, int32_t* passes_out);
 
+/* ---- compressed graph input (-c) -------------------------------------------
+ * Decodes a file in Ligra's compressed format (byte-pd-amortized encoding,
+ * ligra/encodings/byte_pd_amortized.h; layout of
+ * gbbs_io::read_compressed_symmetric_graph, ligra/graph_io.h:344-377) into the
+ * symmetric CSR.  `image` is the whole file copied to device memory (8-byte
+ * aligned), as the reference reads or maps it; n and m are the file's first
+ * two int64 words (checked).  offsets_out (n+1) and edges_out (m) are device
+ * buffers; neighbours come out in decode order.  A malformed file returns
+ * LIGHTNE_ERR_BAD_GRAPH with the reason; nothing is read outside the image.
+ * Sample from the result with ith_rule = LIGHTNE_ITH_BYTEPD to reproduce the
+ * reference's walks on the compressed input. */
+int lightne_graph_decode_compressed(lightne_ctx* ctx, const void* image, uint64_t image_bytes, int64_t n, int64_t m,
+                                    int64_t* offsets_out, uint32_t* edges_out);
+
 /* ---- host graph utilities (no device work) ---------------------------------- */
 /* Ligra AdjacencyGraph text reader/writer (ligra/graph_io.h:185-231).
  * *offsets_host has n+1 entries; free both with lightne_host_free. */
 int lightne_graph_read_adj(const char* path, int64_t* n, int64_t* m, int64_t** offsets_host,
                            uint32_t** edges_host);
 int lightne_graph_write_adj(const char* path, int64_t n, const int64_t* offsets_host, const uint32_t* edges_host);
+/* Binary CSR cache (no reference counterpart: a load path that skips the
+ * AdjacencyGraph text parse of ligra/graph_io.h:185-231).  Layout: 8 bytes
+ * "LNECSR01", int64 n, int64 m, int64 offsets[n+1], uint32 edges[m]. */
+int lightne_graph_read_bin(const char* path, int64_t* n, int64_t* m, int64_t** offsets_host, uint32_t** edges_host);
+int lightne_graph_write_bin(const char* path, int64_t n, const int64_t* offsets_host, const uint32_t* edges_host);
 /* Seeded synthetic graphs: undirected, no self loops, no duplicates, each
  * neighbour list sorted.  Chung-Lu with power-law expected degrees
  * w_i ~ (i + i0)^(-1/(beta-1)) scaled to mean `mean_deg` and capped at

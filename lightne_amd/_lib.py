@@ -39,8 +39,13 @@ class LightNEError(RuntimeError):
         self.code = code
 
 
+ITH_PLAIN = 0  # walks take neighbour i in input order
+ITH_BYTEPD = 1  # walks take the compressed-format lookup (neighbour i - 1 unless 64 | i)
+
+
 class Graph(C.Structure):
-    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("offsets", C.c_void_p), ("edges", C.c_void_p)]
+    _fields_ = [("n", C.c_int64), ("m", C.c_int64), ("offsets", C.c_void_p), ("edges", C.c_void_p),
+                ("ith_rule", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SparsifyParams(C.Structure):
@@ -140,6 +145,9 @@ _SIGS = {
     "lightne_dense_cholqr": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int32)]),
     "lightne_graph_read_adj": (C.c_int, [C.c_char_p, _PI64, _PI64, _PP, _PP]),
     "lightne_graph_write_adj": (C.c_int, [C.c_char_p, _I64, _P, _P]),
+    "lightne_graph_read_bin": (C.c_int, [C.c_char_p, _PI64, _PI64, _PP, _PP]),
+    "lightne_graph_write_bin": (C.c_int, [C.c_char_p, _I64, _P, _P]),
+    "lightne_graph_decode_compressed": (C.c_int, [_P, _P, C.c_uint64, _I64, _I64, _P, _P]),
     "lightne_gen_chung_lu": (C.c_int, [_I64, C.c_double, C.c_double, C.c_double, C.c_uint64, _PI64, _PP, _PP]),
     "lightne_gen_rmat": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                    _PI64, _PI64, _PP, _PP]),

@@ -66,12 +66,12 @@ class HostGraph:
     def degrees(self) -> np.ndarray:
         return np.diff(self.offsets)
 
-    def to_device(self, device: str = "cuda") -> "DeviceGraph":
+    def to_device(self, device: str = "cuda", ith_rule: int = _lib.ITH_PLAIN) -> "DeviceGraph":
         torch = _torch()
         off = torch.from_numpy(self.offsets.astype(np.int64)).to(device)
         # torch has no uint32 kernels we need; int32 storage carries the same bits
         edg = torch.from_numpy(self.edges.astype(np.uint32).view(np.int32)).to(device)
-        return DeviceGraph(off, edg, self.n, self.m)
+        return DeviceGraph(off, edg, self.n, self.m, ith_rule)
 
 
 @dataclasses.dataclass
@@ -80,9 +80,15 @@ class DeviceGraph:
     edges: "object"  # torch.int32 [m], bits of uint32
     n: int
     m: int
+    # how walks map draw i to a neighbour: ITH_PLAIN (input order), or
+    # ITH_BYTEPD for graphs read from the compressed format, whose lookup in
+    # the reference returns neighbour i - 1 unless i is a multiple of 64
+    # (ligra/encodings/byte_pd_amortized.h:653-688)
+    ith_rule: int = _lib.ITH_PLAIN
 
     def desc(self) -> Graph:
-        return Graph(self.n, self.m, self.offsets.data_ptr(), self.edges.data_ptr() if self.m else None)
+        return Graph(self.n, self.m, self.offsets.data_ptr(), self.edges.data_ptr() if self.m else None,
+                     self.ith_rule, 0)
 
 
 def _take_host_graph(n: int, m: int, off_p: C.c_void_p, edg_p: C.c_void_p) -> HostGraph:
@@ -111,6 +117,59 @@ def write_adjacency_graph(path: str, g: HostGraph) -> None:
     off = np.ascontiguousarray(g.offsets, dtype=np.int64)
     edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
     check(lib.lightne_graph_write_adj(path.encode(), g.n, off.ctypes.data, edg.ctypes.data if g.m else None))
+
+
+def read_graph_bin(path: str) -> HostGraph:
+    """Read the binary CSR cache ("LNECSR01", int64 n, m, offsets, uint32 edges)."""
+    lib = load()
+    n, m = C.c_int64(), C.c_int64()
+    op, ep = C.c_void_p(), C.c_void_p()
+    check(lib.lightne_graph_read_bin(path.encode(), C.byref(n), C.byref(m), C.byref(op), C.byref(ep)))
+    return _take_host_graph(n.value, m.value, op, ep)
+
+
+def write_graph_bin(path: str, g: HostGraph) -> None:
+    lib = load()
+    off = np.ascontiguousarray(g.offsets, dtype=np.int64)
+    edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
+    check(lib.lightne_graph_write_bin(path.encode(), g.n, off.ctypes.data, edg.ctypes.data if g.m else None))
+
+
+def write_compressed_graph(path: str, g: HostGraph) -> None:
+    """Write ``g`` in Ligra's compressed format (the ``-c`` input; encoder in
+    lightne_amd.bytepd)."""
+    from . import bytepd
+
+    bytepd.write(path, g.offsets, g.edges)
+
+
+def decode_compressed_graph(image, ith_rule: int = _lib.ITH_BYTEPD, ctx: Optional["Context"] = None) -> DeviceGraph:
+    """Decode a compressed-format file image (uint8 torch tensor on the
+    device, the whole file) into a DeviceGraph (gbbs_io::
+    read_compressed_symmetric_graph, ligra/graph_io.h:344-377, with the block
+    decoder of ligra/encodings/byte_pd_amortized.h).  ``ith_rule`` defaults to
+    the reference's walk lookup on compressed graphs."""
+    torch = _torch()
+    ctx = ctx or context(image.device.index or 0)
+    nbytes = int(image.numel())
+    if nbytes < 24:
+        raise LightNEError(_lib.ERR_BAD_GRAPH, "compressed graph: file shorter than its header")
+    n, m = (int(x) for x in image[:16].cpu().numpy().view(np.int64))
+    if n <= 0 or m < 0:
+        raise LightNEError(_lib.ERR_BAD_GRAPH, f"compressed graph: bad header n = {n}, m = {m}")
+    off = torch.empty(n + 1, dtype=torch.int64, device=image.device)
+    edg = torch.empty(max(m, 1), dtype=torch.int32, device=image.device)
+    ctx.check(ctx.lib.lightne_graph_decode_compressed(ctx.ptr, image.data_ptr(), nbytes, n, m, off.data_ptr(),
+                                                      edg.data_ptr()))
+    return DeviceGraph(off, edg[:m], n, m, ith_rule)
+
+
+def read_compressed_graph(path: str, device: int = 0, ith_rule: int = _lib.ITH_BYTEPD,
+                          ctx: Optional["Context"] = None) -> DeviceGraph:
+    """Read a compressed-format file (``lightne -c``) and decode it on the device."""
+    torch = _torch()
+    img = torch.from_numpy(np.fromfile(path, dtype=np.uint8)).to(f"cuda:{device}")
+    return decode_compressed_graph(img, ith_rule, ctx)
 
 
 def chung_lu(n: int, beta: float, mean_deg: float, max_deg: float, seed: int) -> HostGraph:

@@ -1,5 +1,6 @@
 // Host graph utilities: Ligra AdjacencyGraph text IO (ligra/graph_io.h:185-231,
-// util/x2adj.py:81-119) and seeded synthetic generators (Chung-Lu, R-MAT).
+// util/x2adj.py:81-119), a binary CSR cache, and seeded synthetic generators
+// (Chung-Lu, R-MAT).
 // Generators are counter-based (every draw is a hash of (seed, edge, draw)),
 // so the graph does not depend on the thread count.
 #include "lightne_internal.h"
@@ -197,6 +198,61 @@ int lightne_graph_write_adj(const char* path, int64_t n, const int64_t* off, con
   const bool bad = std::ferror(f) != 0;
   std::fclose(f);
   return bad ? global_fail(LIGHTNE_ERR_IO, "write_adj: write error on %s", path) : LIGHTNE_OK;
+}
+
+// Binary CSR: "LNECSR01", int64 n, int64 m, int64 offsets[n+1], uint32 edges[m]
+static const char kBinMagic[8] = {'L', 'N', 'E', 'C', 'S', 'R', '0', '1'};
+
+int lightne_graph_write_bin(const char* path, int64_t n, const int64_t* off, const uint32_t* adj) {
+  if (!path || n <= 0 || !off || (off[n] > 0 && !adj)) return global_fail(LIGHTNE_ERR_BAD_ARG, "write_bin: bad argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return global_fail(LIGHTNE_ERR_IO, "write_bin: cannot open %s", path);
+  const int64_t hdr[2] = {n, off[n]};
+  bool ok = std::fwrite(kBinMagic, 1, 8, f) == 8 && std::fwrite(hdr, 8, 2, f) == 2 &&
+            std::fwrite(off, 8, (size_t)(n + 1), f) == (size_t)(n + 1) &&
+            std::fwrite(adj, 4, (size_t)off[n], f) == (size_t)off[n];
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? LIGHTNE_OK : global_fail(LIGHTNE_ERR_IO, "write_bin: write error on %s", path);
+}
+
+int lightne_graph_read_bin(const char* path, int64_t* n_out, int64_t* m_out, int64_t** offsets_host,
+                           uint32_t** edges_host) {
+  if (!path || !n_out || !m_out || !offsets_host || !edges_host)
+    return global_fail(LIGHTNE_ERR_BAD_ARG, "read_bin: NULL argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return global_fail(LIGHTNE_ERR_IO, "read_bin: cannot open %s", path);
+  char magic[8];
+  int64_t hdr[2];
+  if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kBinMagic, 8) != 0 || std::fread(hdr, 8, 2, f) != 2) {
+    std::fclose(f);
+    return global_fail(LIGHTNE_ERR_IO, "read_bin: %s is not a binary CSR file", path);
+  }
+  const int64_t n = hdr[0], m = hdr[1];
+  struct stat sb;
+  const bool sized = fstat(fileno(f), &sb) == 0 && n > 0 && n < ((int64_t)1 << 32) && m >= 0 &&
+                     (uint64_t)sb.st_size == 24 + 8 * (uint64_t)(n + 1) + 4 * (uint64_t)m;
+  if (!sized) {
+    std::fclose(f);
+    return global_fail(LIGHTNE_ERR_IO, "read_bin: header of %s does not match its size", path);
+  }
+  auto* off = (int64_t*)std::malloc((size_t)(n + 1) * sizeof(int64_t));
+  auto* adj = (uint32_t*)std::malloc((size_t)std::max<int64_t>(m, 1) * sizeof(uint32_t));
+  bool ok = off && adj && std::fread(off, 8, (size_t)(n + 1), f) == (size_t)(n + 1) &&
+            std::fread(adj, 4, (size_t)m, f) == (size_t)m;
+  std::fclose(f);
+  ok = ok && off[0] == 0 && off[n] == m;
+  for (int64_t i = 0; i < n && ok; ++i) ok = off[i] <= off[i + 1];
+  for (int64_t j = 0; j < m && ok; ++j) ok = adj[j] < (uint64_t)n;
+  if (!ok) {
+    std::free(off);
+    std::free(adj);
+    return global_fail(LIGHTNE_ERR_IO, "read_bin: truncated or inconsistent file %s", path);
+  }
+  *n_out = n;
+  *m_out = m;
+  *offsets_host = off;
+  *edges_host = adj;
+  return LIGHTNE_OK;
 }
 
 int lightne_gen_chung_lu(int64_t n, double beta, double mean_deg, double max_deg, uint64_t seed, int64_t* m_out,

@@ -3,7 +3,11 @@
 // generate_symmetric_main (ligra/ligra.h:494-505,642-667), running the three
 // stages through the liblightne_amd C-ABI on one GPU.
 //
-//   lightne [flags] -s <graph.adj>
+//   lightne [flags] -s <graph.adj>       AdjacencyGraph text, or a binary CSR
+//                                         file (recognised by its magic)
+//   lightne [flags] -s -c <graph.cmp>    Ligra compressed format, decoded on
+//                                         the device (-m is accepted; the file
+//                                         is read whole either way)
 //
 // Flag parsing follows ligra/parse_command_line.h:32-113: exact string match,
 // the value is the next argv entry, the graph file is the LAST argument.
@@ -24,6 +28,9 @@
 //   -ne_in FILE    read the stage-2 embedding (raw FP32 n x dim) instead of
 //                  computing it, so stage 3 can run alone
 //   -device D      HIP device ordinal
+//   -ith_rule plain|bytepd   neighbour lookup of the walks; bytepd (the
+//                  default with -c) reproduces the reference's lookup on
+//                  compressed graphs
 //   -gen SPEC      synthetic graph instead of a file:
 //                  chung_lu:n,beta,mean,max,seed | rmat:scale,ef,a,b,c,seed
 #include <hip/hip_runtime.h>
@@ -142,10 +149,7 @@ int main(int argc, char** argv) {
     printf("# The application expects the input graph to be symmetric (-s flag).\n");
     printf("# Please run on a symmetric input.\n");
   }
-  if (P.has("-c")) {
-    fprintf(stderr, "lightne: compressed graphs (-c) are not supported; convert to AdjacencyGraph text\n");
-    return 2;
-  }
+  const bool compressed = P.has("-c");
   const size_t walks_per_edge = (size_t)P.lng("-walksperedge", 100);
   const size_t walk_len = (size_t)P.lng("-walklen", 10);
   const size_t table_size = (size_t)P.lng("-tablesz", 0);
@@ -191,14 +195,52 @@ int main(int argc, char** argv) {
     return 2;
   }
 
-  // graph load (excluded from the running time, as in the reference)
+  // -c graphs keep the reference's i-th-neighbour lookup for compressed input
+  // unless -ith_rule plain is given
+  const std::string ith_rule = P.str("-ith_rule", compressed ? "bytepd" : "plain");
+  if (ith_rule != "plain" && ith_rule != "bytepd") {
+    fprintf(stderr, "lightne: -ith_rule must be plain or bytepd\n");
+    return 2;
+  }
+
+  // graph load (excluded from the running time, as in the reference): host
+  // reads first, so a missing file is reported before any device work
   int64_t n = 0, m = 0;
   int64_t* off_h = nullptr;
   uint32_t* adj_h = nullptr;
+  std::vector<char> img;  // -c: the whole file, decoded on the device
   if (!gen.empty()) {
     if (!gen_graph(gen, &n, &m, &off_h, &adj_h)) die("graph generation (-gen)");
+  } else if (compressed) {
+    FILE* f = fopen(iFile, "rb");
+    if (!f) {
+      fprintf(stderr, "lightne: reading the graph: cannot open %s\n", iFile);
+      return 1;
+    }
+    fseek(f, 0, SEEK_END);
+    const long len = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    img.resize(len > 0 ? (size_t)len : 0);
+    const bool ok = len >= 24 && fread(img.data(), 1, img.size(), f) == img.size();
+    fclose(f);
+    if (!ok) {
+      fprintf(stderr, "lightne: reading the graph: %s is not a compressed graph file\n", iFile);
+      return 1;
+    }
+    memcpy(&n, img.data(), 8);
+    memcpy(&m, img.data() + 8, 8);
+    if (n <= 0 || m < 0) {
+      fprintf(stderr, "lightne: reading the graph: bad header in %s\n", iFile);
+      return 1;
+    }
   } else {
-    if (lightne_graph_read_adj(iFile, &n, &m, &off_h, &adj_h) != LIGHTNE_OK) die("reading the graph");
+    char magic[8] = {0};
+    FILE* f = fopen(iFile, "rb");
+    const bool bin = f && fread(magic, 1, 8, f) == 8 && memcmp(magic, "LNECSR01", 8) == 0;
+    if (f) fclose(f);
+    const int rc = bin ? lightne_graph_read_bin(iFile, &n, &m, &off_h, &adj_h)
+                       : lightne_graph_read_adj(iFile, &n, &m, &off_h, &adj_h);
+    if (rc != LIGHTNE_OK) die("reading the graph");
   }
   lightne_ctx* ctx = nullptr;
   if (lightne_ctx_create(device, nullptr, &ctx) != LIGHTNE_OK) die("lightne_ctx_create");
@@ -206,11 +248,21 @@ int main(int argc, char** argv) {
   uint32_t* adj_d = nullptr;
   HK(hipMalloc(&off_d, (n + 1) * sizeof(int64_t)));
   HK(hipMalloc(&adj_d, std::max<int64_t>(m, 1) * sizeof(uint32_t)));
-  HK(hipMemcpy(off_d, off_h, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-  if (m) HK(hipMemcpy(adj_d, adj_h, m * sizeof(uint32_t), hipMemcpyHostToDevice));
-  lightne_host_free(off_h);
-  lightne_host_free(adj_h);
-  lightne_graph g{n, m, off_d, adj_d};
+  if (compressed && gen.empty()) {
+    void* img_d = nullptr;
+    HK(hipMalloc(&img_d, img.size()));
+    HK(hipMemcpy(img_d, img.data(), img.size(), hipMemcpyHostToDevice));
+    if (lightne_graph_decode_compressed(ctx, img_d, img.size(), n, m, off_d, adj_d) != LIGHTNE_OK)
+      die("decoding the compressed graph", ctx);
+    HK(hipFree(img_d));
+    std::vector<char>().swap(img);
+  } else {
+    HK(hipMemcpy(off_d, off_h, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (m) HK(hipMemcpy(adj_d, adj_h, m * sizeof(uint32_t), hipMemcpyHostToDevice));
+    lightne_host_free(off_h);
+    lightne_host_free(adj_h);
+  }
+  lightne_graph g{n, m, off_d, adj_d, ith_rule == "bytepd" ? LIGHTNE_ITH_BYTEPD : LIGHTNE_ITH_PLAIN, 0};
 
   printf("### Application: LightNE_mkl\n");
   printf("### Graph: %s\n", gen.empty() ? iFile : gen.c_str());

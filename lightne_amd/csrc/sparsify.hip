@@ -92,6 +92,7 @@ __device__ __forceinline__ void block_add3(unsigned long long* dst, uint64_t a, 
 struct SampleArgs {
   const int64_t* off;
   const uint32_t* adj;
+  uint64_t ith;  // 1: compressed-graph neighbour lookup
   const uint32_t* src;
   int64_t m;
   uint64_t seed;
@@ -106,9 +107,13 @@ struct SampleArgs {
 };
 
 // random_walk (PathEmbed.h:37-49): `len` steps from x, state advances one
-// draw per step.  rand = H(s); next = H(H(s)) = H(rand).
+// draw per step.  rand = H(s); next = H(H(s)) = H(rand).  Draw i picks
+// neighbour i, or with `ith` = 1 (graphs read from the compressed format)
+// neighbour i - 1 unless i is a multiple of 64: the result of
+// get_ith_neighbor on such graphs (ligra/encodings/byte_pd_amortized.h:653-688
+// decodes i - start - 1 differences after the block's first edge).
 __device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                         uint32_t x, uint64_t len, uint64_t& s, bool& bad) {
+                                         uint32_t x, uint64_t len, uint64_t& s, bool& bad, uint64_t ith) {
   for (uint64_t t = 0; t < len; ++t) {
     const int64_t o = off[x];
     const uint64_t d = (uint64_t)(off[x + 1] - o);
@@ -117,7 +122,9 @@ __device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const 
       break;
     }
     const uint64_t r = hash64_2(s);
-    x = adj[o + (int64_t)(r % d)];
+    uint64_t i = r % d;
+    i -= ith & (uint64_t)((i & 63) != 0);
+    x = adj[o + (int64_t)i];
     s = hash64_2(r);
   }
   return x;
@@ -179,8 +186,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       rr = hash64_2(s);
       const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
       s = hash64_2(rr);
-      const uint32_t x = walk(a.off, a.adj, u, (uint64_t)kk - 1, s, bad);
-      const uint32_t z = walk(a.off, a.adj, v, r - kk, s, bad);
+      const uint32_t x = walk(a.off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith);
+      const uint32_t z = walk(a.off, a.adj, v, r - kk, s, bad, a.ith);
       n_steps += r - 1;
       const uint64_t key = ((uint64_t)min(x, z) << 32) + (uint64_t)max(x, z);
       if (!table_add<MODE>(a, key, ipw)) overflow = true;
@@ -400,6 +407,8 @@ int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
   if (!g || g->n <= 0 || g->m < 0 || !g->offsets || (g->m > 0 && !g->edges))
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "invalid graph descriptor");
   if (g->n >= (int64_t)1 << 32) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 vertices");
+  if (g->ith_rule != LIGHTNE_ITH_PLAIN && g->ith_rule != LIGHTNE_ITH_BYTEPD)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph ith_rule must be LIGHTNE_ITH_PLAIN or LIGHTNE_ITH_BYTEPD");
   // the sampler runs one lane per directed edge in one launch
   if (g->m >= ((int64_t)1 << 32) - 256)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 directed edges");
@@ -464,6 +473,7 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     SampleArgs a;
     a.off = g->offsets;
     a.adj = g->edges;
+    a.ith = g->ith_rule == LIGHTNE_ITH_BYTEPD ? 1u : 0u;
     a.src = src;
     a.m = g->m;
     a.seed = p->seed;

@@ -119,12 +119,19 @@ static inline void tab_insert(orc_table* t, uint64_t key, double w) {
 }
 
 /* PathEmbed.h:37-49 random_walk: walk_len steps, i = rand % deg(u) (u64 % u64),
- * u = i-th neighbour in input order (ligra/vertex.h:296-300), state advances. */
+ * u = get_ith_out_neighbor(u, i), state advances.  ith = 0: neighbour i in
+ * input order (ligra/vertex.h:296-300).  ith = 1: a graph read from the
+ * compressed format, whose lookup (ligra/encodings/byte_pd_amortized.h:653-688)
+ * decodes i - start - 1 differences after block start = 64 (i / 64), i.e.
+ * returns neighbour i - 1 unless i is a multiple of 64.
+ * pinned by: the harness's dump of the reference's get_ith_out_neighbor on
+ * compressed files (every index of three graphs). */
 static inline uint32_t walk(const uint64_t* off, const uint32_t* adj, uint32_t u,
-                            uint64_t len, uint64_t* s, uint64_t* steps) {
+                            uint64_t len, uint64_t* s, uint64_t* steps, int ith) {
   for (uint64_t i = 0; i < len; ++i) {
     uint64_t d = off[u + 1] - off[u];
     uint64_t idx = r_rand(*s) % d;
+    if (ith && (idx % 64) != 0) idx -= 1;
     u = adj[off[u] + idx];
     *s = r_next(*s);
   }
@@ -134,10 +141,10 @@ static inline uint32_t walk(const uint64_t* off, const uint32_t* adj, uint32_t u
 
 /* PathEmbed.h:51-58 path_sample with walk length r. */
 static inline void path_sample(const uint64_t* off, const uint32_t* adj, uint32_t u, uint32_t v,
-                               uint64_t r, uint64_t* s, uint32_t* x, uint32_t* z, uint64_t* steps) {
+                               uint64_t r, uint64_t* s, uint32_t* x, uint32_t* z, uint64_t* steps, int ith) {
   uint32_t k = (uint32_t)(r_rand(*s) % r + 1); *s = r_next(*s);
-  *x = walk(off, adj, u, (uint64_t)k - 1, s, steps);
-  *z = walk(off, adj, v, r - k, s, steps);
+  *x = walk(off, adj, u, (uint64_t)k - 1, s, steps, ith);
+  *z = walk(off, adj, v, r - k, s, steps, ith);
 }
 
 typedef struct {
@@ -184,12 +191,13 @@ static int bits_for(uint64_t n) { int b = 0; while (((uint64_t)1 << b) < n) ++b;
  *   nthreads  : OpenMP threads for the edge loop (1 = serial, deterministic weights)
  *   u_begin, u_end, u_stride : sample only the edges of source vertices
  *               u_begin, u_begin + u_stride, ... < u_end (u_end = 0: n)
+ *   ith       : neighbour lookup of the walks (0 plain, 1 compressed graph)
  * pinned by: golden key->count tables from the reference-header harness;
  * T=1 closed form; zero-state fixed point; sum(cnt) == samples invariant. */
 orc_sample_result* orc_sample(uint64_t n, const uint64_t* off, const uint32_t* adj, uint64_t seed,
                               uint64_t walk_len, int sample, uint64_t exp_rv, double frac,
                               double logn, int cap_log2, int nthreads, uint64_t u_begin, uint64_t u_end,
-                              uint64_t u_stride) {
+                              uint64_t u_stride, int ith) {
   orc_table t;
   tab_init(&t, (uint64_t)1 << cap_log2);
   uint64_t samples = 0, attempts = 0, steps = 0;
@@ -213,7 +221,7 @@ orc_sample_result* orc_sample(uint64_t n, const uint64_t* off, const uint32_t* a
           uint64_t r = (uint32_t)(r_rand(s) % walk_len + 1);
           s = r_next(s);
           uint32_t x, z;
-          path_sample(off, adj, (uint32_t)u, v, r, &s, &x, &z, &steps);
+          path_sample(off, adj, (uint32_t)u, v, r, &s, &x, &z, &steps, ith);
           uint64_t key = ((uint64_t)(x < z ? x : z) << 32) + (uint64_t)(x < z ? z : x);
           tab_insert(&t, key, 1.0);
           samples++;
@@ -231,7 +239,7 @@ orc_sample_result* orc_sample(uint64_t n, const uint64_t* off, const uint32_t* a
             uint64_t r = (uint32_t)(r_rand(s) % walk_len + 1);
             s = r_next(s);
             uint32_t x, z;
-            path_sample(off, adj, (uint32_t)u, v, r, &s, &x, &z, &steps);
+            path_sample(off, adj, (uint32_t)u, v, r, &s, &x, &z, &steps, ith);
             uint64_t key = ((uint64_t)(x < z ? x : z) << 32) + (uint64_t)(x < z ? z : x);
             tab_insert(&t, key, ipw);
             samples++;

@@ -42,7 +42,7 @@ def load():
     sig = {
         "orc_hash64_2": (u64, [u64]),
         "orc_sample_scalars": (None, [u64, u64, u64, f32, f32, u64, P, P]),
-        "orc_sample": (P, [u64, P, P, u64, u64, C.c_int, u64, f64, f64, C.c_int, C.c_int, u64, u64, u64]),
+        "orc_sample": (P, [u64, P, P, u64, u64, C.c_int, u64, f64, f64, C.c_int, C.c_int, u64, u64, u64, C.c_int]),
         "orc_sample_nkeys": (u64, [P]),
         "orc_sample_stats": (None, [P, P]),
         "orc_sample_copy": (None, [P, P, P, P]),
@@ -87,10 +87,10 @@ def sample_scalars(n: int, m: int, walk_len: int, sample_ratio: float, mem_ratio
 
 
 def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sample_ratio: float,
-           sample_mode: int = 0, threads: int = 0, cap_log2: int = 0, vertex_range=None):
+           sample_mode: int = 0, threads: int = 0, cap_log2: int = 0, vertex_range=None, ith: int = 0):
     """Per-directed-edge sampler + table (PathEmbed.h:128-188).  ``vertex_range``
     = (u0, u1, stride) samples only the edges of source vertices u0, u0 + stride,
-    ... < u1.
+    ... < u1.  ``ith`` = 1 walks with the compressed-graph neighbour lookup.
 
     Returns (keys uint64 sorted, counts uint64, weights float64, stats dict)."""
     lib = load()
@@ -106,7 +106,8 @@ def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sam
         threads = int(lib.orc_max_threads())
     u0, u1, us = vertex_range if vertex_range is not None else (0, 0, 1)
     h = lib.orc_sample(n, _p(off), _p(adj), seed & 0xFFFFFFFFFFFFFFFF, walk_len, sample_mode,
-                       sc["exp_rv_rounded"], sc["fractional_bits"], sc["logn"], cap_log2, threads, u0, u1, us)
+                       sc["exp_rv_rounded"], sc["fractional_bits"], sc["logn"], cap_log2, threads, u0, u1, us,
+                       int(ith))
     try:
         nk = int(lib.orc_sample_nkeys(h))
         st = np.zeros(4, np.uint64)
@@ -120,6 +121,105 @@ def sample(offsets: np.ndarray, edges: np.ndarray, seed: int, walk_len: int, sam
     stats = dict(sc)
     stats.update(samples=int(st[0]), attempts=int(st[1]), walk_steps=int(st[2]), overflow=int(st[3]))
     return keys, cnt, wgh, stats
+
+
+# ---------------------------------------------------------------------------
+# compressed graph format (ligra/encodings/byte_pd_amortized.h)
+# ---------------------------------------------------------------------------
+def _u32(img: np.ndarray, pos: int) -> int:
+    return int(img[pos]) | int(img[pos + 1]) << 8 | int(img[pos + 2]) << 16 | int(img[pos + 3]) << 24
+
+
+def _eat_first(img: np.ndarray, pos: int, src: int):
+    """byte_pd_amortized.h:86-100: 6 bits + sign, then 7 bits per byte."""
+    fb = int(img[pos]); pos += 1
+    val = fb & 0x3F
+    if fb & 0x80:
+        shift = 6
+        while True:
+            b = int(img[pos]); pos += 1
+            val |= (b & 0x7F) << shift
+            if b & 0x80:
+                shift += 7
+            else:
+                break
+    val &= 0xFFFFFFFF
+    return ((src - val) if fb & 0x40 else (src + val)) & 0xFFFFFFFF, pos
+
+
+def _eat_edge(img: np.ndarray, pos: int):
+    """byte_pd_amortized.h:105-119: unsigned LEB128."""
+    val, shift = 0, 0
+    while True:
+        b = int(img[pos]); pos += 1
+        val += (b & 0x7F) << shift
+        if b & 0x80:
+            shift += 7
+        else:
+            break
+    return val & 0xFFFFFFFF, pos
+
+
+def _bytepd_sections(img: np.ndarray):
+    n, m, _ = (int(x) for x in np.frombuffer(img[:24].tobytes(), np.int64))
+    voff = np.frombuffer(img[24:24 + 8 * (n + 1)].tobytes(), np.uint64).astype(np.int64)
+    o = 24 + 8 * (n + 1)
+    deg = np.frombuffer(img[o:o + 4 * n].tobytes(), np.uint32).astype(np.int64)
+    return n, m, voff, deg, img[o + 4 * n:]
+
+
+def decode_bytepd(image: np.ndarray):
+    """Sequential decode of a compressed-format file (graph_io.h:344-377 +
+    byte_pd_amortized.h:330-386, blocks in order).  Returns (offsets int64,
+    edges uint32).  Small graphs only (a Python loop per edge).
+    pinned by: the reference reader's own decode of the same files
+    (harness dump, tests/golden/bytepd_*.npz)."""
+    img = np.ascontiguousarray(image, np.uint8)
+    n, m, voff, deg, e = _bytepd_sections(img)
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(deg)
+    out = np.zeros(m, np.uint32)
+    for v in range(n):
+        d = int(deg[v])
+        if d == 0:
+            continue
+        base = int(voff[v])
+        vd = _u32(e, base)
+        nb = 1 + (vd - 1) // 64
+        for b in range(nb):
+            pos = base + 4 + 4 * (nb - 1) if b == 0 else base + _u32(e, base + 4 + 4 * (b - 1))
+            start = _u32(e, pos)
+            end = d if b == nb - 1 else _u32(e, base + _u32(e, base + 4 + 4 * b))
+            pos += 4
+            if start < end:
+                ngh, pos = _eat_first(e, pos, v)
+                out[off[v] + start] = ngh
+                for j in range(start + 1, end):
+                    delta, pos = _eat_edge(e, pos)
+                    ngh = (ngh + delta) & 0xFFFFFFFF
+                    out[off[v] + j] = ngh
+    return off, out
+
+
+def ith_neighbor_bytepd(image: np.ndarray, v: int, i: int) -> int:
+    """get_ith_neighbor (byte_pd_amortized.h:653-688) as written: decode the
+    block of i from its first edge, then i - start - 1 further differences."""
+    img = np.ascontiguousarray(image, np.uint8)
+    n, m, voff, deg, e = _bytepd_sections(img)
+    d = int(deg[v])
+    base = int(voff[v])
+    nb = 1 + (d - 1) // 64
+    block = i // 64
+    pos = base + 4 + 4 * (nb - 1) if block == 0 else base + _u32(e, base + 4 + 4 * (block - 1))
+    start = _u32(e, pos)
+    pos += 4
+    ngh, pos = _eat_first(e, pos, v)
+    if i == start:
+        return ngh
+    for _ in range(start + 1, i):
+        delta, pos = _eat_edge(e, pos)
+        ngh = (ngh + delta) & 0xFFFFFFFF
+    return ngh
 
 
 def trunc_log_csr(offsets: np.ndarray, keys: np.ndarray, vals: np.ndarray, negative: int = 1, upper: int = 0,

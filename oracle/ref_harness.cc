@@ -16,8 +16,17 @@
 // (PathEmbed.h:37-58,128-171) and the trunc-log map (PathEmbed.h:199-218) are
 // therefore restated below, driven entirely by the reference primitives above.
 //
-// usage: ref_harness graph.adj seed walk_len sample sample_ratio mem_ratio
-//                    tablesz negative out.bin
+// With a tenth argument "-c" the graph is read by
+// gbbs_io::read_compressed_symmetric_graph (ligra/graph_io.h:344-377), so the
+// walks go through the compressed vertex's get_ith_out_neighbor
+// (ligra/encodings/byte_pd_amortized.h:653-688) and map_edges through its
+// block decoder; the harness then also writes out.bin.csr:
+//   u64 n, u64 m, u64 offsets[n+1] (prefix sums of the degrees),
+//   u32 decoded[m] (mapOutNghIdx: edge j of u at offsets[u] + j),
+//   u32 ith[m]     (get_ith_out_neighbor(u, j) at offsets[u] + j).
+//
+// usage: ref_harness graph seed walk_len sample sample_ratio mem_ratio
+//                    tablesz negative out.bin [-c]
 // out.bin: u64 magic 'LNEGOLD1', u64 nkeys, u64 keys[nkeys] (sorted),
 //          f64 vals[nkeys], f64 trunclog[nkeys] (NaN where erased),
 //          f64 {logn, M, one_over_lambda, frac, M_exact, factor},
@@ -32,6 +41,7 @@
 #include <cmath>
 #include <cstdio>
 #include <limits>
+#include <string>
 #include <vector>
 
 namespace {
@@ -50,12 +60,8 @@ uintE walk_steps(Graph& GA, uintE u, size_t len, pbbs::random& s) {
 
 }  // namespace
 
-int main(int argc, char** argv) {
-  if (argc != 10) {
-    std::fprintf(stderr, "usage: %s graph seed walk_len sample sample_ratio mem_ratio tablesz negative out\n", argv[0]);
-    return 2;
-  }
-  auto GA = gbbs_io::read_unweighted_symmetric_graph(argv[1], false);
+template <class Graph>
+int run(Graph& GA, char** argv) {
   size_t seed = std::stoull(argv[2]);
   size_t walk_len = std::stoull(argv[3]);
   bool sample = std::stoi(argv[4]) != 0;
@@ -177,4 +183,43 @@ int main(int argc, char** argv) {
   std::printf("n=%zu m=%zu nkeys=%zu kept=%zu M_exact=%.17g factor=%.17g\n", n, m, before.size(),
               after.size(), M_exact, factor);
   return 0;
+}
+
+template <class Graph>
+int dump_csr(Graph& GA, const char* path) {
+  const size_t n = GA.n, m = GA.m;
+  std::vector<uint64_t> off(n + 1, 0);
+  for (size_t u = 0; u < n; ++u) off[u + 1] = off[u] + GA.get_vertex(u).getOutDegree();
+  std::vector<uint32_t> dec(m, 0xffffffffu), ith(m, 0xffffffffu);
+  for (size_t u = 0; u < n; ++u) {
+    auto f = [&](const uintE& src, const uintE& v, const pbbs::empty&, const uintT& j) { dec[off[src] + j] = v; };
+    GA.get_vertex(u).mapOutNghIdx(u, f, false);
+    const size_t d = off[u + 1] - off[u];
+    for (size_t j = 0; j < d; ++j) ith[off[u] + j] = std::get<0>(GA.get_vertex(u).get_ith_out_neighbor(u, j));
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 3;
+  const uint64_t hdr[2] = {n, m};
+  std::fwrite(hdr, 8, 2, f);
+  std::fwrite(off.data(), 8, n + 1, f);
+  std::fwrite(dec.data(), 4, m, f);
+  std::fwrite(ith.data(), 4, m, f);
+  std::fclose(f);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  const bool compressed = argc == 11 && std::string(argv[10]) == "-c";
+  if (argc != 10 && !compressed) {
+    std::fprintf(stderr, "usage: %s graph seed walk_len sample sample_ratio mem_ratio tablesz negative out [-c]\n",
+                 argv[0]);
+    return 2;
+  }
+  if (compressed) {
+    auto GA = gbbs_io::read_compressed_symmetric_graph<pbbslib::empty>(argv[1], true, false);
+    const int rc = run(GA, argv);
+    return rc ? rc : dump_csr(GA, (std::string(argv[9]) + ".csr").c_str());
+  }
+  auto GA = gbbs_io::read_unweighted_symmetric_graph(argv[1], false);
+  return run(GA, argv);
 }

@@ -128,8 +128,15 @@ def test_cli_usage_and_errors(tmp_path):
     assert out.returncode == 2 and "usage" in out.stdout
     out = subprocess.run([CLI, "-rounds", "1", "-s", str(tmp_path / "missing.adj")], capture_output=True, text=True)
     assert out.returncode == 1 and "cannot open" in out.stderr
-    out = subprocess.run([CLI, "-c", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
-    assert out.returncode == 2 and "compressed" in out.stderr
+    out = subprocess.run([CLI, "-rounds", "1", "-c", "-s", str(tmp_path / "missing.cmp")], capture_output=True,
+                         text=True)
+    assert out.returncode == 1 and "cannot open" in out.stderr
+    (tmp_path / "short.cmp").write_bytes(b"\x01" * 10)
+    out = subprocess.run([CLI, "-rounds", "1", "-c", "-s", str(tmp_path / "short.cmp")], capture_output=True,
+                         text=True)
+    assert out.returncode == 1 and "not a compressed graph" in out.stderr
+    out = subprocess.run([CLI, "-ith_rule", "nope", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
+    assert out.returncode == 2
     out = subprocess.run([CLI, "-ne_method", "nope", "-s", str(tmp_path / "x.adj")], capture_output=True, text=True)
     assert out.returncode == 2
 

@@ -15,14 +15,26 @@ def dgraph(off, edg):
     return L.HostGraph(off, edg).to_device()
 
 
+def case_graph(rec, ctx):
+    """The case's graph on the device; compressed-format cases go through the
+    device decoder of the encoded file, as `lightne -c` does."""
+    off, edg = H.golden_graph(rec["graph"])
+    if not rec.get("compressed"):
+        return off, edg, dgraph(off, edg), 0
+    import torch
+    from lightne_amd import bytepd
+
+    img = torch.from_numpy(bytepd.encode(off, edg)).cuda()
+    return off, edg, L.decode_compressed_graph(img, ctx=ctx), 1
+
+
 @pytest.mark.parametrize("rec", H.manifest(), ids=lambda r: r["case"])
 def test_counts_match_golden_and_oracle(gpu, rec):
-    off, edg = H.golden_graph(rec["graph"])
-    g = dgraph(off, edg)
+    off, edg, g, ith = case_graph(rec, gpu)
     t = L.sparsify_counts(g, int(rec["seed"]), rec["walk_len"], bool(rec["sample"]),
                           sample_ratio=rec["sample_ratio"], ctx=gpu)
     keys, cnt, wgh, st = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
-                                  threads=1)
+                                  threads=1, ith=ith)
     np.testing.assert_array_equal(t.keys, keys)
     np.testing.assert_array_equal(t.counts.astype(np.uint64), cnt)
     if rec["sample"] == 0:
@@ -49,12 +61,11 @@ def _compare_csr(csr, ref, ulps=1):
 
 @pytest.mark.parametrize("rec", H.manifest(), ids=lambda r: r["case"])
 def test_csr_matches_oracle(gpu, rec):
-    off, edg = H.golden_graph(rec["graph"])
-    g = dgraph(off, edg)
+    off, edg, g, ith = case_graph(rec, gpu)
     csr, st = L.generate_trunc_log_matrix(g, int(rec["seed"]), walk_len=rec["walk_len"], sample=bool(rec["sample"]),
                                           sample_ratio=rec["sample_ratio"], ctx=gpu)
     keys, cnt, wgh, ost = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
-                                   threads=1)
+                                   threads=1, ith=ith)
     if rec["sample"] == 0:
         assert st["M_exact"] == ost["samples"]
         ref = O.trunc_log_csr(off, keys, wgh)

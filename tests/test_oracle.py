@@ -27,11 +27,13 @@ def test_hash64_2_known_answers():
 @pytest.mark.parametrize("rec", H.manifest(), ids=lambda r: r["case"])
 def test_sampler_matches_reference_harness(rec):
     """Key -> value tables of the reference-header harness (pbbs::random, hash64_2,
-    par_table, ligra reader) reproduced bit for bit, both -sample modes."""
+    par_table, ligra reader) reproduced bit for bit, both -sample modes; for the
+    compressed-format cases the harness read the -c file, so its walks used the
+    reference's compressed-graph neighbour lookup."""
     off, edg = H.golden_graph(rec["graph"])
     gd = H.golden_case(rec["case"])
     keys, cnt, wgh, st = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"],
-                                  rec["sample"], threads=1)
+                                  rec["sample"], threads=1, ith=int(rec.get("compressed", False)))
     sc = gd["scalars"]
     assert st["logn"] == sc[0] and st["M"] == sc[1] and st["one_over_lambda"] == sc[2]
     assert st["fractional_bits"] == sc[3] and st["exp_rv_rounded"] == int(gd["scalars_u"][0])
@@ -61,7 +63,7 @@ def test_trunc_log_matches_reference_harness(rec):
         keys, vals, tl = gd["keys"], gd["vals"], gd["trunclog"]
     else:
         keys, _, vals, _ = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"],
-                                    rec["sample"], threads=1)
+                                    rec["sample"], threads=1, ith=int(rec.get("compressed", False)))
         tl = None
     rp, col, val, info = O.trunc_log_csr(off, keys, vals, M_exact=float(gd["scalars"][4]))
     assert info["factor"] == gd["scalars"][5]
