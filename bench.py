@@ -37,10 +37,11 @@ CONFIGS = {
     # R-MAT scale 24, edge factor 16, (a, b, c) = (.57, .19, .19); T = 5, M = 10 m, q = 2
     "c3": dict(rmat=(24, 16.0, 0.57, 0.19, 0.19), gseed=3, T=5, per_edge=5.0, k=144, d=128, q=2, K=10,
                cpu_frac=1 / 64, cpu_prop_one=True),
-    # R-MAT scale 26, edge factor 14; the sparsifier sized to fit one GPU's
-    # 288 GB: 1.5 samples per directed edge (M = 3 m) instead of 5 (M = 10 m)
-    "c5": dict(rmat=(26, 14.0, 0.57, 0.19, 0.19), gseed=5, T=5, per_edge=1.5, k=144, d=128, q=1, K=10,
-               cpu_frac=1 / 256, cpu_prop_one=True),
+    # R-MAT scale 26, edge factor 14, T = 5, M = 10 m; the table takes 8-byte
+    # slots and the CSR is built by row ranges to fit one GPU; the propagation
+    # writes its result over its input (one n x d buffer less)
+    "c5": dict(rmat=(26, 14.0, 0.57, 0.19, 0.19), gseed=5, T=5, per_edge=5.0, k=144, d=128, q=1, K=10,
+               cpu_frac=1 / 256, cpu_prop_one=True, inplace=True),
 }
 
 
@@ -58,7 +59,7 @@ def run_pipeline(L, g, cfg, ratio, precision, ctx, seed=1):
     emb, S, st = L.NetSMF(g, walk_len=cfg["T"], rank=cfg["k"], dim=cfg["d"], sample_ratio=ratio, seed=seed,
                           svd_seed=777, power_iters=cfg["q"], precision=precision, ctx=ctx)
     pro, pst = L.chebyshev_expansion(emb, g, cfg["d"], cfg["d"], order=cfg["K"], mu=0.2, s=0.5, precision=precision,
-                                     ctx=ctx)
+                                     out=emb if cfg.get("inplace") else None, ctx=ctx)
     torch.cuda.synchronize()
     return emb, S, pro, st, pst
 
@@ -247,6 +248,8 @@ def main():
     ctx = L.context(0)
     g = hg.to_device("cuda:0")
     torch.cuda.synchronize()
+    free_b, total_b = torch.cuda.mem_get_info(0)
+    log(f"device memory: {total_b / 1e9:.1f} GB total, {free_b / 1e9:.1f} GB free with the graph resident")
     for w in range(args.warmup):
         run_pipeline(L, g, cfg, ratio, L.FP32, ctx)
         log(f"warmup {w} done")
@@ -382,8 +385,14 @@ def main():
         log("cpu baseline ...")
         emb = pro = None
         torch.cuda.empty_cache()  # torch's cached blocks back to the device for the sparsifier's CSR
+        if cfg.get("inplace"):
+            # the pipeline context's workspace goes back first; only the rows
+            # the CPU products use come to the host
+            ctx.close()
+            L.api._CTX.pop(0, None)
+            ctx = L.context(0)
         csr, _ = L.generate_trunc_log_matrix(g, 1, walk_len=cfg["T"], sample_ratio=ratio, ctx=ctx)
-        csr_box = [csr.to_numpy()]
+        csr_box = [csr.to_numpy(nnz_fraction=cfg.get("cpu_frac", 0.25) if cfg.get("inplace") else None)]
         csr.free()
         log("cpu: sparsifier copied to the host")
         cpu = cpu_baseline(hg, cfg, ratio, csr_box, frac=cfg.get("cpu_frac", 0.25))
@@ -398,7 +407,8 @@ def main():
                    "parallelism": "1 GPU"},
         "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
         "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
-        "table_capacity": sp["table_capacity"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
+        "table_capacity": sp["table_capacity"], "table_slot_bytes": 8 if sp["table_layout"] == 2 else 16,
+        "csr_row_ranges": sp["csr_parts"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

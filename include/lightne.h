@@ -115,6 +115,11 @@ typedef struct {
   uint64_t tablesz;         /* -tablesz                                      */
   uint64_t seed;            /* pbbs::random state (reference: time(0))      */
   uint64_t max_table_bytes; /* cap on the hash table (0 = no cap)            */
+  int32_t table_layout;     /* 0 auto (8-byte slots when n <= 2^26), 1 16-byte
+                               slots, 2 8-byte slots                        */
+  int32_t csr_parts;        /* 0 auto: the CSR is built by row ranges only
+                               when one global sort does not fit; k >= 2:
+                               at least k ranges                           */
 } lightne_sparsify_params;
 
 typedef struct {
@@ -133,6 +138,8 @@ typedef struct {
   uint64_t nnz;            /* CSR entries                                  */
   /* device time in seconds (HIP events on the context stream) */
   double t_sample, t_table, t_csr;
+  uint32_t table_layout;   /* 1 = 16-byte slots, 2 = 8-byte slots          */
+  uint32_t csr_parts;      /* row ranges the CSR was built in (1 = one sort) */
 } lightne_sparsify_stats;
 
 /* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */
@@ -216,7 +223,8 @@ int lightne_embed_dense(lightne_ctx* ctx, const float* mat, int64_t n, int64_t r
                         float* emb_out, float* S_host);
 
 /* chebyshev_expansion(a, GA, rank=dim, dim, order, mu, s): propagation, then
- * get_embedding_dense.  emb_out: n x dim. */
+ * get_embedding_dense.  emb_out: n x dim; it may be a itself (a is read in
+ * full before emb_out is written), which saves one n x dim buffer. */
 int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                                 const lightne_prop_params* p, float* emb_out, lightne_prop_stats* stats);
 

@@ -250,10 +250,13 @@ def sample_scalars(n: int, m: int, walk_len: int = 10, sample_ratio: float = 1.0
 
 
 def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                     max_table_bytes) -> SparsifyParams:
+                     max_table_bytes, table_layout=0, csr_parts=0) -> SparsifyParams:
+    """``table_layout``: 0 auto, 1 16-byte slots, 2 8-byte slots (n <= 2^26).
+    ``csr_parts``: 0 auto, k >= 2 builds the CSR in at least k row ranges."""
     return SparsifyParams(walk_len=walk_len, negative=negative, sample=int(bool(sample)), upper=int(bool(upper)),
                           sample_ratio=sample_ratio, mem_ratio=mem_ratio, tablesz=table_size,
-                          seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes)
+                          seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes,
+                          table_layout=table_layout, csr_parts=csr_parts)
 
 
 @dataclasses.dataclass
@@ -266,11 +269,11 @@ class SparseTable:
 
 def sparsify_counts(graph: DeviceGraph, seed: int, walk_len: int = 10, sample: bool = False,
                     negative: int = 1, sample_ratio: float = 1.0, mem_ratio: float = 1.0, table_size: int = 0,
-                    max_table_bytes: int = 0, ctx: Optional[Context] = None) -> SparseTable:
+                    max_table_bytes: int = 0, table_layout: int = 0, ctx: Optional[Context] = None) -> SparseTable:
     """Key -> (count, weight) multiset of the sampler (PathEmbed.h:128-188)."""
     ctx = ctx or context()
     p = _sparsify_params(seed, walk_len, False, sample, table_size, negative, sample_ratio, mem_ratio,
-                         max_table_bytes)
+                         max_table_bytes, table_layout)
     st = SparsifyStats()
     tab = C.c_void_p()
     g = graph.desc()
@@ -317,13 +320,31 @@ class CSR:
                                              ptr(val, np.float32), C.byref(h)))
         return cls(ctx, h)
 
-    def to_numpy(self):
+    def to_numpy(self, nnz_fraction: Optional[float] = None):
+        """Host copies (row_ptr, col, val).  With ``nnz_fraction`` only the
+        entries of the first rows holding that fraction of the nonzeros are
+        copied (row_ptr stays whole)."""
         rp = np.empty(self.n + 1, np.int64)
-        col = np.empty(self.nnz, np.int32)
-        val = np.empty(self.nnz, np.float32)
-        self.ctx.check(self.ctx.lib.lightne_csr_copy(self.ctx.ptr, self.handle, rp.ctypes.data, col.ctypes.data,
-                                                     val.ctypes.data))
-        return rp, col, val
+        if nnz_fraction is None:
+            col = np.empty(self.nnz, np.int32)
+            val = np.empty(self.nnz, np.float32)
+            self.ctx.check(self.ctx.lib.lightne_csr_copy(self.ctx.ptr, self.handle, rp.ctypes.data,
+                                                         col.ctypes.data, val.ctypes.data))
+            return rp, col, val
+        self.ctx.check(self.ctx.lib.lightne_csr_copy(self.ctx.ptr, self.handle, rp.ctypes.data, None, None))
+        r = int(np.searchsorted(rp, nnz_fraction * self.nnz))
+        nz = int(rp[min(r, self.n)])
+        torch = _torch()
+        col = torch.empty(nz, dtype=torch.int32)
+        val = torch.empty(nz, dtype=torch.float32)
+        if nz:
+            rpp, cp, vp = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            check(self.ctx.lib.lightne_csr_device_ptrs(self.handle, C.byref(rpp), C.byref(cp), C.byref(vp)))
+            hip = C.CDLL("libamdhip64.so")
+            for dst, src, b in ((col, cp, 4 * nz), (val, vp, 4 * nz)):
+                if hip.hipMemcpy(C.c_void_p(dst.data_ptr()), src, C.c_size_t(b), 2) != 0:  # device to host
+                    raise LightNEError(_lib.ERR_HIP, "hipMemcpy of the CSR prefix failed")
+        return rp, col.numpy(), val.numpy()
 
     def free(self) -> None:
         if self.handle:
@@ -340,13 +361,13 @@ class CSR:
 def generate_trunc_log_matrix(graph: DeviceGraph, seed: int, walks_per_edge: int = 100, walk_len: int = 10,
                               upper: bool = False, sample: bool = False, table_size: int = 0, negative: int = 1,
                               sample_ratio: float = 1.0, mem_ratio: float = 1.0, max_table_bytes: int = 0,
-                              ctx: Optional[Context] = None):
+                              table_layout: int = 0, csr_parts: int = 0, ctx: Optional[Context] = None):
     """Stage 1 (PathEmbed.h:81-316): returns (CSR, stats).  ``walks_per_edge`` is
     accepted and unused, as in the reference (PathEmbed.h:107-108)."""
     del walks_per_edge
     ctx = ctx or context()
     p = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                         max_table_bytes)
+                         max_table_bytes, table_layout, csr_parts)
     st = SparsifyStats()
     h = C.c_void_p()
     g = graph.desc()
@@ -406,14 +427,15 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
            negative: int = 1, normalize: bool = False, sample_ratio: float = 1.0, mem_ratio: float = 1.0,
            random_project_only: bool = False, seed: int = 0, svd_seed: int = 777, power_iters: int = 1,
            precision: int = FP32, max_table_bytes: int = 0, sparse_project: bool = False,
-           sparse_project_s: float = 100.0, ctx: Optional[Context] = None):
+           sparse_project_s: float = 100.0, table_layout: int = 0, csr_parts: int = 0,
+           ctx: Optional[Context] = None):
     """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats).
     ``sparse_project_s`` defaults to 100 as LightNE.cc:50 does."""
     del walks_per_edge, analyze
     torch = _torch()
     ctx = ctx or context()
     sp = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                          max_table_bytes)
+                          max_table_bytes, table_layout, csr_parts)
     rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
                     random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed,
                     sparse_project=int(bool(sparse_project)), sparse_project_s=sparse_project_s)
@@ -486,15 +508,19 @@ def get_embedding_dense(mat, n: int, rank: int, dim: int, ctx: Optional[Context]
 
 
 def chebyshev_expansion(a, graph: DeviceGraph, rank: int, dim: int, order: int = 10, mu: float = 0.2,
-                        s: float = 0.5, precision: int = FP32, ctx: Optional[Context] = None):
-    """Stage 3 (SpectralPropagation.h:45-229), non-mutating.  Returns (emb, stats)."""
+                        s: float = 0.5, precision: int = FP32, out=None, ctx: Optional[Context] = None):
+    """Stage 3 (SpectralPropagation.h:45-229).  Returns (emb, stats).  ``out``
+    (n x dim FP32) receives the result and may be ``a`` itself: the input is
+    read in full before the output is written."""
     torch = _torch()
     if rank != dim:
         raise LightNEError(_lib.ERR_BAD_ARG, "chebyshev_expansion: the runner passes rank == dim (LightNE.cc:114)")
     ctx = ctx or context()
     a = a.contiguous()
     n = a.shape[0]
-    emb = torch.empty((n, dim), dtype=torch.float32, device=a.device)
+    emb = out if out is not None else torch.empty((n, dim), dtype=torch.float32, device=a.device)
+    if emb.shape != (n, dim) or emb.dtype != torch.float32 or not emb.is_contiguous():
+        raise LightNEError(_lib.ERR_BAD_ARG, "chebyshev_expansion: out must be a contiguous n x dim float32 tensor")
     p = PropParams(order=order, mu=mu, theta=s, precision=precision)
     st = PropStats()
     g = graph.desc()

@@ -3,12 +3,16 @@
 //
 //  k_edge_src      : source vertex of every directed edge (binary search in
 //                    the offsets), so the sampler can run one lane per edge.
-//  k_sample<MODE>  : one lane per directed edge (u, v); its pbbs::random chain
-//                    is seeded by fork((u << 32) | v) and drives exp_rv path
-//                    samples exactly as PathEmbed.h:128-171; every sample is
-//                    aggregated in an open-addressing table (par_table.h:86-109)
-//                    of 16-byte slots {key, count}, slot = hash64_2(key) & mask,
-//                    linear probing, one 64-bit CAS to claim a slot.
+//  k_sample<MODE, PACKED> : one lane per directed edge (u, v); its
+//                    pbbs::random chain is seeded by fork((u << 32) | v) and
+//                    drives exp_rv path samples exactly as PathEmbed.h:128-171;
+//                    every sample is aggregated in an open-addressing table
+//                    (par_table.h:86-109), linear probing from
+//                    hash64_2(key) scaled to the capacity, one 64-bit CAS to
+//                    claim a slot.  Slots are 8 bytes when n <= 2^26 (endpoints,
+//                    an occupied bit and a 10-bit count whose wraps are added to
+//                    a small side table, so counts stay exact), else 16 bytes
+//                    {key, count}.
 //  k_table_reduce  : distinct keys and total weight (PathEmbed.h:175-188),
 //                    deterministic two-level reduction.
 //  k_trunc_count / k_trunc_write : trunc-log (PathEmbed.h:199-218) fused with
@@ -17,6 +21,12 @@
 //                    scan, then writes in slot order (no contended atomics).
 //  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
 //                    row_ptr by boundary detection (no atomics, deterministic).
+//  k_trunc_hist / k_trunc_part : when the sort buffers for every entry do not
+//                    fit beside the table and the CSR, the rows are cut into
+//                    ranges by a row histogram; each range is emitted from the
+//                    table, sorted and written into its slice of the CSR.  The
+//                    (row, col) keys are unique, so the result is the same as
+//                    one global sort.
 #include "lightne_internal.h"
 
 #include <algorithm>
@@ -27,16 +37,96 @@ namespace {
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
 
+// wide slot: any n < 2^32
 struct __align__(16) Slot {
   unsigned long long key;
   unsigned int cnt;
   unsigned int pad;
 };
 
+// packed slot (n <= 2^26), 0 = empty:
+//   bits 0..51  (min << 26) | max     bit 52 occupied
+//   bit 53      the count wrapped at least once
+//   bits 54..63 count mod 1024; an increment adds 1 << 54 and the carry out of
+//               bit 63 is lost, so a wrap leaves the key bits intact; the lane
+//               whose increment wrapped adds 1024 to the side table
+constexpr int PK_VB = 26;
+constexpr uint64_t PK_OCC = 1ull << 52;
+constexpr uint64_t PK_OVF = 1ull << 53;
+constexpr uint64_t PK_ONE = 1ull << 54;
+constexpr uint64_t PK_WORD = (PK_OCC << 1) - 1;  // key bits + occupied bit
+constexpr uint64_t PK_VMASK = (1ull << PK_VB) - 1;
+constexpr uint64_t OVF_CAP = 1ull << 21;  // side table: {word, wrapped count} pairs (32 MiB)
+
 __host__ __device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
   x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
   x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
   return x ^ (x >> 31);
+}
+
+// first probe: the hash scaled to [0, cap)
+__device__ __forceinline__ uint64_t slot_of(uint64_t key, uint64_t cap) { return __umul64hi(hash64_2(key), cap); }
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+// side table of wrapped counts: probe for `word`, claim an empty pair
+__device__ __forceinline__ bool ovf_add(unsigned long long* ovf, uint64_t word, uint64_t add) {
+  uint64_t h = slot_of(word, OVF_CAP);
+  for (uint64_t p = 0; p < OVF_CAP; ++p) {
+    unsigned long long cur = __hip_atomic_load(&ovf[2 * h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == 0ull) {
+      cur = atomicCAS(&ovf[2 * h], 0ull, (unsigned long long)word);
+      if (cur == 0ull) cur = word;
+    }
+    if (cur == word) {
+      atomicAdd(&ovf[2 * h + 1], (unsigned long long)add);
+      return true;
+    }
+    h = (h + 1) & (OVF_CAP - 1);
+  }
+  return false;
+}
+
+__device__ __forceinline__ uint64_t ovf_get(const unsigned long long* __restrict__ ovf, uint64_t word) {
+  uint64_t h = slot_of(word, OVF_CAP);
+  for (uint64_t p = 0; p < OVF_CAP; ++p) {
+    const uint64_t cur = ovf[2 * h];
+    if (cur == word) return ovf[2 * h + 1];
+    if (cur == 0ull) return 0;
+    h = (h + 1) & (OVF_CAP - 1);
+  }
+  return 0;
+}
+
+// read-only view of either table layout
+struct TabView {
+  const Slot* wide;                // 16-byte slots, or
+  const unsigned long long* pk;    // 8-byte slots
+  const unsigned long long* ovf;   // side table of the packed layout
+  const double* wtab;              // -sample 1 weights (else null)
+  uint64_t cap;
+};
+
+// slot i: endpoints u <= v and the exact count; false when empty
+__device__ __forceinline__ bool tab_get(const TabView& t, uint64_t i, uint32_t& u, uint32_t& v, uint64_t& cnt) {
+  if (t.pk) {
+    const uint64_t w = t.pk[i];
+    if (!(w & PK_OCC)) return false;
+    u = (uint32_t)((w >> PK_VB) & PK_VMASK);
+    v = (uint32_t)(w & PK_VMASK);
+    cnt = w >> 54;
+    if (w & PK_OVF) cnt += ovf_get(t.ovf, w & PK_WORD);
+    return true;
+  }
+  const Slot sl = t.wide[i];
+  if (sl.key == EMPTY_KEY) return false;
+  u = (uint32_t)(sl.key >> 32);
+  v = (uint32_t)sl.key;
+  cnt = sl.cnt;
+  return true;
 }
 
 __global__ void k_fill_slots(Slot* __restrict__ tab, uint64_t cap) {
@@ -100,10 +190,12 @@ struct SampleArgs {
   uint64_t exp_rv;
   double frac;
   double logn;
-  Slot* tab;
+  Slot* tab;                 // wide layout
+  unsigned long long* pk;    // packed layout
+  unsigned long long* ovf;   // packed layout: side table of wrapped counts
   double* wtab;
-  uint64_t mask;
-  unsigned long long* stats;  // 0 samples, 1 steps, 2 attempts, 3 table overflow, 4 bad graph
+  uint64_t cap;
+  unsigned long long* stats;  // 0 samples, 1 steps, 2 attempts, 3 table overflow, 4 bad graph, 5 side table full
 };
 
 // random_walk (PathEmbed.h:37-49): `len` steps from x, state advances one
@@ -130,10 +222,38 @@ __device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const 
   return x;
 }
 
-template <int MODE>
-__device__ __forceinline__ bool table_add(const SampleArgs& a, uint64_t key, double w) {
-  uint64_t h = hash64_2(key) & a.mask;
-  for (uint64_t p = 0; p <= a.mask; ++p) {
+// par_table::insert_add (par_table.h:86-109): count (and weight) of `key`;
+// 0 when the table is full, -1 when the packed layout's side table is full
+template <int MODE, bool PACKED>
+__device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint32_t hi, double w) {
+  const uint64_t key = ((uint64_t)lo << 32) + (uint64_t)hi;
+  uint64_t h = slot_of(key, a.cap);
+  if (PACKED) {
+    const uint64_t word = ((uint64_t)lo << PK_VB) | (uint64_t)hi | PK_OCC;
+    for (uint64_t p = 0; p < a.cap; ++p) {
+      unsigned long long cur = __hip_atomic_load(&a.pk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == 0ull) {
+        cur = atomicCAS(&a.pk[h], 0ull, (unsigned long long)(word | PK_ONE));
+        if (cur == 0ull) {
+          if (MODE == 1) atomicAdd(&a.wtab[h], w);
+          return 1;
+        }
+      }
+      if ((cur & PK_WORD) == word) {
+        const unsigned long long old = atomicAdd(&a.pk[h], (unsigned long long)PK_ONE);
+        int rc = 1;
+        if ((old >> 54) == 1023ull) {
+          atomicOr(&a.pk[h], (unsigned long long)PK_OVF);
+          if (!ovf_add(a.ovf, word, 1024)) rc = -1;
+        }
+        if (MODE == 1) atomicAdd(&a.wtab[h], w);
+        return rc;
+      }
+      h = (h + 1 == a.cap) ? 0 : h + 1;
+    }
+    return 0;
+  }
+  for (uint64_t p = 0; p < a.cap; ++p) {
     unsigned long long cur = __hip_atomic_load(&a.tab[h].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == EMPTY_KEY) {
       cur = atomicCAS(&a.tab[h].key, EMPTY_KEY, (unsigned long long)key);
@@ -142,18 +262,18 @@ __device__ __forceinline__ bool table_add(const SampleArgs& a, uint64_t key, dou
     if (cur == key) {
       atomicAdd(&a.tab[h].cnt, 1u);
       if (MODE == 1) atomicAdd(&a.wtab[h], w);
-      return true;
+      return 1;
     }
-    h = (h + 1) & a.mask;
+    h = (h + 1 == a.cap) ? 0 : h + 1;
   }
-  return false;
+  return 0;
 }
 
-template <int MODE>
+template <int MODE, bool PACKED>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t n_samples = 0, n_steps = 0, n_attempts = 0;
-  bool overflow = false, bad = false;
+  bool overflow = false, bad = false, side_full = false;
   if (j < a.m) {
     const uint32_t u = a.src[j];
     const uint32_t v = a.adj[j];
@@ -189,32 +309,34 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       const uint32_t x = walk(a.off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith);
       const uint32_t z = walk(a.off, a.adj, v, r - kk, s, bad, a.ith);
       n_steps += r - 1;
-      const uint64_t key = ((uint64_t)min(x, z) << 32) + (uint64_t)max(x, z);
-      if (!table_add<MODE>(a, key, ipw)) overflow = true;
+      const int rc = table_add<MODE, PACKED>(a, min(x, z), max(x, z), ipw);
+      if (rc == 0) overflow = true;
+      if (rc < 0) side_full = true;
       ++n_samples;
     }
   }
   block_add3(a.stats, n_samples, n_steps, n_attempts);
   if (overflow) atomicOr(&a.stats[3], 1ull);
   if (bad) atomicOr(&a.stats[4], 1ull);
+  if (side_full) atomicOr(&a.stats[5], 1ull);
 }
 
 // per-block partial (distinct, count sum, weight sum); fixed reduction order
-__global__ __launch_bounds__(256) void k_table_reduce(const Slot* __restrict__ tab, const double* __restrict__ wtab,
-                                                      uint64_t cap, unsigned long long* __restrict__ part_u,
+__global__ __launch_bounds__(256) void k_table_reduce(TabView t, unsigned long long* __restrict__ part_u,
                                                       double* __restrict__ part_w) {
   __shared__ unsigned long long s_d[256], s_c[256];
   __shared__ double s_w[256];
   uint64_t d = 0, c = 0;
   double w = 0.0;
-  const uint64_t per = (cap + gridDim.x - 1) / gridDim.x;
-  const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = min(cap, b0 + per);
+  const uint64_t per = (t.cap + gridDim.x - 1) / gridDim.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x * per, b1 = min(t.cap, b0 + per);
   for (uint64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-    Slot s = tab[i];
-    if (s.key != EMPTY_KEY) {
+    uint32_t u, v;
+    uint64_t cnt;
+    if (tab_get(t, i, u, v, cnt)) {
       ++d;
-      c += s.cnt;
-      if (wtab) w += wtab[i];
+      c += cnt;
+      if (t.wtab) w += t.wtab[i];
     }
   }
   s_d[threadIdx.x] = d;
@@ -237,13 +359,14 @@ __global__ __launch_bounds__(256) void k_table_reduce(const Slot* __restrict__ t
 }
 
 // compact the table into (key, count[, weight]) arrays for the counts API
-__global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ tab, const double* __restrict__ wtab,
-                                                       uint64_t cap, uint64_t* __restrict__ keys,
+__global__ __launch_bounds__(256) void k_table_compact(TabView t, uint64_t* __restrict__ keys,
                                                        uint32_t* __restrict__ idx,
                                                        unsigned long long* __restrict__ counter) {
-  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < cap; i0 += (uint64_t)gridDim.x * blockDim.x) {
+  for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < t.cap; i0 += (uint64_t)gridDim.x * blockDim.x) {
   const uint64_t i = i0 + threadIdx.x;
-  const bool has = i < cap && tab[i].key != EMPTY_KEY;
+  uint32_t u = 0, v = 0;
+  uint64_t cnt = 0;
+  const bool has = i < t.cap && tab_get(t, i, u, v, cnt);
   const uint64_t bal = __ballot(has);
   const int lane = threadIdx.x & 63;
   unsigned long long base = 0;
@@ -251,26 +374,26 @@ __global__ __launch_bounds__(256) void k_table_compact(const Slot* __restrict__ 
   base = __shfl(base, 0, 64);
   if (has) {
     const uint64_t pos = base + __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-    keys[pos] = tab[i].key;
+    keys[pos] = ((uint64_t)u << 32) + v;
     idx[pos] = (uint32_t)i;  // slot index (cap <= 2^32 enforced by the host)
   }
   }
 }
 
-__global__ void k_gather_counts(const Slot* __restrict__ tab, const double* __restrict__ wtab,
-                                const uint32_t* __restrict__ idx, int64_t nk, uint32_t* __restrict__ cnt,
+__global__ void k_gather_counts(TabView t, const uint32_t* __restrict__ idx, int64_t nk, uint32_t* __restrict__ cnt,
                                 double* __restrict__ w) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t s = idx[i];
-    cnt[i] = tab[s].cnt;
-    if (w) w[i] = wtab ? wtab[s] : (double)tab[s].cnt;
+    uint32_t u, v;
+    uint64_t c = 0;
+    tab_get(t, s, u, v, c);
+    cnt[i] = (uint32_t)c;
+    if (w) w[i] = t.wtab ? t.wtab[s] : (double)c;
   }
 }
 
 struct TruncArgs {
-  const Slot* tab;
-  const double* wtab;
-  uint64_t cap;
+  TabView t;
   const int64_t* off;
   double factor;
   int upper;
@@ -281,14 +404,20 @@ struct TruncArgs {
 
 // PathEmbed.h:203-217 per slot: returns the number of CSR entries it emits
 // (0, 1 or 2) and the row / column / float value.
-__device__ __forceinline__ int trunc_slot(const TruncArgs& a, uint64_t i, uint32_t& u, uint32_t& v, double& adj) {
-  const Slot s = a.tab[i];
-  if (s.key == EMPTY_KEY) return 0;
-  u = (uint32_t)(s.key >> 32);
-  v = (uint32_t)s.key;
+__device__ __forceinline__ double trunc_adj(const TruncArgs& a, uint64_t i, uint32_t u, uint32_t v, uint64_t cnt) {
   const double du = (double)(a.off[u + 1] - a.off[u]);
   const double dv = (double)(a.off[v + 1] - a.off[v]);
-  const double val = a.wtab ? a.wtab[i] : (double)s.cnt;
+  const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
+  const double wgh = (u != v) ? val : val * 2;
+  return wgh * a.factor / du / dv;
+}
+
+__device__ __forceinline__ int trunc_slot(const TruncArgs& a, uint64_t i, uint32_t& u, uint32_t& v, double& adj) {
+  uint64_t cnt;
+  if (!tab_get(a.t, i, u, v, cnt)) return 0;
+  const double du = (double)(a.off[u + 1] - a.off[u]);
+  const double dv = (double)(a.off[v + 1] - a.off[v]);
+  const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
   const double wgh = (u != v) ? val : val * 2;
   adj = wgh * a.factor / du / dv;
   // log(adj) > 0  <=>  adj > 1 for any faithful log
@@ -315,7 +444,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
                                                             uint32_t* __restrict__ blk_emit,
                                                             uint32_t* __restrict__ blk_kept) {
   __shared__ uint32_t s_w[TR_THREADS / 64];
-  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.cap, b0 + per_block);
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
   uint32_t e = 0, kk = 0;
   for (uint64_t i = b0 + threadIdx.x; i < b1; i += TR_THREADS) {
     uint32_t u, v;
@@ -336,7 +465,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_t per_block,
                                                             const uint64_t* __restrict__ blk_off) {
   __shared__ uint32_t s_w[TR_THREADS / 64];
-  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.cap, b0 + per_block);
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint64_t run = blk_off[blockIdx.x];
   for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS) {
@@ -373,16 +502,89 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_
   }
 }
 
+// Row histogram of the emitted entries (HIST_BUCKETS ranges of 2^shift rows)
+// and the kept-key count, for cutting the CSR build into row ranges.
+constexpr int HIST_BUCKETS = 4096;
+
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_hist(TruncArgs a, int shift,
+                                                           unsigned long long* __restrict__ hist) {
+  __shared__ uint32_t s_h[HIST_BUCKETS];
+  for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * TR_THREADS + threadIdx.x; i < a.t.cap;
+       i += (uint64_t)gridDim.x * TR_THREADS) {
+    uint32_t u, v;
+    double adj;
+    const int c = trunc_slot(a, i, u, v, adj);
+    if (c >= 1) atomicAdd(&s_h[u >> shift], 1u);
+    if (c == 2) atomicAdd(&s_h[v >> shift], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS)
+    if (s_h[b]) atomicAdd(&hist[b], (unsigned long long)s_h[b]);
+}
+
+// Emits the entries whose row lies in [r0, r1) as sort keys
+// ((row - r0) << colbits | col) and float values, in any order (the keys are
+// unique; the sort that follows fixes the order).  Slots with neither endpoint
+// in the range are skipped before the degree reads.
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_part(TruncArgs a, uint32_t r0, uint32_t r1,
+                                                           unsigned long long* __restrict__ cursor) {
+  const int lane = threadIdx.x & 63;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * TR_THREADS; i0 < a.t.cap; i0 += (uint64_t)gridDim.x * TR_THREADS) {
+    const uint64_t i = i0 + threadIdx.x;
+    uint32_t u = 0, v = 0;
+    uint64_t cnt = 0;
+    int c = 0;
+    bool eu = false, ev = false;
+    double adj = 0.0;
+    if (i < a.t.cap && tab_get(a.t, i, u, v, cnt)) {
+      eu = u >= r0 && u < r1;
+      ev = !a.upper && u != v && v >= r0 && v < r1;
+      if (eu || ev) {
+        adj = trunc_adj(a, i, u, v, cnt);
+        if (adj > 1.0) c = (int)eu + (int)ev;  // log(adj) > 0 (PathEmbed.h:211-214)
+      }
+    }
+    int incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += t;
+    }
+    const int tot = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 63 && tot) base = atomicAdd(cursor, (unsigned long long)tot);
+    base = __shfl(base, 63, 64);
+    if (c) {
+      const float lw = (float)log(adj);
+      uint64_t pos = base + (uint64_t)(incl - c);
+      if (eu) {
+        a.skey[pos] = ((uint64_t)(u - r0) << a.colbits) | v;
+        a.sval[pos] = lw;
+        ++pos;
+      }
+      if (ev) {
+        a.skey[pos] = ((uint64_t)(v - r0) << a.colbits) | u;
+        a.sval[pos] = lw;
+      }
+    }
+  }
+}
+
 // row_ptr by boundary detection over the sorted keys; col extraction
 // vin / vout: the values when the sort left them elsewhere (else NULL)
+// base: offset of these entries in the CSR (a row range's slice): row_ptr
+// values are base + i
 __global__ void k_csr_finish(const uint64_t* __restrict__ skey, int64_t nnz, int64_t n, int colbits,
                              int64_t* __restrict__ row_ptr, int32_t* __restrict__ col,
-                             const float* __restrict__ vin = nullptr, float* __restrict__ vout = nullptr) {
+                             const float* __restrict__ vin = nullptr, float* __restrict__ vout = nullptr,
+                             int64_t base = 0) {
   const uint64_t cmask = (colbits >= 64) ? ~0ull : ((1ull << colbits) - 1);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nnz; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
     const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
-    for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = i;
+    for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = base + i;
     if (i < nnz) {
       col[i] = (int32_t)(skey[i] & cmask);
       if (vin) vout[i] = vin[i];
@@ -394,13 +596,6 @@ int bits_for(uint64_t n) {
   int b = 0;
   while (b < 63 && ((uint64_t)1 << b) < n) ++b;
   return b;
-}
-
-// smallest power of two >= x, saturating at 2^63
-uint64_t next_pow2(uint64_t x) {
-  uint64_t c = 1;
-  while (c < x && c < (1ull << 63)) c <<= 1;
-  return c;
 }
 
 int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
@@ -418,10 +613,16 @@ int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
 // Sampler + table.  On success the table (slots, optional weights) occupies
 // the start of the context workspace; stats are filled.
 int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
-                lightne_sparsify_stats* st, Slot** tab_out, double** wtab_out, uint64_t* cap_out) {
+                lightne_sparsify_stats* st, TabView* tv) {
   LN_CHECK(check_graph(ctx, g));
   if (!p || p->walk_len == 0 || p->negative == 0)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "walk_len and negative must be >= 1");
+  if (p->table_layout < 0 || p->table_layout > 2 || p->csr_parts < 0)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "table_layout must be 0, 1 or 2 and csr_parts >= 0");
+  const bool fits_packed = (uint64_t)g->n <= (1ull << PK_VB);
+  if (p->table_layout == 2 && !fits_packed)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "the packed table layout needs n <= 2^26");
+  const bool packed = p->table_layout == 2 || (p->table_layout == 0 && fits_packed);
   LN_HIP(ctx, hipSetDevice(ctx->device));
   *st = lightne_sparsify_stats{};
   LN_CHECK(lightne_sample_scalars(g->n, g->m, p, st));
@@ -431,39 +632,50 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   const double m = (double)(uint64_t)g->m;
   const double fr = st->fractional_bits > 0.0000000001 ? st->fractional_bits : 0.0;
   const double bound = m * (double)st->exp_rv_rounded + m * fr + 8.0 * std::sqrt(m * fr * (1.0 - fr)) + 64.0;
-  // capacity: load factor <= 0.7 even if every sample is a distinct key (the
-  // reference sizes by -tablesz / -mem_ratio, 2^ceil(log2(2 table_size)), and
-  // exits when that is too small, par_table.h:104-107)
-  const double want = bound / 0.7 + 1.0;
-  uint64_t cap = want > 1.0e12 ? (1ull << 41) : next_pow2(std::max<uint64_t>((uint64_t)want, 1024));
-  if (p->max_table_bytes) {
-    const uint64_t slot_bytes = sizeof(Slot) + (p->sample ? sizeof(double) : 0);
-    uint64_t lim = 1024;
-    while (lim < (1ull << 40) && lim * 2 * slot_bytes <= p->max_table_bytes) lim *= 2;
-    if (cap > lim) cap = lim;
-  }
+  // capacity: load factor <= 0.7 (wide) / 0.8 (packed) even if every sample
+  // is a distinct key (the reference sizes by -tablesz / -mem_ratio,
+  // 2^ceil(log2(2 table_size)), and exits when that is too small,
+  // par_table.h:104-107)
+  const double want = bound / (packed ? 0.8 : 0.7) + 1.0;
+  uint64_t cap = want > 1.0e12 ? (1ull << 41) : std::max<uint64_t>(((uint64_t)want + 255) & ~255ull, 1024);
+  const uint64_t slot_bytes = (packed ? 8 : sizeof(Slot)) + (p->sample ? sizeof(double) : 0);
+  if (p->max_table_bytes && cap > p->max_table_bytes / slot_bytes)
+    cap = std::max<uint64_t>((p->max_table_bytes / slot_bytes) & ~255ull, 1024);
   if (cap > (1ull << 40))
     return ctx_fail(ctx, LIGHTNE_ERR_OOM, "sampling budget of %.3g samples needs a table of more than 2^40 slots",
                     bound);
   st->table_capacity = cap;
+  st->table_layout = packed ? 2 : 1;
 
   Slot* tab = nullptr;
+  unsigned long long* pk = nullptr;
   double* wtab = nullptr;
   // workspace front: the table, then each edge's source vertex (dead after
   // sampling)
   uint32_t* src = nullptr;
   WsCarve wc;
-  wc.add((void**)&tab, cap * sizeof(Slot));
+  if (packed)
+    wc.add((void**)&pk, cap * 8);
+  else
+    wc.add((void**)&tab, cap * sizeof(Slot));
   if (p->sample) wc.add((void**)&wtab, cap * sizeof(double));
   wc.add((void**)&src, std::max<int64_t>(g->m, 1) * sizeof(uint32_t));
   LN_CHECK(wc.commit(ctx));
   auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
-  if (!stats) return LIGHTNE_ERR_OOM;
+  auto* ovf = packed ? (unsigned long long*)ctx_buf(ctx, "sample_ovf", OVF_CAP * 16) : nullptr;
+  if (!stats || (packed && !ovf)) return LIGHTNE_ERR_OOM;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-  hipLaunchKernelGGL(k_fill_slots, dim3(4096), dim3(256), 0, s, tab, cap);
-  LN_LAUNCH(ctx);
+  if (packed) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(4096), dim3(256), 0, s, pk, cap);
+    LN_LAUNCH(ctx);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s, ovf, 2 * OVF_CAP);
+    LN_LAUNCH(ctx);
+  } else {
+    hipLaunchKernelGGL(k_fill_slots, dim3(4096), dim3(256), 0, s, tab, cap);
+    LN_LAUNCH(ctx);
+  }
   if (wtab) LN_HIP(ctx, hipMemsetAsync(wtab, 0, cap * sizeof(double), s));
   LN_HIP(ctx, hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
   if (g->m > 0) {
@@ -482,14 +694,23 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     a.frac = st->fractional_bits;
     a.logn = st->logn;
     a.tab = tab;
+    a.pk = pk;
+    a.ovf = ovf;
     a.wtab = wtab;
-    a.mask = cap - 1;
+    a.cap = cap;
     a.stats = stats;
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
-    if (p->sample)
-      hipLaunchKernelGGL(k_sample<1>, dim3(nb), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL(k_sample<0>, dim3(nb), dim3(256), 0, s, a);
+    if (p->sample) {
+      if (packed)
+        hipLaunchKernelGGL((k_sample<1, true>), dim3(nb), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_sample<1, false>), dim3(nb), dim3(256), 0, s, a);
+    } else {
+      if (packed)
+        hipLaunchKernelGGL((k_sample<0, true>), dim3(nb), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_sample<0, false>), dim3(nb), dim3(256), 0, s, a);
+    }
     LN_LAUNCH(ctx);
   } else {
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
@@ -507,19 +728,20 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     return ctx_fail(ctx, LIGHTNE_ERR_TABLE_FULL,
                     "hash table full (%llu slots); raise max_table_bytes (the reference exits here, par_table.h:104-107)",
                     (unsigned long long)cap);
-  *tab_out = tab;
-  *wtab_out = wtab;
-  *cap_out = cap;
+  if (hs[5])
+    return ctx_fail(ctx, LIGHTNE_ERR_TABLE_FULL,
+                    "more than %llu keys were sampled over 1023 times; use the wide table layout",
+                    (unsigned long long)OVF_CAP);
+  *tv = TabView{tab, pk, ovf, wtab, cap};
   return LIGHTNE_OK;
 }
 
-int table_totals(lightne_ctx* ctx, const Slot* tab, const double* wtab, uint64_t cap, uint64_t* distinct,
-                 uint64_t* cnt_sum, double* w_sum) {
+int table_totals(lightne_ctx* ctx, const TabView& tv, uint64_t* distinct, uint64_t* cnt_sum, double* w_sum) {
   const int nb = 2048;
   auto* pu = (unsigned long long*)ctx_buf(ctx, "tred_u", 2 * nb * sizeof(unsigned long long));
   auto* pw = (double*)ctx_buf(ctx, "tred_w", nb * sizeof(double));
   if (!pu || !pw) return LIGHTNE_ERR_OOM;
-  hipLaunchKernelGGL(k_table_reduce, dim3(nb), dim3(256), 0, ctx->stream, tab, wtab, cap, pu, pw);
+  hipLaunchKernelGGL(k_table_reduce, dim3(nb), dim3(256), 0, ctx->stream, tv, pu, pw);
   LN_LAUNCH(ctx);
   std::vector<unsigned long long> hu(2 * nb);
   std::vector<double> hw(nb);
@@ -572,6 +794,14 @@ __global__ void k_up_emit(const int64_t* __restrict__ rp, const int32_t* __restr
 
 }  // namespace
 
+// Bytes this context holds in its cached buffers (reused or reclaimable by
+// the call in progress).
+static size_t ctx_held_bytes(lightne_ctx* ctx) {
+  size_t b = 0;
+  for (auto& kv : ctx->bufs) b += kv.second.bytes;
+  return b;
+}
+
 // Sampler, table, trunc-log filter and CSR.  ctx_owned: the CSR arrays are
 // context buffers reused by the next call (the pipeline's own use); else the
 // handle owns them.
@@ -579,110 +809,268 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
                   lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve) {
   *out = nullptr;
   lightne_sparsify_stats st{};
-  Slot* tab;
-  double* wtab;
-  uint64_t cap;
-  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  TabView tv;
+  LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
   hipStream_t s = ctx->stream;
-  LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
-  uint64_t distinct, csum;
-  double wsum;
-  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
-  st.distinct_keys = distinct;
-  // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
-  st.M_exact = p->sample ? wsum : (double)csum;
-  const double vol = (double)(uint64_t)g->m;
-  st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
-  LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
-
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
-  TruncArgs ta;
-  ta.tab = tab;
-  ta.wtab = wtab;
-  ta.cap = cap;
-  ta.off = g->offsets;
-  ta.factor = st.factor;
-  ta.upper = p->upper;
-  ta.colbits = colbits;
-  // contiguous slot ranges, one per block; counts first, then a scan, then the
-  // entries in slot order (deterministic, no contended atomics)
   const int nblk = 4096;
-  const uint64_t per_block = (cap + nblk - 1) / nblk;
   auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
   auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
   auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
   if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
-  hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
-  LN_LAUNCH(ctx);
-  LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
-  std::vector<uint32_t> hk(nblk);
-  uint64_t htot = 0;
-  LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  const int64_t nnz = (int64_t)htot;
-  st.kept_upper = 0;
-  for (uint32_t v : hk) st.kept_upper += v;
-  st.nnz = (uint64_t)nnz;
+  TruncArgs ta;
+  int64_t nnz = 0;
+  // totals (PathEmbed.h:175-200), then per-block counts of the kept entries
+  // over contiguous slot ranges and their scan; repeated after a workspace
+  // regrowth re-ran the (deterministic) sampler
+  auto count_pass = [&]() -> int {
+    LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
+    uint64_t distinct, csum;
+    double wsum;
+    LN_CHECK(table_totals(ctx, tv, &distinct, &csum, &wsum));
+    st.distinct_keys = distinct;
+    // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
+    st.M_exact = p->sample ? wsum : (double)csum;
+    const double vol = (double)(uint64_t)g->m;
+    st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
+    LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    ta = TruncArgs{};
+    ta.t = tv;
+    ta.off = g->offsets;
+    ta.factor = st.factor;
+    ta.upper = p->upper;
+    ta.colbits = colbits;
+    const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+    hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
+    LN_LAUNCH(ctx);
+    LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+    std::vector<uint32_t> hk(nblk);
+    uint64_t htot = 0;
+    LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+    nnz = (int64_t)htot;
+    st.kept_upper = 0;
+    for (uint32_t v : hk) st.kept_upper += v;
+    st.nnz = (uint64_t)nnz;
+    return LIGHTNE_OK;
+  };
+  LN_CHECK(count_pass());
   const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
-  // Workspace layout when it is large enough: the table and then the sort
-  // keys at the front, the CSR at the back, clear of the front region the
-  // next stage carves (front_reserve), so that a repeated pipeline holds one
-  // allocation.  Otherwise the keys (and a context-owned CSR) are buffers of
-  // their own.
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  const size_t ws_bytes = ctx->bufs["ws"].bytes;
-  const size_t tab_bytes = al(cap * sizeof(Slot)) + (wtab ? al(cap * sizeof(double)) : 0);
+  const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
   const size_t csr_bytes = al((g->n + 1) * 8) + 2 * al(nz * 4);
-  const size_t front = std::max(tab_bytes + al(nz * 8), front_reserve);
-  const bool in_ws = ctx_owned && front + csr_bytes <= ws_bytes;
-  char* ws = (char*)tab;  // the table starts the workspace
-  uint64_t* skey = in_ws ? (uint64_t*)(ws + tab_bytes) : (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
-  if (!skey) return LIGHTNE_ERR_OOM;
+  // Memory: one global sort needs the table, 8-byte keys, the CSR and (unless
+  // they fit in the dead table) 12 bytes of ping-pong buffers per entry.  When
+  // that exceeds what is free (plus what this context already holds), the CSR
+  // is built by row ranges with sort buffers for one range at a time.
+  size_t free_b = 0, total_b = 0;
+  LN_HIP(ctx, hipMemGetInfo(&free_b, &total_b));
+  const size_t margin = ((size_t)3 << 30) + total_b / 100;
+  const size_t avail = free_b + ctx_held_bytes(ctx) > margin ? free_b + ctx_held_bytes(ctx) - margin : 0;
+  const size_t sort_tmp = (al(nz * 8) + nz * 4 <= tab_bytes) ? 0 : al(nz * 8) + al(nz * 4);
+  const size_t single_need = std::max(tab_bytes + al(nz * 8), front_reserve) + csr_bytes + sort_tmp;
+  const bool parts = p->csr_parts >= 2 || (nnz > 0 && single_need > avail);
+  st.csr_parts = 1;
+
   DevMem mrp, mcol, mval;
   int64_t* rp = nullptr;
   int32_t* col = nullptr;
   float* val = nullptr;
-  if (in_ws) {
-    char* back = ws + (ws_bytes - csr_bytes);
-    rp = (int64_t*)back;
-    col = (int32_t*)(back + al((g->n + 1) * 8));
-    val = (float*)(back + al((g->n + 1) * 8) + al(nz * 4));
-  } else if (ctx_owned) {
-    rp = (int64_t*)ctx_buf(ctx, "s1_rp", (g->n + 1) * 8);
-    col = (int32_t*)ctx_buf(ctx, "s1_col", nz * 4);
-    val = (float*)ctx_buf(ctx, "s1_val", nz * 4);
-    if (!rp || !col || !val) return LIGHTNE_ERR_OOM;
+  if (!parts) {
+    // Workspace layout when it is large enough: the table and then the sort
+    // keys at the front, the CSR at the back, clear of the front region the
+    // next stage carves (front_reserve), so that a repeated pipeline holds one
+    // allocation.  Otherwise the keys (and a context-owned CSR) are buffers of
+    // their own.
+    const size_t ws_bytes = ctx->bufs["ws"].bytes;
+    const size_t front = std::max(tab_bytes + al(nz * 8), front_reserve);
+    const bool in_ws = ctx_owned && front + csr_bytes <= ws_bytes;
+    char* ws = (char*)ctx->bufs["ws"].p;  // the table starts the workspace
+    uint64_t* skey = in_ws ? (uint64_t*)(ws + tab_bytes) : (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
+    if (!skey) return LIGHTNE_ERR_OOM;
+    if (in_ws) {
+      char* back = ws + (ws_bytes - csr_bytes);
+      rp = (int64_t*)back;
+      col = (int32_t*)(back + al((g->n + 1) * 8));
+      val = (float*)(back + al((g->n + 1) * 8) + al(nz * 4));
+    } else if (ctx_owned) {
+      rp = (int64_t*)ctx_buf(ctx, "s1_rp", (g->n + 1) * 8);
+      col = (int32_t*)ctx_buf(ctx, "s1_col", nz * 4);
+      val = (float*)ctx_buf(ctx, "s1_val", nz * 4);
+      if (!rp || !col || !val) return LIGHTNE_ERR_OOM;
+    } else {
+      LN_CHECK(dev_alloc(ctx, mval, nz * 4, "sparsifier values"));
+      LN_CHECK(dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr"));
+      LN_CHECK(dev_alloc(ctx, mcol, nz * 4, "csr col"));
+      rp = (int64_t*)mrp.p;
+      col = (int32_t*)mcol.p;
+      val = (float*)mval.p;
+    }
+    ta.skey = skey;
+    ta.sval = val;
+    const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+    hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
+    LN_LAUNCH(ctx);
+    // the table is dead once the kept entries are out: the sort's ping-pong
+    // buffers reuse its memory when they fit there
+    const size_t kbytes = al(nz * 8);
+    uint64_t* k_tmp = nullptr;
+    float* v_tmp = nullptr;
+    if (kbytes + nz * 4 <= tab_bytes) {
+      k_tmp = (uint64_t*)ws;
+      v_tmp = (float*)(ws + kbytes);
+    }
+    // rows sorted by column: a key's table slot depends on the order concurrent
+    // inserts claimed slots, so the column order is what makes the matrix (and
+    // every summation over a row) deterministic
+    uint64_t* kres = nullptr;
+    float* vres = nullptr;
+    LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
+    hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, nnz,
+                       g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val, (int64_t)0);
+    LN_LAUNCH(ctx);
   } else {
-    LN_CHECK(dev_alloc(ctx, mval, nz * 4, "sparsifier values"));
-    LN_CHECK(dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr"));
-    LN_CHECK(dev_alloc(ctx, mcol, nz * 4, "csr col"));
-    rp = (int64_t*)mrp.p;
-    col = (int32_t*)mcol.p;
-    val = (float*)mval.p;
+    // row histogram of the emitted entries
+    const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
+    const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
+    auto* hist = (unsigned long long*)ctx_buf(ctx, "tr_hist", HIST_BUCKETS * 8);
+    auto* cursor = (unsigned long long*)ctx_buf(ctx, "tr_cursor", 8);
+    if (!hist || !cursor) return LIGHTNE_ERR_OOM;
+    LN_HIP(ctx, hipMemsetAsync(hist, 0, HIST_BUCKETS * 8, s));
+    hipLaunchKernelGGL(k_trunc_hist, dim3(grid_blocks((int64_t)tv.cap, TR_THREADS, 8192)), dim3(TR_THREADS), 0, s,
+                       ta, shift, hist);
+    LN_LAUNCH(ctx);
+    std::vector<unsigned long long> hh(HIST_BUCKETS);
+    LN_HIP(ctx, hipMemcpyAsync(hh.data(), hist, HIST_BUCKETS * 8, hipMemcpyDeviceToHost, s));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+    uint64_t hsum = 0, hmax = 0;
+    for (int b = 0; b < nbuck; ++b) {
+      hsum += hh[b];
+      hmax = std::max<uint64_t>(hmax, hh[b]);
+    }
+    if (hsum != (uint64_t)nnz)
+      return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row histogram holds %llu entries, the count pass %lld",
+                      (unsigned long long)hsum, (long long)nnz);
+    // entries per range: forced count, or what the memory left beside the
+    // table and the CSR allows (24 bytes per entry: keys, values, ping-pong).
+    // Context-owned, the workspace is rebuilt as [table | ranges | CSR];
+    // otherwise the range buffers use the workspace's room behind the table
+    // or memory of their own.
+    const size_t ws_now = ctx->bufs["ws"].bytes;
+    size_t part_room;
+    if (ctx_owned)
+      part_room = avail > tab_bytes + csr_bytes ? avail - tab_bytes - csr_bytes : 0;
+    else
+      part_room = std::max(ws_now - tab_bytes, avail > ws_now + csr_bytes ? avail - ws_now - csr_bytes : 0);
+    uint64_t cap_e;
+    if (p->csr_parts >= 2) {
+      cap_e = ((uint64_t)nnz + p->csr_parts - 1) / p->csr_parts;
+    } else {
+      cap_e = part_room > 4096 ? (part_room - 4096) / 24 : 0;
+      cap_e = std::min<uint64_t>(cap_e, ((uint64_t)nnz + 3) / 4);  // at least 4 ranges: memory for later stages
+    }
+    cap_e = std::max<uint64_t>(cap_e, hmax);
+    cap_e = std::max<uint64_t>(cap_e, 1);
+    if (p->csr_parts < 2 && 24 * cap_e + 4096 > part_room)
+      return ctx_fail(ctx, LIGHTNE_ERR_OOM,
+                      "sparsifier of %lld entries: %.1f GB of table and %.1f GB of CSR leave %.1f GB for sort "
+                      "buffers, the densest row range needs %.1f GB",
+                      (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
+    // ranges of whole buckets, at most 2^22 rows each (6 radix passes at n = 2^26)
+    std::vector<std::pair<uint64_t, uint64_t>> rng;  // row ranges
+    std::vector<uint64_t> rcnt;
+    {
+      const uint64_t max_rows = 1ull << 22;
+      uint64_t r0 = 0, acc = 0;
+      for (int b = 0; b < nbuck; ++b) {
+        const uint64_t b0 = (uint64_t)b << shift;
+        const bool span = b0 - r0 + (1ull << shift) > max_rows;
+        if (b0 > r0 && (acc + hh[b] > cap_e || span)) {
+          rng.push_back({r0, b0});
+          rcnt.push_back(acc);
+          r0 = b0;
+          acc = 0;
+        }
+        acc += hh[b];
+      }
+      rng.push_back({r0, (uint64_t)g->n});
+      rcnt.push_back(acc);
+    }
+    st.csr_parts = (uint32_t)rng.size();
+    uint64_t pmax = 0;
+    for (uint64_t c : rcnt) pmax = std::max(pmax, c);
+    const size_t pe = (size_t)std::max<uint64_t>(pmax, 1);
+    const size_t part_bytes = 2 * al(pe * 8) + 2 * al(pe * 4);
+    // Context-owned: [table | range buffers | ... | CSR] in the workspace,
+    // which grows (and the deterministic sampler runs again) when it is too
+    // small.  Otherwise the range buffers follow the table when they fit and
+    // the CSR arrays belong to the handle.
+    char* pbuf = nullptr;
+    if (ctx_owned) {
+      const size_t need = std::max(tab_bytes + part_bytes, front_reserve) + csr_bytes;
+      if (ctx->bufs["ws"].bytes < need) {
+        if (p->csr_parts < 2 && need > avail)
+          return ctx_fail(ctx, LIGHTNE_ERR_OOM, "stage 1 needs a %.1f GB workspace, %.1f GB are available",
+                          need / 1e9, avail / 1e9);
+        if (!ctx_buf(ctx, "ws", need)) return LIGHTNE_ERR_OOM;
+        const lightne_sparsify_stats keep = st;
+        LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
+        LN_CHECK(count_pass());
+        if (st.nnz != keep.nnz || st.distinct_keys != keep.distinct_keys)
+          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different table on its second run");
+        st.csr_parts = keep.csr_parts;
+      }
+      char* ws = (char*)ctx->bufs["ws"].p;
+      const size_t ws_bytes = ctx->bufs["ws"].bytes;
+      pbuf = ws + tab_bytes;
+      char* back = ws + (ws_bytes - csr_bytes);
+      rp = (int64_t*)back;
+      col = (int32_t*)(back + al((g->n + 1) * 8));
+      val = (float*)(back + al((g->n + 1) * 8) + al(nz * 4));
+    } else {
+      const size_t ws_bytes = ctx->bufs["ws"].bytes;
+      pbuf = tab_bytes + part_bytes <= ws_bytes ? (char*)ctx->bufs["ws"].p + tab_bytes
+                                                : (char*)ctx_buf(ctx, "s1_part", part_bytes);
+      if (!pbuf) return LIGHTNE_ERR_OOM;
+      LN_CHECK(dev_alloc(ctx, mval, nz * 4, "sparsifier values"));
+      LN_CHECK(dev_alloc(ctx, mrp, (g->n + 1) * 8, "csr row_ptr"));
+      LN_CHECK(dev_alloc(ctx, mcol, nz * 4, "csr col"));
+      rp = (int64_t*)mrp.p;
+      col = (int32_t*)mcol.p;
+      val = (float*)mval.p;
+    }
+    auto* pkey = (uint64_t*)pbuf;
+    auto* ktmp = (uint64_t*)(pbuf + al(pe * 8));
+    auto* pval = (float*)(pbuf + 2 * al(pe * 8));
+    auto* vtmp = (float*)(pbuf + 2 * al(pe * 8) + al(pe * 4));
+    ta.skey = pkey;
+    ta.sval = pval;
+    int64_t base = 0;
+    for (size_t r = 0; r < rng.size(); ++r) {
+      const uint64_t r0 = rng[r].first, r1 = rng[r].second;
+      const int64_t cnt = (int64_t)rcnt[r];
+      uint64_t* kres = pkey;
+      float* vres = pval;
+      if (cnt > 0) {
+        LN_HIP(ctx, hipMemsetAsync(cursor, 0, 8, s));
+        hipLaunchKernelGGL(k_trunc_part, dim3(grid_blocks((int64_t)tv.cap, TR_THREADS)), dim3(TR_THREADS), 0, s, ta,
+                           (uint32_t)r0, (uint32_t)r1, cursor);
+        LN_LAUNCH(ctx);
+        unsigned long long got = 0;
+        LN_HIP(ctx, hipMemcpyAsync(&got, cursor, 8, hipMemcpyDeviceToHost, s));
+        LN_HIP(ctx, hipStreamSynchronize(s));
+        if ((int64_t)got != cnt)
+          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu emitted %llu entries, the histogram %lld",
+                          (unsigned long long)r0, (unsigned long long)r1, got, (long long)cnt);
+        LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + colbits, ktmp, vtmp, 0, &kres, &vres));
+      }
+      hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(cnt + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, cnt,
+                         (int64_t)(r1 - r0), colbits, rp + r0, col + base, (const float*)vres, val + base, base);
+      LN_LAUNCH(ctx);
+      base += cnt;
+    }
   }
-  ta.skey = skey;
-  ta.sval = val;
-  hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
-  LN_LAUNCH(ctx);
-  // the table is dead once the kept entries are out: the sort's ping-pong
-  // buffers reuse its memory when they fit there
-  const size_t kbytes = al(nz * 8);
-  uint64_t* k_tmp = nullptr;
-  float* v_tmp = nullptr;
-  if (kbytes + nz * 4 <= tab_bytes) {
-    k_tmp = (uint64_t*)tab;
-    v_tmp = (float*)((char*)tab + kbytes);
-  }
-  // rows sorted by column: a key's table slot depends on the order concurrent
-  // inserts claimed slots, so the column order is what makes the matrix (and
-  // every summation over a row) deterministic
-  uint64_t* kres = nullptr;
-  float* vres = nullptr;
-  LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
-  hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, nnz,
-                     g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val);
-  LN_LAUNCH(ctx);
   LN_HIP(ctx, hipEventRecord(ctx->ev[5], s));
   LN_HIP(ctx, hipStreamSynchronize(s));
   auto* csr = new lightne_csr();
@@ -771,17 +1159,16 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   CallScope scope(ctx);
   *out = nullptr;
   lightne_sparsify_stats st{};
-  Slot* tab;
-  double* wtab;
-  uint64_t cap;
-  LN_CHECK(run_sampler(ctx, g, p, &st, &tab, &wtab, &cap));
+  TabView tv;
+  LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
+  const uint64_t cap = tv.cap;
   // the compacted table addresses slots with 32-bit indices
   if (cap > (1ull << 32))
     return ctx_fail(ctx, LIGHTNE_ERR_OOM, "lightne_sparsify_counts: table of %llu slots exceeds 2^32",
                     (unsigned long long)cap);
   uint64_t distinct, csum;
   double wsum;
-  LN_CHECK(table_totals(ctx, tab, wtab, cap, &distinct, &csum, &wsum));
+  LN_CHECK(table_totals(ctx, tv, &distinct, &csum, &wsum));
   st.distinct_keys = distinct;
   st.M_exact = p->sample ? wsum : (double)csum;
   DevMem mk, mc, mw, mi;
@@ -793,12 +1180,12 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   auto* counter = (unsigned long long*)ctx_buf(ctx, "compact_ctr", 8);
   if (!counter) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemsetAsync(counter, 0, 8, ctx->stream));
-  hipLaunchKernelGGL(k_table_compact, dim3(grid_blocks((int64_t)cap, 256)), dim3(256), 0, ctx->stream, tab, wtab, cap,
+  hipLaunchKernelGGL(k_table_compact, dim3(grid_blocks((int64_t)cap, 256)), dim3(256), 0, ctx->stream, tv,
                      (uint64_t*)mk.p, (uint32_t*)mi.p, counter);
   LN_LAUNCH(ctx);
   LN_CHECK(radix_sort_pairs_u32(ctx, (uint64_t*)mk.p, (uint32_t*)mi.p, (int64_t)nk, 32 + bits_for((uint64_t)g->n)));
   if (nk) {
-    hipLaunchKernelGGL(k_gather_counts, dim3(grid_blocks((int64_t)nk, 256)), dim3(256), 0, ctx->stream, tab, wtab,
+    hipLaunchKernelGGL(k_gather_counts, dim3(grid_blocks((int64_t)nk, 256)), dim3(256), 0, ctx->stream, tv,
                        (const uint32_t*)mi.p, (int64_t)nk, (uint32_t*)mc.p, (double*)mw.p);
     LN_LAUNCH(ctx);
   }

@@ -137,3 +137,85 @@ def test_asymmetric_graph_is_rejected(gpu):
     with pytest.raises(L.LightNEError) as e:
         L.sparsify_counts(g, 1, 10, False, sample_ratio=50.0, ctx=gpu)
     assert e.value.code == _lib.ERR_BAD_GRAPH
+
+
+def _rec(case):
+    return next(r for r in H.manifest() if r["case"] == case)
+
+
+@pytest.mark.parametrize("case", ["toy_t3_s1", "cl2000_t10_s0", "cl2000_t5_s1", "blog_t10_s0"])
+def test_table_layouts_agree(gpu, case):
+    """8-byte and 16-byte slots hold the same key -> (count, weight) multiset."""
+    rec = _rec(case)
+    off, edg = H.golden_graph(rec["graph"])
+    g = dgraph(off, edg)
+    kw = dict(sample_ratio=rec["sample_ratio"], ctx=gpu)
+    tw = L.sparsify_counts(g, int(rec["seed"]), rec["walk_len"], bool(rec["sample"]), table_layout=1, **kw)
+    tp = L.sparsify_counts(g, int(rec["seed"]), rec["walk_len"], bool(rec["sample"]), table_layout=2, **kw)
+    assert tw.stats["table_layout"] == 1 and tp.stats["table_layout"] == 2
+    np.testing.assert_array_equal(tp.keys, tw.keys)
+    np.testing.assert_array_equal(tp.counts, tw.counts)
+    if rec["sample"] == 0:
+        np.testing.assert_array_equal(tp.weights, tw.weights)
+    else:  # FP64 sums of 1/p in the order the atomics landed
+        np.testing.assert_allclose(tp.weights, tw.weights, rtol=1e-12)
+
+
+def test_packed_counts_wrap_exactly(gpu):
+    """Counts far above the packed slot's 10 bits (several wraps into the side
+    table) stay exact: packed == wide == oracle."""
+    off, edg = H.golden_graph("toy")
+    g = dgraph(off, edg)
+    tp = L.sparsify_counts(g, 11, 10, False, sample_ratio=1100.0, table_layout=2, ctx=gpu)
+    tw = L.sparsify_counts(g, 11, 10, False, sample_ratio=1100.0, table_layout=1, ctx=gpu)
+    keys, cnt, _, st = O.sample(off, edg, 11, 10, 1100.0, 0, threads=8)
+    assert int(cnt.max()) > 3 * 1024
+    np.testing.assert_array_equal(tp.keys, keys)
+    np.testing.assert_array_equal(tp.counts.astype(np.uint64), cnt)
+    np.testing.assert_array_equal(tw.counts, tp.counts)
+    assert tp.stats["samples"] == st["samples"]
+
+
+@pytest.mark.parametrize("case,parts,upper", [("cl2000_t10_s0", 5, False), ("cl2000_t5_s1", 3, False),
+                                              ("blog_t10_s0", 9, False), ("blog_t10_s0", 4, True),
+                                              ("toy_t10_s0", 3, False)])
+def test_csr_by_row_ranges_matches_one_sort(gpu, case, parts, upper):
+    """Building the CSR range by range (emission in any order, one sort per
+    range) gives the matrix of the single global sort."""
+    rec = _rec(case)
+    off, edg = H.golden_graph(rec["graph"])
+    g = dgraph(off, edg)
+    kw = dict(walk_len=rec["walk_len"], sample=bool(rec["sample"]), sample_ratio=rec["sample_ratio"], upper=upper,
+              ctx=gpu)
+    a, sa = L.generate_trunc_log_matrix(g, int(rec["seed"]), **kw)
+    b, sb = L.generate_trunc_log_matrix(g, int(rec["seed"]), csr_parts=parts, **kw)
+    assert sa["csr_parts"] == 1 and sb["csr_parts"] >= 2
+    assert sa["nnz"] == sb["nnz"] == a.nnz == b.nnz
+    rb = b.to_numpy()
+    if rec["sample"] == 0:
+        for x, y in zip(a.to_numpy(), rb):
+            np.testing.assert_array_equal(x, y)
+    else:  # M_exact and the weights are FP64 sums in atomic order
+        _compare_csr(a, (*rb, None))
+    a.free()
+    b.free()
+
+
+def test_netsmf_row_ranges_in_a_fresh_context(gpu):
+    """The pipeline's context-owned build by row ranges (the workspace grows and
+    the deterministic sampler runs again) matches the single-sort pipeline,
+    on the first and on a repeated call."""
+    hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
+    g = hg.to_device()
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
+    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3)
+    e1, S1, st1 = L.NetSMF(g, ctx=gpu, **kw)
+    ctx2 = L.Context(0)
+    try:
+        for _ in range(2):
+            e2, S2, st2 = L.NetSMF(g, csr_parts=6, ctx=ctx2, **kw)
+            assert st2["sparsify"]["csr_parts"] >= 2 and st1["sparsify"]["csr_parts"] == 1
+            np.testing.assert_array_equal(S2, S1)
+            np.testing.assert_array_equal(e2.cpu().numpy(), e1.cpu().numpy())
+    finally:
+        ctx2.close()

@@ -72,6 +72,11 @@ def test_chebyshev_expansion_end_to_end(gpu):
     _, z, _ = L.propagate(torch.from_numpy(a).cuda(), g, ctx=gpu)
     emb2, _ = L.get_embedding_dense(z, hg.n, d, d, ctx=gpu)
     np.testing.assert_array_equal(emb.cpu().numpy(), emb2.cpu().numpy())
+    # in place: the output may be the input itself
+    b = torch.from_numpy(a).cuda()
+    emb3, _ = L.chebyshev_expansion(b, g, d, d, out=b, ctx=gpu)
+    assert emb3.data_ptr() == b.data_ptr()
+    np.testing.assert_array_equal(emb3.cpu().numpy(), emb.cpu().numpy())
     _, z64 = O.propagate(hg.offsets, hg.edges, a)
     ref, S64 = O.embed_dense(z64, d)
     e = H.sign_align(emb.cpu().numpy(), ref)
