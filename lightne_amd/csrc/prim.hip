@@ -123,7 +123,9 @@ int scan_impl(lightne_ctx* ctx, const T* in, uint64_t* out, int64_t n, int depth
 // ---------------------------------------------------------------------------
 // radix sort
 // ---------------------------------------------------------------------------
-constexpr int RS_THREADS = 256;
+// 8192-element tiles (512 threads x 16): digit runs of ~32 entries per tile
+// keep the scattered writes long
+constexpr int RS_THREADS = 512;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
@@ -131,7 +133,7 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
                                                         uint32_t* __restrict__ hist, int64_t ntiles) {
   __shared__ uint32_t s_h[256];
-  s_h[threadIdx.x] = 0;
+  if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
@@ -140,13 +142,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
     if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+  if (threadIdx.x < 256) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// Stable scatter of one 8-bit digit.  Every tile ranks its elements in LDS
-// (wave peers by 8 ballots, waves in order), stages them in local digit order
-// and then writes each digit run to its global offset with consecutive lanes
-// on consecutive addresses.
+// Stable scatter of one 8-bit digit.  Each wave ranks its own contiguous
+// 1024-element part of the tile (peers by 8 ballots per item, a running
+// per-digit count in LDS that only this wave touches, so no block barrier per
+// item); one block pass turns the waves' counts into LDS offsets; the tile is
+// staged in digit order and each digit run is written to its global offset
+// with consecutive lanes on consecutive addresses.
 template <typename V>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                            const V* __restrict__ vin, uint64_t* __restrict__ kout,
@@ -157,14 +161,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
   __shared__ V s_vals[RS_TILE];
   __shared__ uint64_t s_base[256];
   __shared__ uint32_t s_dstart[256];
-  __shared__ uint32_t s_run[256];
   __shared__ uint32_t s_wcnt[RS_WAVES][256];
-  __shared__ uint32_t s_woff[RS_WAVES][256];
   __shared__ uint32_t s_scan[RS_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  // tile-local start of every digit: exclusive scan of this tile's histogram
+  // tile-local start of every digit (threads 0..255, one digit each):
+  // exclusive scan of this tile's histogram
   {
-    const uint32_t c = hist[(int64_t)tid * ntiles + blockIdx.x];
+    const uint32_t c = tid < 256 ? hist[(int64_t)tid * ntiles + blockIdx.x] : 0u;
     uint32_t incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -175,18 +178,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     __syncthreads();
     uint32_t off = 0;
     for (int w = 0; w < wid; ++w) off += s_scan[w];
-    s_dstart[tid] = off + incl - c;
+    if (tid < 256) {
+      s_dstart[tid] = off + incl - c;
+      s_base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
+    }
   }
-  s_base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
-  s_run[tid] = 0;
-#pragma unroll
-  for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
+  for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   const int nvalid = (int)min<int64_t>(RS_TILE, n - base);
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t kr[RS_ITEMS];
+  V vr[RS_ITEMS];
+  uint32_t pr[RS_ITEMS];  // digit | (position among the wave's elements of that digit) << 8
+#pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    const int li = it * RS_THREADS + tid;
+    const int li = wid * (RS_ITEMS * 64) + it * 64 + lane;
     const bool valid = li < nvalid;
     const uint64_t k = valid ? kin[base + li] : 0;
     const V v = valid ? vin[base + li] : V(0);
@@ -199,23 +206,32 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
       peers &= bit ? bal : ~bal;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-    if (valid && rank == 0) s_wcnt[wid][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      uint32_t run = s_run[tid];
+    // every lane reads the wave's running count before the leader adds to it
+    // (one wave's LDS operations complete in order)
+    const uint32_t before = s_wcnt[wid][d];
+    if (valid && rank == 0) s_wcnt[wid][d] = before + (uint32_t)__popcll(peers);
+    kr[it] = k;
+    vr[it] = v;
+    pr[it] = valid ? (d | ((before + rank) << 8)) : ~0u;
+  }
+  __syncthreads();
+  // per digit: LDS start of each wave's run = digit start + earlier waves' counts
+  if (tid < 256) {
+    uint32_t run = s_dstart[tid];
 #pragma unroll
-      for (int w = 0; w < RS_WAVES; ++w) {
-        s_woff[w][tid] = run;
-        run += s_wcnt[w][tid];
-        s_wcnt[w][tid] = 0;
-      }
-      s_run[tid] = run;
+    for (int w = 0; w < RS_WAVES; ++w) {
+      const uint32_t c = s_wcnt[w][tid];
+      s_wcnt[w][tid] = run;
+      run += c;
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t lp = s_dstart[d] + s_woff[wid][d] + rank;
-      s_keys[lp] = k;
-      s_vals[lp] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    if (pr[it] != ~0u) {
+      const uint32_t lp = s_wcnt[wid][pr[it] & 255u] + (pr[it] >> 8);
+      s_keys[lp] = kr[it];
+      s_vals[lp] = vr[it];
     }
   }
   __syncthreads();

@@ -21,12 +21,14 @@
 //                    scan, then writes in slot order (no contended atomics).
 //  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
 //                    row_ptr by boundary detection (no atomics, deterministic).
-//  k_trunc_hist / k_trunc_part : when the sort buffers for every entry do not
-//                    fit beside the table and the CSR, the rows are cut into
-//                    ranges by a row histogram; each range is emitted from the
-//                    table, sorted and written into its slice of the CSR.  The
-//                    (row, col) keys are unique, so the result is the same as
-//                    one global sort.
+//  k_trunc_range : when the sort buffers for every entry do not fit beside
+//                    the table and the CSR, the rows are cut into ranges by
+//                    the count pass's per-block row histograms; each range is
+//                    emitted from the table (a scan of its per-block counts,
+//                    ordered writes), sorted and written into its slice of the
+//                    CSR.  The (row, col)
+//                    keys are unique, so the result is the same as one global
+//                    sort.
 #include "lightne_internal.h"
 
 #include <algorithm>
@@ -394,7 +396,7 @@ __global__ void k_gather_counts(TabView t, const uint32_t* __restrict__ idx, int
 
 struct TruncArgs {
   TabView t;
-  const int64_t* off;
+  const uint32_t* deg;  // vertex degrees (GA.get_vertex(u).getOutDegree())
   double factor;
   int upper;
   int colbits;
@@ -405,8 +407,8 @@ struct TruncArgs {
 // PathEmbed.h:203-217 per slot: returns the number of CSR entries it emits
 // (0, 1 or 2) and the row / column / float value.
 __device__ __forceinline__ double trunc_adj(const TruncArgs& a, uint64_t i, uint32_t u, uint32_t v, uint64_t cnt) {
-  const double du = (double)(a.off[u + 1] - a.off[u]);
-  const double dv = (double)(a.off[v + 1] - a.off[v]);
+  const double du = (double)a.deg[u];
+  const double dv = (double)a.deg[v];
   const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
   const double wgh = (u != v) ? val : val * 2;
   return wgh * a.factor / du / dv;
@@ -415,8 +417,8 @@ __device__ __forceinline__ double trunc_adj(const TruncArgs& a, uint64_t i, uint
 __device__ __forceinline__ int trunc_slot(const TruncArgs& a, uint64_t i, uint32_t& u, uint32_t& v, double& adj) {
   uint64_t cnt;
   if (!tab_get(a.t, i, u, v, cnt)) return 0;
-  const double du = (double)(a.off[u + 1] - a.off[u]);
-  const double dv = (double)(a.off[v + 1] - a.off[v]);
+  const double du = (double)a.deg[u];
+  const double dv = (double)a.deg[v];
   const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
   const double wgh = (u != v) ? val : val * 2;
   adj = wgh * a.factor / du / dv;
@@ -439,11 +441,22 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
   return t;
 }
 
-// pass 1: per-block (contiguous slot range) emitted entries and kept keys
+constexpr int HIST_BUCKETS = 4096;
+
+// pass 1: per-block (contiguous slot range) emitted entries and kept keys;
+// with HIST also the block's row histogram (HIST_BUCKETS ranges of 2^shift
+// rows), stored bucket-major: bh[bucket * gridDim.x + block]
+template <bool HIST>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_t per_block,
                                                             uint32_t* __restrict__ blk_emit,
-                                                            uint32_t* __restrict__ blk_kept) {
+                                                            uint32_t* __restrict__ blk_kept, int shift, int nbuck,
+                                                            uint32_t* __restrict__ bh) {
   __shared__ uint32_t s_w[TR_THREADS / 64];
+  __shared__ uint32_t s_h[HIST ? HIST_BUCKETS : 1];
+  if (HIST) {
+    for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
+    __syncthreads();
+  }
   const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
   uint32_t e = 0, kk = 0;
   for (uint64_t i = b0 + threadIdx.x; i < b1; i += TR_THREADS) {
@@ -452,6 +465,10 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
     const int c = trunc_slot(a, i, u, v, adj);
     e += c;
     kk += c > 0;
+    if (HIST) {
+      if (c >= 1) atomicAdd(&s_h[u >> shift], 1u);
+      if (c == 2) atomicAdd(&s_h[v >> shift], 1u);
+    }
   }
   const uint32_t te = block_sum_u32(e, s_w);
   const uint32_t tk = block_sum_u32(kk, s_w);
@@ -459,6 +476,33 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
     blk_emit[blockIdx.x] = te;
     blk_kept[blockIdx.x] = tk;
   }
+  if (HIST)
+    for (int b = threadIdx.x; b < nbuck; b += TR_THREADS) bh[(uint64_t)b * gridDim.x + blockIdx.x] = s_h[b];
+}
+
+// row histogram totals: bucket b sums its nblk block counts
+__global__ void k_bh_totals(const uint32_t* __restrict__ bh, int nbuck, int nblk,
+                            unsigned long long* __restrict__ hist) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbuck) return;
+  unsigned long long t = 0;
+  for (int k = 0; k < nblk; ++k) t += bh[(uint64_t)b * nblk + k];
+  hist[b] = t;
+}
+
+// per-block entries of the buckets [lo, hi) (a row range)
+__global__ void k_range_counts(const uint32_t* __restrict__ bh, int lo, int hi, int nblk,
+                               uint32_t* __restrict__ blk_cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  uint32_t t = 0;
+  for (int b = lo; b < hi; ++b) t += bh[(uint64_t)b * nblk + k];
+  blk_cnt[k] = t;
+}
+
+__global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* __restrict__ deg) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    deg[i] = (uint32_t)(off[i + 1] - off[i]);
 }
 
 // pass 2: same partition, entries written in slot order at the block's offset
@@ -502,63 +546,54 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_
   }
 }
 
-// Row histogram of the emitted entries (HIST_BUCKETS ranges of 2^shift rows)
-// and the kept-key count, for cutting the CSR build into row ranges.
-constexpr int HIST_BUCKETS = 4096;
-
-__global__ __launch_bounds__(TR_THREADS) void k_trunc_hist(TruncArgs a, int shift,
-                                                           unsigned long long* __restrict__ hist) {
-  __shared__ uint32_t s_h[HIST_BUCKETS];
-  for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
-  __syncthreads();
-  for (uint64_t i = (uint64_t)blockIdx.x * TR_THREADS + threadIdx.x; i < a.t.cap;
-       i += (uint64_t)gridDim.x * TR_THREADS) {
-    uint32_t u, v;
-    double adj;
-    const int c = trunc_slot(a, i, u, v, adj);
-    if (c >= 1) atomicAdd(&s_h[u >> shift], 1u);
-    if (c == 2) atomicAdd(&s_h[v >> shift], 1u);
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS)
-    if (s_h[b]) atomicAdd(&hist[b], (unsigned long long)s_h[b]);
+// Entries of slot i whose row lies in [r0, r1): 0, 1 or 2 (eu: row u, ev:
+// row v).  Slots with neither endpoint in the range skip the degree reads.
+__device__ __forceinline__ int range_slot(const TruncArgs& a, uint64_t i, uint32_t r0, uint32_t r1, uint32_t& u,
+                                          uint32_t& v, double& adj, bool& eu, bool& ev) {
+  uint64_t cnt;
+  eu = ev = false;
+  if (!tab_get(a.t, i, u, v, cnt)) return 0;
+  eu = u >= r0 && u < r1;
+  ev = !a.upper && u != v && v >= r0 && v < r1;
+  if (!eu && !ev) return 0;
+  adj = trunc_adj(a, i, u, v, cnt);
+  if (!(adj > 1.0)) return 0;  // log(adj) > 0 (PathEmbed.h:211-214)
+  return (int)eu + (int)ev;
 }
 
-// Emits the entries whose row lies in [r0, r1) as sort keys
-// ((row - r0) << colbits | col) and float values, in any order (the keys are
-// unique; the sort that follows fixes the order).  Slots with neither endpoint
-// in the range are skipped before the degree reads.
-__global__ __launch_bounds__(TR_THREADS) void k_trunc_part(TruncArgs a, uint32_t r0, uint32_t r1,
-                                                           unsigned long long* __restrict__ cursor) {
-  const int lane = threadIdx.x & 63;
-  for (uint64_t i0 = (uint64_t)blockIdx.x * TR_THREADS; i0 < a.t.cap; i0 += (uint64_t)gridDim.x * TR_THREADS) {
-    const uint64_t i = i0 + threadIdx.x;
+// One row range: the entries in slot order at each block's offset (the
+// scan of the blocks' counts of the range, from the count pass's row
+// histograms) as sort keys ((row - r0) << colbits | col) and float values.
+// No contended atomics; the emission order is deterministic.
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_range(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
+                                                            const uint64_t* __restrict__ blk_off) {
+  __shared__ uint32_t s_w[TR_THREADS / 64];
+  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint64_t run = blk_off[blockIdx.x];
+  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS) {
+    const uint64_t i = t0 + threadIdx.x;
     uint32_t u = 0, v = 0;
-    uint64_t cnt = 0;
-    int c = 0;
-    bool eu = false, ev = false;
     double adj = 0.0;
-    if (i < a.t.cap && tab_get(a.t, i, u, v, cnt)) {
-      eu = u >= r0 && u < r1;
-      ev = !a.upper && u != v && v >= r0 && v < r1;
-      if (eu || ev) {
-        adj = trunc_adj(a, i, u, v, cnt);
-        if (adj > 1.0) c = (int)eu + (int)ev;  // log(adj) > 0 (PathEmbed.h:211-214)
-      }
-    }
+    bool eu = false, ev = false;
+    const int c = (i < b1) ? range_slot(a, i, r0, r1, u, v, adj, eu, ev) : 0;
     int incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const int t = __shfl_up(incl, d, 64);
       if (lane >= d) incl += t;
     }
-    const int tot = __shfl(incl, 63, 64);
-    unsigned long long base = 0;
-    if (lane == 63 && tot) base = atomicAdd(cursor, (unsigned long long)tot);
-    base = __shfl(base, 63, 64);
+    __syncthreads();
+    if (lane == 63) s_w[wid] = (uint32_t)incl;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int w = 0; w < TR_THREADS / 64; ++w) {
+      if (w < wid) wbase += s_w[w];
+      tot += s_w[w];
+    }
     if (c) {
       const float lw = (float)log(adj);
-      uint64_t pos = base + (uint64_t)(incl - c);
+      uint64_t pos = run + wbase + (uint32_t)(incl - c);
       if (eu) {
         a.skey[pos] = ((uint64_t)(u - r0) << a.colbits) | v;
         a.sval[pos] = lw;
@@ -569,6 +604,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_part(TruncArgs a, uint32_t
         a.sval[pos] = lw;
       }
     }
+    run += tot;
   }
 }
 
@@ -821,10 +857,18 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
   TruncArgs ta;
   int64_t nnz = 0;
+  auto* deg = (uint32_t*)ctx_buf(ctx, "s1_deg", (size_t)g->n * 4);
+  if (!deg) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_degrees, dim3(grid_blocks(g->n, 256)), dim3(256), 0, s, g->offsets, g->n, deg);
+  LN_LAUNCH(ctx);
+  // row buckets for a build by row ranges: HIST_BUCKETS ranges of 2^shift rows
+  const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
+  const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
+  uint32_t* bh = nullptr;  // per-block row histograms, when taken
   // totals (PathEmbed.h:175-200), then per-block counts of the kept entries
   // over contiguous slot ranges and their scan; repeated after a workspace
   // regrowth re-ran the (deterministic) sampler
-  auto count_pass = [&]() -> int {
+  auto count_pass = [&](bool hist) -> int {
     LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     uint64_t distinct, csum;
     double wsum;
@@ -837,12 +881,20 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     ta = TruncArgs{};
     ta.t = tv;
-    ta.off = g->offsets;
+    ta.deg = deg;
     ta.factor = st.factor;
     ta.upper = p->upper;
     ta.colbits = colbits;
     const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
-    hipLaunchKernelGGL(k_trunc_count, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept);
+    if (hist) {
+      bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
+      if (!bh) return LIGHTNE_ERR_OOM;
+      hipLaunchKernelGGL(k_trunc_count<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept,
+                         shift, nbuck, bh);
+    } else {
+      hipLaunchKernelGGL(k_trunc_count<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept,
+                         0, 0, (uint32_t*)nullptr);
+    }
     LN_LAUNCH(ctx);
     LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
     std::vector<uint32_t> hk(nblk);
@@ -856,7 +908,8 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     st.nnz = (uint64_t)nnz;
     return LIGHTNE_OK;
   };
-  LN_CHECK(count_pass());
+  // tables that may need a build by row ranges take the row histograms now
+  LN_CHECK(count_pass(p->csr_parts >= 2 || tv.cap >= (1ull << 30)));
   const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
   const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
   const size_t csr_bytes = al((g->n + 1) * 8) + 2 * al(nz * 4);
@@ -931,18 +984,16 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
                        g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val, (int64_t)0);
     LN_LAUNCH(ctx);
   } else {
-    // row histogram of the emitted entries
-    const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
-    const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
+    // row histogram of the emitted entries: per block (taken by the count
+    // pass), summed per bucket
+    if (!bh) LN_CHECK(count_pass(true));
     auto* hist = (unsigned long long*)ctx_buf(ctx, "tr_hist", HIST_BUCKETS * 8);
-    auto* cursor = (unsigned long long*)ctx_buf(ctx, "tr_cursor", 8);
-    if (!hist || !cursor) return LIGHTNE_ERR_OOM;
-    LN_HIP(ctx, hipMemsetAsync(hist, 0, HIST_BUCKETS * 8, s));
-    hipLaunchKernelGGL(k_trunc_hist, dim3(grid_blocks((int64_t)tv.cap, TR_THREADS, 8192)), dim3(TR_THREADS), 0, s,
-                       ta, shift, hist);
+    if (!hist) return LIGHTNE_ERR_OOM;
+    hipLaunchKernelGGL(k_bh_totals, dim3((nbuck + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh, nbuck, nblk,
+                       hist);
     LN_LAUNCH(ctx);
     std::vector<unsigned long long> hh(HIST_BUCKETS);
-    LN_HIP(ctx, hipMemcpyAsync(hh.data(), hist, HIST_BUCKETS * 8, hipMemcpyDeviceToHost, s));
+    LN_HIP(ctx, hipMemcpyAsync(hh.data(), hist, nbuck * 8, hipMemcpyDeviceToHost, s));
     LN_HIP(ctx, hipStreamSynchronize(s));
     uint64_t hsum = 0, hmax = 0;
     for (int b = 0; b < nbuck; ++b) {
@@ -978,23 +1029,22 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
                       "buffers, the densest row range needs %.1f GB",
                       (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
     // ranges of whole buckets, at most 2^22 rows each (6 radix passes at n = 2^26)
-    std::vector<std::pair<uint64_t, uint64_t>> rng;  // row ranges
+    std::vector<std::pair<int, int>> rng;  // bucket ranges
     std::vector<uint64_t> rcnt;
     {
-      const uint64_t max_rows = 1ull << 22;
-      uint64_t r0 = 0, acc = 0;
+      const int max_buckets = std::max(1, (1 << 22) >> shift);
+      int lo = 0;
+      uint64_t acc = 0;
       for (int b = 0; b < nbuck; ++b) {
-        const uint64_t b0 = (uint64_t)b << shift;
-        const bool span = b0 - r0 + (1ull << shift) > max_rows;
-        if (b0 > r0 && (acc + hh[b] > cap_e || span)) {
-          rng.push_back({r0, b0});
+        if (b > lo && (acc + hh[b] > cap_e || b - lo + 1 > max_buckets)) {
+          rng.push_back({lo, b});
           rcnt.push_back(acc);
-          r0 = b0;
+          lo = b;
           acc = 0;
         }
         acc += hh[b];
       }
-      rng.push_back({r0, (uint64_t)g->n});
+      rng.push_back({lo, nbuck});
       rcnt.push_back(acc);
     }
     st.csr_parts = (uint32_t)rng.size();
@@ -1016,7 +1066,7 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
         if (!ctx_buf(ctx, "ws", need)) return LIGHTNE_ERR_OOM;
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
-        LN_CHECK(count_pass());
+        LN_CHECK(count_pass(true));
         if (st.nnz != keep.nnz || st.distinct_keys != keep.distinct_keys)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different table on its second run");
         st.csr_parts = keep.csr_parts;
@@ -1048,21 +1098,26 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     ta.sval = pval;
     int64_t base = 0;
     for (size_t r = 0; r < rng.size(); ++r) {
-      const uint64_t r0 = rng[r].first, r1 = rng[r].second;
+      const uint64_t r0 = (uint64_t)rng[r].first << shift;
+      const uint64_t r1 = std::min<uint64_t>((uint64_t)rng[r].second << shift, (uint64_t)g->n);
       const int64_t cnt = (int64_t)rcnt[r];
       uint64_t* kres = pkey;
       float* vres = pval;
       if (cnt > 0) {
-        LN_HIP(ctx, hipMemsetAsync(cursor, 0, 8, s));
-        hipLaunchKernelGGL(k_trunc_part, dim3(grid_blocks((int64_t)tv.cap, TR_THREADS)), dim3(TR_THREADS), 0, s, ta,
-                           (uint32_t)r0, (uint32_t)r1, cursor);
+        const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+        hipLaunchKernelGGL(k_range_counts, dim3((nblk + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh,
+                           rng[r].first, rng[r].second, nblk, blk_emit);
         LN_LAUNCH(ctx);
-        unsigned long long got = 0;
-        LN_HIP(ctx, hipMemcpyAsync(&got, cursor, 8, hipMemcpyDeviceToHost, s));
+        LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+        uint64_t got = 0;
+        LN_HIP(ctx, hipMemcpyAsync(&got, blk_off + nblk, 8, hipMemcpyDeviceToHost, s));
         LN_HIP(ctx, hipStreamSynchronize(s));
         if ((int64_t)got != cnt)
-          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu emitted %llu entries, the histogram %lld",
-                          (unsigned long long)r0, (unsigned long long)r1, got, (long long)cnt);
+          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu holds %llu entries, the histogram %lld",
+                          (unsigned long long)r0, (unsigned long long)r1, (unsigned long long)got, (long long)cnt);
+        hipLaunchKernelGGL(k_trunc_range, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
+                           (uint32_t)r1, (const uint64_t*)blk_off);
+        LN_LAUNCH(ctx);
         LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + colbits, ktmp, vtmp, 0, &kres, &vres));
       }
       hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(cnt + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, cnt,

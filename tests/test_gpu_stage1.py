@@ -180,8 +180,8 @@ def test_packed_counts_wrap_exactly(gpu):
                                               ("blog_t10_s0", 9, False), ("blog_t10_s0", 4, True),
                                               ("toy_t10_s0", 3, False)])
 def test_csr_by_row_ranges_matches_one_sort(gpu, case, parts, upper):
-    """Building the CSR range by range (emission in any order, one sort per
-    range) gives the matrix of the single global sort."""
+    """Building the CSR range by range (one sort per row range) gives the matrix
+    of the single global sort."""
     rec = _rec(case)
     off, edg = H.golden_graph(rec["graph"])
     g = dgraph(off, edg)
