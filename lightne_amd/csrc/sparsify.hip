@@ -15,13 +15,13 @@
 //                    {key, count}.
 //  k_table_reduce  : distinct keys and total weight (PathEmbed.h:175-188),
 //                    deterministic two-level reduction.
-//  k_trunc_count / k_trunc_write : trunc-log (PathEmbed.h:199-218) fused with
+//  k_trunc_count / k_trunc_emit : trunc-log (PathEmbed.h:199-218) fused with
 //                    the symmetric expansion (PathEmbed.h:278-311): kept keys emit
 //                    (row, col) sort keys and float values; per-block counts, a
 //                    scan, then writes in slot order (no contended atomics).
 //  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
 //                    row_ptr by boundary detection (no atomics, deterministic).
-//  k_trunc_range : when the sort buffers for every entry do not fit beside
+//  by row ranges : when the sort buffers for every entry do not fit beside
 //                    the table and the CSR, the rows are cut into ranges by
 //                    the count pass's per-block row histograms; each range is
 //                    emitted from the table (a scan of its per-block counts,
@@ -112,10 +112,25 @@ struct TabView {
   uint64_t cap;
 };
 
-// slot i: endpoints u <= v and the exact count; false when empty
-__device__ __forceinline__ bool tab_get(const TabView& t, uint64_t i, uint32_t& u, uint32_t& v, uint64_t& cnt) {
+// a slot as loaded (packed: the word; wide: key and count), so that scans can
+// issue several loads before decoding any
+struct RawSlot {
+  uint64_t a;
+  uint32_t b;
+};
+
+__device__ __forceinline__ RawSlot raw_load(const TabView& t, uint64_t i) {
+  if (t.pk) return RawSlot{t.pk[i], 0u};
+  const Slot sl = t.wide[i];
+  return RawSlot{sl.key, sl.cnt};
+}
+
+__device__ __forceinline__ RawSlot raw_empty(const TabView& t) { return RawSlot{t.pk ? 0ull : EMPTY_KEY, 0u}; }
+
+// endpoints u <= v and the exact count; false when empty
+__device__ __forceinline__ bool raw_get(const TabView& t, const RawSlot& r, uint32_t& u, uint32_t& v, uint64_t& cnt) {
   if (t.pk) {
-    const uint64_t w = t.pk[i];
+    const uint64_t w = r.a;
     if (!(w & PK_OCC)) return false;
     u = (uint32_t)((w >> PK_VB) & PK_VMASK);
     v = (uint32_t)(w & PK_VMASK);
@@ -123,12 +138,15 @@ __device__ __forceinline__ bool tab_get(const TabView& t, uint64_t i, uint32_t& 
     if (w & PK_OVF) cnt += ovf_get(t.ovf, w & PK_WORD);
     return true;
   }
-  const Slot sl = t.wide[i];
-  if (sl.key == EMPTY_KEY) return false;
-  u = (uint32_t)(sl.key >> 32);
-  v = (uint32_t)sl.key;
-  cnt = sl.cnt;
+  if (r.a == EMPTY_KEY) return false;
+  u = (uint32_t)(r.a >> 32);
+  v = (uint32_t)r.a;
+  cnt = r.b;
   return true;
+}
+
+__device__ __forceinline__ bool tab_get(const TabView& t, uint64_t i, uint32_t& u, uint32_t& v, uint64_t& cnt) {
+  return raw_get(t, raw_load(t, i), u, v, cnt);
 }
 
 __global__ void k_fill_slots(Slot* __restrict__ tab, uint64_t cap) {
@@ -404,27 +422,14 @@ struct TruncArgs {
   float* sval;
 };
 
-// PathEmbed.h:203-217 per slot: returns the number of CSR entries it emits
-// (0, 1 or 2) and the row / column / float value.
+// PathEmbed.h:203-210: adjusted weight of a key, evaluated in the reference's
+// order (w * factor / d_u / d_v, diagonal weight doubled)
 __device__ __forceinline__ double trunc_adj(const TruncArgs& a, uint64_t i, uint32_t u, uint32_t v, uint64_t cnt) {
   const double du = (double)a.deg[u];
   const double dv = (double)a.deg[v];
   const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
   const double wgh = (u != v) ? val : val * 2;
   return wgh * a.factor / du / dv;
-}
-
-__device__ __forceinline__ int trunc_slot(const TruncArgs& a, uint64_t i, uint32_t& u, uint32_t& v, double& adj) {
-  uint64_t cnt;
-  if (!tab_get(a.t, i, u, v, cnt)) return 0;
-  const double du = (double)a.deg[u];
-  const double dv = (double)a.deg[v];
-  const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
-  const double wgh = (u != v) ? val : val * 2;
-  adj = wgh * a.factor / du / dv;
-  // log(adj) > 0  <=>  adj > 1 for any faithful log
-  if (!(adj > 1.0)) return 0;
-  return (a.upper || u == v) ? 1 : 2;
 }
 
 constexpr int TR_THREADS = 256;
@@ -442,6 +447,27 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
 }
 
 constexpr int HIST_BUCKETS = 4096;
+constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in flight
+
+// CSR entries of a loaded slot whose row lies in [r0, r1): 0, 1 or 2 (eu:
+// row u, ev: row v); trunc-log keeps a key iff log(adj) > 0
+// (PathEmbed.h:203-217).  Slots with neither endpoint in the range skip the
+// degree reads.
+__device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const RawSlot& r, uint32_t r0, uint32_t r1,
+                                         uint32_t& u, uint32_t& v, double& adj, bool& eu, bool& ev) {
+  uint64_t cnt;
+  eu = ev = false;
+  if (!raw_get(a.t, r, u, v, cnt)) return 0;
+  const bool iu = u >= r0 && u < r1;
+  const bool iv = !a.upper && u != v && v >= r0 && v < r1;
+  if (!iu && !iv) return 0;
+  adj = trunc_adj(a, i, u, v, cnt);
+  // log(adj) > 0  <=>  adj > 1 for any faithful log
+  if (!(adj > 1.0)) return 0;
+  eu = iu;
+  ev = iv;
+  return (int)iu + (int)iv;
+}
 
 // pass 1: per-block (contiguous slot range) emitted entries and kept keys;
 // with HIST also the block's row histogram (HIST_BUCKETS ranges of 2^shift
@@ -459,15 +485,23 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
   }
   const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
   uint32_t e = 0, kk = 0;
-  for (uint64_t i = b0 + threadIdx.x; i < b1; i += TR_THREADS) {
-    uint32_t u, v;
-    double adj;
-    const int c = trunc_slot(a, i, u, v, adj);
-    e += c;
-    kk += c > 0;
-    if (HIST) {
-      if (c >= 1) atomicAdd(&s_h[u >> shift], 1u);
-      if (c == 2) atomicAdd(&s_h[v >> shift], 1u);
+  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
+    const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
+    RawSlot rs[TR_ITEMS];
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      uint32_t u, v;
+      double adj;
+      bool eu, ev;
+      const int c = emit_slot(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, adj, eu, ev);
+      e += c;
+      kk += c > 0;
+      if (HIST) {
+        if (eu) atomicAdd(&s_h[u >> shift], 1u);
+        if (ev) atomicAdd(&s_h[v >> shift], 1u);
+      }
     }
   }
   const uint32_t te = block_sum_u32(e, s_w);
@@ -505,19 +539,28 @@ __global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* 
     deg[i] = (uint32_t)(off[i + 1] - off[i]);
 }
 
-// pass 2: same partition, entries written in slot order at the block's offset
-__global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_t per_block,
-                                                            const uint64_t* __restrict__ blk_off) {
+// pass 2: same partition; the entries of rows [r0, r1) (all rows: 0, ~0) in
+// slot order at the block's offset (the scan of pass 1's per-block counts,
+// or of a row range's), as sort keys ((row - r0) << colbits | col) and float
+// values.  No contended atomics; the emission order is deterministic.
+__global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
+                                                           const uint64_t* __restrict__ blk_off) {
   __shared__ uint32_t s_w[TR_THREADS / 64];
   const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint64_t run = blk_off[blockIdx.x];
-  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS) {
-    const uint64_t i = t0 + threadIdx.x;
-    uint32_t u = 0, v = 0;
-    double adj = 0.0;
-    const int c = (i < b1) ? trunc_slot(a, i, u, v, adj) : 0;
-    // block-wide exclusive scan of c
+  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
+    const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
+    RawSlot rs[TR_ITEMS];
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
+    uint32_t uu[TR_ITEMS], vv[TR_ITEMS];
+    double adj[TR_ITEMS];
+    bool eu[TR_ITEMS], ev[TR_ITEMS];
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) c += emit_slot(a, ib + j, rs[j], r0, r1, uu[j], vv[j], adj[j], eu[j], ev[j]);
+    // block-wide exclusive scan of the threads' counts
     int incl = c;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -532,76 +575,20 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_write(TruncArgs a, uint64_
       if (w < wid) wbase += s_w[w];
       tot += s_w[w];
     }
-    if (c) {
-      const float lw = (float)log(adj);
-      const uint64_t pos = run + wbase + (uint32_t)(incl - c);
-      a.skey[pos] = ((uint64_t)u << a.colbits) | v;
-      a.sval[pos] = lw;
-      if (c == 2) {
-        a.skey[pos + 1] = ((uint64_t)v << a.colbits) | u;
-        a.sval[pos + 1] = lw;
-      }
-    }
-    run += tot;
-  }
-}
-
-// Entries of slot i whose row lies in [r0, r1): 0, 1 or 2 (eu: row u, ev:
-// row v).  Slots with neither endpoint in the range skip the degree reads.
-__device__ __forceinline__ int range_slot(const TruncArgs& a, uint64_t i, uint32_t r0, uint32_t r1, uint32_t& u,
-                                          uint32_t& v, double& adj, bool& eu, bool& ev) {
-  uint64_t cnt;
-  eu = ev = false;
-  if (!tab_get(a.t, i, u, v, cnt)) return 0;
-  eu = u >= r0 && u < r1;
-  ev = !a.upper && u != v && v >= r0 && v < r1;
-  if (!eu && !ev) return 0;
-  adj = trunc_adj(a, i, u, v, cnt);
-  if (!(adj > 1.0)) return 0;  // log(adj) > 0 (PathEmbed.h:211-214)
-  return (int)eu + (int)ev;
-}
-
-// One row range: the entries in slot order at each block's offset (the
-// scan of the blocks' counts of the range, from the count pass's row
-// histograms) as sort keys ((row - r0) << colbits | col) and float values.
-// No contended atomics; the emission order is deterministic.
-__global__ __launch_bounds__(TR_THREADS) void k_trunc_range(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
-                                                            const uint64_t* __restrict__ blk_off) {
-  __shared__ uint32_t s_w[TR_THREADS / 64];
-  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint64_t run = blk_off[blockIdx.x];
-  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS) {
-    const uint64_t i = t0 + threadIdx.x;
-    uint32_t u = 0, v = 0;
-    double adj = 0.0;
-    bool eu = false, ev = false;
-    const int c = (i < b1) ? range_slot(a, i, r0, r1, u, v, adj, eu, ev) : 0;
-    int incl = c;
+    uint64_t pos = run + wbase + (uint32_t)(incl - c);
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
-    __syncthreads();
-    if (lane == 63) s_w[wid] = (uint32_t)incl;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
-    for (int w = 0; w < TR_THREADS / 64; ++w) {
-      if (w < wid) wbase += s_w[w];
-      tot += s_w[w];
-    }
-    if (c) {
-      const float lw = (float)log(adj);
-      uint64_t pos = run + wbase + (uint32_t)(incl - c);
-      if (eu) {
-        a.skey[pos] = ((uint64_t)(u - r0) << a.colbits) | v;
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      if (!eu[j] && !ev[j]) continue;
+      const float lw = (float)log(adj[j]);
+      if (eu[j]) {
+        a.skey[pos] = ((uint64_t)(uu[j] - r0) << a.colbits) | vv[j];
         a.sval[pos] = lw;
         ++pos;
       }
-      if (ev) {
-        a.skey[pos] = ((uint64_t)(v - r0) << a.colbits) | u;
+      if (ev[j]) {
+        a.skey[pos] = ((uint64_t)(vv[j] - r0) << a.colbits) | uu[j];
         a.sval[pos] = lw;
+        ++pos;
       }
     }
     run += tot;
@@ -963,7 +950,8 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     ta.skey = skey;
     ta.sval = val;
     const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
-    hipLaunchKernelGGL(k_trunc_write, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (const uint64_t*)blk_off);
+    hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
+                       (const uint64_t*)blk_off);
     LN_LAUNCH(ctx);
     // the table is dead once the kept entries are out: the sort's ping-pong
     // buffers reuse its memory when they fit there
@@ -1115,7 +1103,7 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
         if ((int64_t)got != cnt)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu holds %llu entries, the histogram %lld",
                           (unsigned long long)r0, (unsigned long long)r1, (unsigned long long)got, (long long)cnt);
-        hipLaunchKernelGGL(k_trunc_range, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
+        hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
                            (uint32_t)r1, (const uint64_t*)blk_off);
         LN_LAUNCH(ctx);
         LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + colbits, ktmp, vtmp, 0, &kres, &vres));
