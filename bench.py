@@ -265,6 +265,8 @@ def main():
         log(f"step {s}: {times[-1]:.3f}s")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
+    free_b, total_b = torch.cuda.mem_get_info(0)
+    log(f"device memory after the timed steps: {(total_b - free_b) / 1e9:.1f} of {total_b / 1e9:.1f} GB in use")
     if args.copy_calibration:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
         dst = torch.empty_like(src)
