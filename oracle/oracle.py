@@ -13,6 +13,9 @@ numpy/scipy, following the reference's schedule line by line:
 * ``u_sigma_root``-- LightNE/PathEmbed.h:371-408
 * ``propagate``   -- LightNE/SpectralPropagation.h:51-221
 * ``embed_dense`` -- LightNE/SpectralPropagation.h:16-43
+* ``decode_bytepd`` / ``ith_neighbor_bytepd`` -- the compressed graph format
+  (ligra/graph_io.h:344-377, ligra/encodings/byte_pd_amortized.h:86-119,
+  330-386, 653-688), sequential
 
 Parity pins for every function are listed in the header of lightne_oracle.c
 and in tests/ (golden vectors from oracle/_ref, closed forms, library routines).
