@@ -283,8 +283,10 @@ def main():
     t_spmm_call = rs["t_spmm"] / max(rs["spmm_calls"], 1)
     spmm_gbs = spmm_bytes / t_spmm_call / 1e9
     sampled_eps = sp["samples"] / sp["t_sample"] if sp["t_sample"] > 0 else None
-    # sampler byte model: 20 B / walk step + 24 B / sample + 12 B / directed edge
-    sampler_bytes = 20 * sp["walk_steps"] + 24 * sp["samples"] + 12 * hg.m
+    # sampler byte model: 20 B / walk step + one slot read and written per
+    # sample (8-byte or 16-byte slots) + 12 B / directed edge
+    slot_b = 8 if sp["table_layout"] == 2 else 16
+    sampler_bytes = 20 * sp["walk_steps"] + 2 * slot_b * sp["samples"] + 12 * hg.m
     sampler_gbs = sampler_bytes / sp["t_sample"] / 1e9
     stages = {"sample": sp["t_sample"], "table": sp["t_table"], "csr": sp["t_csr"], "sparsify_wall": st["t_sparsify"],
               "rsvd_spmm": rs["t_spmm"], "rsvd_qr": rs["t_qr"], "rsvd_gemm": rs["t_gemm"],
