@@ -161,6 +161,11 @@ __global__ void k_fill_slots(Slot* __restrict__ tab, uint64_t cap) {
   }
 }
 
+__global__ void k_off32(const int64_t* __restrict__ off, int64_t count, uint32_t* __restrict__ o32) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    o32[i] = (uint32_t)off[i];
+}
+
 __global__ void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m, uint32_t* __restrict__ src) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
@@ -200,7 +205,7 @@ __device__ __forceinline__ void block_add3(unsigned long long* dst, uint64_t a, 
 }
 
 struct SampleArgs {
-  const int64_t* off;
+  const uint32_t* off;  // row pointers as 32 bits (2m < 2^32): half the bytes of the walk's hot array
   const uint32_t* adj;
   uint64_t ith;  // 1: compressed-graph neighbour lookup
   const uint32_t* src;
@@ -224,10 +229,10 @@ struct SampleArgs {
 // neighbour i - 1 unless i is a multiple of 64: the result of
 // get_ith_neighbor on such graphs (ligra/encodings/byte_pd_amortized.h:653-688
 // decodes i - start - 1 differences after the block's first edge).
-__device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const uint32_t* __restrict__ adj,
+__device__ __forceinline__ uint32_t walk(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
                                          uint32_t x, uint64_t len, uint64_t& s, bool& bad, uint64_t ith) {
   for (uint64_t t = 0; t < len; ++t) {
-    const int64_t o = off[x];
+    const uint32_t o = off[x];
     const uint64_t d = (uint64_t)(off[x + 1] - o);
     if (d == 0) {
       bad = true;
@@ -236,7 +241,7 @@ __device__ __forceinline__ uint32_t walk(const int64_t* __restrict__ off, const 
     const uint64_t r = hash64_2(s);
     uint64_t i = r % d;
     i -= ith & (uint64_t)((i & 63) != 0);
-    x = adj[o + (int64_t)i];
+    x = adj[(uint64_t)o + i];
     s = hash64_2(r);
   }
   return x;
@@ -686,7 +691,8 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   LN_CHECK(wc.commit(ctx));
   auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
   auto* ovf = packed ? (unsigned long long*)ctx_buf(ctx, "sample_ovf", OVF_CAP * 16) : nullptr;
-  if (!stats || (packed && !ovf)) return LIGHTNE_ERR_OOM;
+  auto* off32 = (uint32_t*)ctx_buf(ctx, "sample_off32", (size_t)(g->n + 1) * 4);
+  if (!stats || (packed && !ovf) || !off32) return LIGHTNE_ERR_OOM;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
@@ -705,8 +711,10 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     const unsigned nb = (unsigned)((g->m + 255) / 256);
     hipLaunchKernelGGL(k_edge_src, dim3(nb), dim3(256), 0, s, g->offsets, g->n, g->m, src);
     LN_LAUNCH(ctx);
+    hipLaunchKernelGGL(k_off32, dim3(grid_blocks(g->n + 1, 256)), dim3(256), 0, s, g->offsets, g->n + 1, off32);
+    LN_LAUNCH(ctx);
     SampleArgs a;
-    a.off = g->offsets;
+    a.off = off32;
     a.adj = g->edges;
     a.ith = g->ith_rule == LIGHTNE_ITH_BYTEPD ? 1u : 0u;
     a.src = src;
