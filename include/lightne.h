@@ -167,6 +167,9 @@ int lightne_csr_create(lightne_ctx* ctx, int64_t n, int64_t nnz, const int64_t* 
 int lightne_csr_info(const lightne_csr* c, int64_t* n, int64_t* nnz);
 int lightne_csr_device_ptrs(const lightne_csr* c, const int64_t** row_ptr, const int32_t** col, const float** val);
 int lightne_csr_copy(lightne_ctx* ctx, const lightne_csr* c, int64_t* row_ptr, int32_t* col, float* val);
+/* the first `count` entries of col / val (count <= nnz; e.g. row_ptr[r] for
+ * the entries of rows [0, r)), host or device destinations */
+int lightne_csr_copy_prefix(lightne_ctx* ctx, const lightne_csr* c, int64_t count, int32_t* col, float* val);
 int lightne_csr_free(lightne_csr* c);
 
 /* ---- stage 2: randomized SVD (mklredsvd.cc:93-403) -------------------------- */

@@ -126,6 +126,7 @@ _SIGS = {
     "lightne_csr_info": (C.c_int, [_P, _PI64, _PI64]),
     "lightne_csr_device_ptrs": (C.c_int, [_P, _PP, _PP, _PP]),
     "lightne_csr_copy": (C.c_int, [_P, _P, _P, _P, _P]),
+    "lightne_csr_copy_prefix": (C.c_int, [_P, _P, _I64, _P, _P]),
     "lightne_csr_free": (C.c_int, [_P]),
     "lightne_rsvd": (C.c_int, [_P, _P, C.POINTER(RsvdParams), _P, _P, C.POINTER(RsvdStats)]),
     "lightne_u_sigma_root": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, C.c_int32, _P]),

@@ -334,17 +334,11 @@ class CSR:
         self.ctx.check(self.ctx.lib.lightne_csr_copy(self.ctx.ptr, self.handle, rp.ctypes.data, None, None))
         r = int(np.searchsorted(rp, nnz_fraction * self.nnz))
         nz = int(rp[min(r, self.n)])
-        torch = _torch()
-        col = torch.empty(nz, dtype=torch.int32)
-        val = torch.empty(nz, dtype=torch.float32)
-        if nz:
-            rpp, cp, vp = C.c_void_p(), C.c_void_p(), C.c_void_p()
-            check(self.ctx.lib.lightne_csr_device_ptrs(self.handle, C.byref(rpp), C.byref(cp), C.byref(vp)))
-            hip = C.CDLL("libamdhip64.so")
-            for dst, src, b in ((col, cp, 4 * nz), (val, vp, 4 * nz)):
-                if hip.hipMemcpy(C.c_void_p(dst.data_ptr()), src, C.c_size_t(b), 2) != 0:  # device to host
-                    raise LightNEError(_lib.ERR_HIP, "hipMemcpy of the CSR prefix failed")
-        return rp, col.numpy(), val.numpy()
+        col = np.empty(nz, np.int32)
+        val = np.empty(nz, np.float32)
+        self.ctx.check(self.ctx.lib.lightne_csr_copy_prefix(self.ctx.ptr, self.handle, nz, col.ctypes.data,
+                                                            val.ctypes.data))
+        return rp, col, val
 
     def free(self) -> None:
         if self.handle:

@@ -198,6 +198,19 @@ int lightne_csr_copy(lightne_ctx* ctx, const lightne_csr* c, int64_t* row_ptr, i
   return LIGHTNE_OK;
 }
 
+int lightne_csr_copy_prefix(lightne_ctx* ctx, const lightne_csr* c, int64_t count, int32_t* col, float* val) {
+  if (!ctx || !c) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_csr_copy_prefix: NULL argument");
+  if (count < 0 || count > c->nnz)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_csr_copy_prefix: count %lld outside [0, %lld]",
+                    (long long)count, (long long)c->nnz);
+  if (count) {
+    if (col) LN_HIP(ctx, hipMemcpyAsync(col, c->col, count * 4, hipMemcpyDefault, ctx->stream));
+    if (val) LN_HIP(ctx, hipMemcpyAsync(val, c->val, count * 4, hipMemcpyDefault, ctx->stream));
+  }
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return LIGHTNE_OK;
+}
+
 int lightne_csr_free(lightne_csr* c) {
   if (!c) return LIGHTNE_OK;
   if (c->plan) {

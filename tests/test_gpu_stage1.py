@@ -219,3 +219,20 @@ def test_netsmf_row_ranges_in_a_fresh_context(gpu):
             np.testing.assert_array_equal(e2.cpu().numpy(), e1.cpu().numpy())
     finally:
         ctx2.close()
+
+
+def test_csr_prefix_copy(gpu):
+    """The entries of the first rows holding a fraction of the nonzeros come to
+    the host alone (row_ptr whole); out-of-range counts are errors."""
+    off, edg = H.golden_graph("cl2000")
+    csr, _ = L.generate_trunc_log_matrix(dgraph(off, edg), 42, sample_ratio=0.9, ctx=gpu)
+    rp, col, val = csr.to_numpy()
+    rp2, col2, val2 = csr.to_numpy(nnz_fraction=0.3)
+    np.testing.assert_array_equal(rp2, rp)
+    nz = len(col2)
+    assert 0 < nz < len(col) and nz == rp[np.searchsorted(rp, 0.3 * len(col))]
+    np.testing.assert_array_equal(col2, col[:nz])
+    np.testing.assert_array_equal(val2, val[:nz])
+    with pytest.raises(L.LightNEError):
+        gpu.check(gpu.lib.lightne_csr_copy_prefix(gpu.ptr, csr.handle, csr.nnz + 1, None, None))
+    csr.free()
