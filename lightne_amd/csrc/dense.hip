@@ -91,8 +91,11 @@ struct GramRows {
 
 // One block owns a contiguous row range; the slab of X (and Y) is staged in
 // LDS while the next slab is already in flight in registers.  TI = float for
-// the FP32 panels, double for the FP64 form.
-template <typename TI>
+// the FP32 panels, double for the FP64 form.  Every wave owns exactly PPW
+// output tiles (the list is padded with dummy tiles whose results are not
+// written), so a 4-row step issues all its LDS reads at once and then its
+// MFMAs back to back, with no per-tile branch between them.
+template <typename TI, int PPW>
 __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ X, const TI* __restrict__ Y,
                                                          int64_t n, int k1, int k2, int npairs,
                                                          const int2* __restrict__ pairs, int64_t rows_per_block,
@@ -106,15 +109,17 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ 
   TI* sY = same ? s_mem : s_mem + ROWS * k1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  int2 pr[GR_MAX_PAIRS_PER_WAVE];
+  int2 pr[PPW];
+  int pa[PPW], pb[PPW];
 #pragma unroll
-  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
-    const int pi = wid + q * GR_WAVES;
-    pr[q] = pi < npairs ? pairs[pi] : make_int2(0, 0);
+  for (int q = 0; q < PPW; ++q) {
+    pr[q] = pairs[wid + q * GR_WAVES];  // padded to PPW * GR_WAVES entries
+    pa[q] = pr[q].x * 16;
+    pb[q] = pr[q].y * 16;
   }
-  doublex4 acc[GR_MAX_PAIRS_PER_WAVE];
+  doublex4 acc[PPW];
 #pragma unroll
-  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) acc[q] = doublex4{0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < PPW; ++q) acc[q] = doublex4{0.0, 0.0, 0.0, 0.0};
   const int64_t r_begin = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r_end = min(n, r_begin + rows_per_block);
   const int nxv = ROWS * k1 / E, nyv = same ? 0 : ROWS * k2 / E;
@@ -147,22 +152,23 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ 
     }
     __syncthreads();
     if (r0 + ROWS < r_end) fetch(r0 + ROWS);
+#pragma unroll 2
     for (int s = 0; s < ROWS; s += 4) {
       const TI* xr = sX + (s + lk) * k1 + li;
       const TI* yr = sY + (s + lk) * k2 + li;
+      double av[PPW], bv[PPW];
 #pragma unroll
-      for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
-        if (wid + q * GR_WAVES < npairs) {
-          const double av = (double)xr[pr[q].x * 16];
-          const double bv = (double)yr[pr[q].y * 16];
-          acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
-        }
+      for (int q = 0; q < PPW; ++q) {
+        av[q] = (double)xr[pa[q]];
+        bv[q] = (double)yr[pb[q]];
       }
+#pragma unroll
+      for (int q = 0; q < PPW; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc[q], 0, 0, 0);
     }
   }
   double* out = part + (int64_t)blockIdx.x * k1 * k2;
 #pragma unroll
-  for (int q = 0; q < GR_MAX_PAIRS_PER_WAVE; ++q) {
+  for (int q = 0; q < PPW; ++q) {
     if (wid + q * GR_WAVES < npairs) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) out[(int64_t)(pr[q].x * 16 + lk + 4 * v) * k2 + pr[q].y * 16 + li] = acc[q][v];
@@ -375,24 +381,48 @@ int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1,
       if (!symmetric || a <= b) pairs.push_back(make_int2(a, b));
   const int npairs_all = (int)pairs.size();
   const int per_launch = GR_MAX_PAIRS_PER_WAVE * GR_WAVES;
+  // each launch's list padded to a whole number of tiles per wave
+  std::vector<int2> padded;
+  std::vector<int> launch_np, launch_ppw, launch_off;
+  for (int p0 = 0; p0 < npairs_all; p0 += per_launch) {
+    const int np = std::min(per_launch, npairs_all - p0);
+    const int ppw = (np + GR_WAVES - 1) / GR_WAVES;
+    launch_np.push_back(np);
+    launch_ppw.push_back(ppw);
+    launch_off.push_back((int)padded.size());
+    for (int i = 0; i < ppw * GR_WAVES; ++i) padded.push_back(i < np ? pairs[p0 + i] : make_int2(0, 0));
+  }
   int nblk = std::max(1, ctx->num_cus);
   int64_t rpb = (n + nblk - 1) / nblk;
   rpb = ((rpb + ROWS - 1) / ROWS) * ROWS;
   if (rpb == 0) rpb = ROWS;
   nblk = (int)std::max<int64_t>(1, (n + rpb - 1) / rpb);
   const int64_t len = k1 * k2;
-  auto* dpairs = (int2*)ctx_buf(ctx, "gram_pairs", pairs.size() * sizeof(int2));
+  auto* dpairs = (int2*)ctx_buf(ctx, "gram_pairs", std::max<size_t>(padded.size(), 1) * sizeof(int2));
   auto* part = (double*)ctx_buf(ctx, "gram_part", (size_t)nblk * len * sizeof(double));
   auto* gsum = (double*)ctx_buf(ctx, "gram_sum", len * sizeof(double));
   if (!dpairs || !part || !gsum) return LIGHTNE_ERR_OOM;
-  LN_HIP(ctx, hipMemcpyAsync(dpairs, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+  if (!padded.empty())
+    LN_HIP(ctx, hipMemcpyAsync(dpairs, padded.data(), padded.size() * sizeof(int2), hipMemcpyHostToDevice,
+                               ctx->stream));
   const size_t lds = (size_t)ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(TI);
   if (n > 0) {
     // each launch owns a disjoint set of output tiles
-    for (int p0 = 0; p0 < npairs_all; p0 += per_launch) {
-      const int np = std::min(per_launch, npairs_all - p0);
-      hipLaunchKernelGGL(k_gram_f64<TI>, dim3(nblk), dim3(GR_THREADS), lds, ctx->stream, X, Y, n, (int)k1, (int)k2, np,
-                         (const int2*)(dpairs + p0), rpb, part);
+    for (size_t L = 0; L < launch_np.size(); ++L) {
+      const int2* dp = dpairs + launch_off[L];
+      const int np = launch_np[L];
+      switch (launch_ppw[L]) {
+#define GRAM_CASE(P)                                                                                          \
+  case P:                                                                                                     \
+    hipLaunchKernelGGL((k_gram_f64<TI, P>), dim3(nblk), dim3(GR_THREADS), lds, ctx->stream, X, Y, n, (int)k1, \
+                       (int)k2, np, dp, rpb, part);                                                          \
+    break;
+        GRAM_CASE(1) GRAM_CASE(2) GRAM_CASE(3) GRAM_CASE(4) GRAM_CASE(5) GRAM_CASE(6)
+        GRAM_CASE(7) GRAM_CASE(8) GRAM_CASE(9) GRAM_CASE(10) GRAM_CASE(11) GRAM_CASE(12)
+#undef GRAM_CASE
+        default:
+          return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: %d tiles per wave", launch_ppw[L]);
+      }
       LN_LAUNCH(ctx);
     }
   } else {
