@@ -1,8 +1,10 @@
 // Device primitives: deterministic exclusive scan and a stable LSD radix sort
 // of (u64 key, 32-bit payload) pairs.  Written for 64-wide wavefronts: peer
 // detection inside a wave uses 8 ballots over the digit bits, cross-wave
-// ranking goes through LDS.
+// ranking goes through LDS, cross-tile offsets through decoupled look-back.
 #include "lightne_internal.h"
+
+#include <algorithm>
 
 namespace {
 
@@ -121,7 +123,9 @@ int scan_impl(lightne_ctx* ctx, const T* in, uint64_t* out, int64_t n, int depth
 }
 
 // ---------------------------------------------------------------------------
-// radix sort
+// radix sort: one histogram read for all digits, then one kernel per 8-bit
+// digit that ranks a tile, finds its global offsets by decoupled look-back
+// over the preceding tiles and scatters.
 // ---------------------------------------------------------------------------
 // 8192-element tiles (512 threads x 16): digit runs of ~32 entries per tile
 // keep the scattered writes long
@@ -129,63 +133,85 @@ constexpr int RS_THREADS = 512;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_MAX_PASSES = 8;
+constexpr int RS_HIST_BLOCKS = 2048;
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
-                                                        uint32_t* __restrict__ hist, int64_t ntiles) {
-  __shared__ uint32_t s_h[256];
-  if (threadIdx.x < 256) s_h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
-  for (int it = 0; it < RS_ITEMS; ++it) {
-    int64_t i = base + (int64_t)it * RS_THREADS + threadIdx.x;
-    if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  if (threadIdx.x < 256) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = s_h[threadIdx.x];
+// Look-back word of (tile, digit): kind in bits 62-63 (0 none yet, 1 the
+// tile's own count, 2 the inclusive prefix over tiles 0..tile), the pass tag
+// (pass + 1) in bits 56-61, the value in bits 0-55.  Count and kind travel
+// in one 64-bit store, so a reader never sees one without the other.
+constexpr uint64_t LB_VALUE = (1ull << 56) - 1;
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 22;
+constexpr int LB_WINDOW = 8;
+
+__device__ __forceinline__ uint64_t lb_word(uint64_t kind, uint32_t tag, uint64_t v) {
+  return (kind << 62) | ((uint64_t)tag << 56) | v;
 }
 
-// Stable scatter of one 8-bit digit.  Each wave ranks its own contiguous
-// 1024-element part of the tile (peers by 8 ballots per item, a running
-// per-digit count in LDS that only this wave touches, so no block barrier per
-// item); one block pass turns the waves' counts into LDS offsets; the tile is
-// staged in digit order and each digit run is written to its global offset
-// with consecutive lanes on consecutive addresses.
+// per-digit totals of every pass in one read of the keys
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, int lo,
+                                                            int npass, unsigned long long* __restrict__ ghist) {
+  __shared__ uint32_t s_h[RS_MAX_PASSES * 256];
+  for (int i = threadIdx.x; i < npass * 256; i += RS_THREADS) s_h[i] = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RS_THREADS) {
+    const uint64_t k = keys[i];
+    for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * 256 + (uint32_t)((k >> (lo + 8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < npass * 256; i += RS_THREADS)
+    if (s_h[i]) atomicAdd(&ghist[i], (unsigned long long)s_h[i]);
+}
+
+// block p: exclusive scan of pass p's 256 digit totals
+__global__ __launch_bounds__(256) void k_rs_digit_starts(const unsigned long long* __restrict__ ghist,
+                                                         uint64_t* __restrict__ dstart) {
+  __shared__ uint64_t s_w[4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint64_t c = ghist[blockIdx.x * 256 + tid];
+  const uint64_t incl = wave_incl_scan(c);
+  if (lane == 63) s_w[wid] = incl;
+  __syncthreads();
+  uint64_t off = 0;
+  for (int w = 0; w < wid; ++w) off += s_w[w];
+  dstart[blockIdx.x * 256 + tid] = off + incl - c;
+}
+
+// One stable pass over one 8-bit digit.  The block takes the next tile in
+// launch order (a counter, so every earlier tile belongs to a block that is
+// already running), ranks it (each wave ranks its own contiguous 1024
+// elements: peers by 8 ballots per item and a per-wave running count per
+// digit in LDS, so no block barrier per item), publishes the tile's digit
+// counts, and reads back over the earlier tiles' words until an inclusive
+// prefix, adding their counts.  The tile's keys, then its values, are staged
+// in digit order in one LDS buffer (75 KB per block, two blocks per CU) and
+// each digit run is written at its global offset with consecutive lanes on
+// consecutive addresses.  The look-back wait is bounded: past the limit it
+// raises *err and gives up, so the kernel always ends.
 template <typename V>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin,
-                                                           const V* __restrict__ vin, uint64_t* __restrict__ kout,
-                                                           V* __restrict__ vout, int64_t n, int shift,
-                                                           const uint64_t* __restrict__ offs,
-                                                           const uint32_t* __restrict__ hist, int64_t ntiles) {
-  __shared__ uint64_t s_keys[RS_TILE];
-  __shared__ V s_vals[RS_TILE];
+__global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __restrict__ kin,
+                                                            const V* __restrict__ vin, uint64_t* __restrict__ kout,
+                                                            V* __restrict__ vout, int64_t n, int shift,
+                                                            const uint64_t* __restrict__ dstart,
+                                                            uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
+                                                            uint32_t tag, uint32_t* __restrict__ err) {
+  static_assert(sizeof(V) <= sizeof(uint64_t), "values are staged in the key buffer");
+  __shared__ uint64_t s_stage[RS_TILE];
   __shared__ uint64_t s_base[256];
   __shared__ uint32_t s_dstart[256];
   __shared__ uint32_t s_wcnt[RS_WAVES][256];
   __shared__ uint32_t s_scan[RS_WAVES];
+  __shared__ uint32_t s_tile, s_fail;
+  V* s_vstage = reinterpret_cast<V*>(s_stage);
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  // tile-local start of every digit (threads 0..255, one digit each):
-  // exclusive scan of this tile's histogram
-  {
-    const uint32_t c = tid < 256 ? hist[(int64_t)tid * ntiles + blockIdx.x] : 0u;
-    uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
-    if (lane == 63) s_scan[wid] = incl;
-    __syncthreads();
-    uint32_t off = 0;
-    for (int w = 0; w < wid; ++w) off += s_scan[w];
-    if (tid < 256) {
-      s_dstart[tid] = off + incl - c;
-      s_base[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
-    }
+  if (tid == 0) {
+    s_tile = atomicAdd(tile_ctr, 1u);
+    s_fail = 0;
   }
   for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t tile = s_tile;
+  const int64_t base = tile * RS_TILE;
   const int nvalid = (int)min<int64_t>(RS_TILE, n - base);
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t kr[RS_ITEMS];
@@ -215,33 +241,124 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     pr[it] = valid ? (d | ((before + rank) << 8)) : ~0u;
   }
   __syncthreads();
-  // per digit: LDS start of each wave's run = digit start + earlier waves' counts
+  // thread d < 256 owns digit d: the tile's count, published at once
+  uint32_t c = 0;
   if (tid < 256) {
-    uint32_t run = s_dstart[tid];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) c += s_wcnt[w][tid];
+    __hip_atomic_store(status + tile * 256 + tid, lb_word(tile == 0 ? 2 : 1, tag, c), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // tile-local start of every digit: exclusive scan of the counts
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  if (lane == 63) s_scan[wid] = incl;
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t off = 0;
+    for (int w = 0; w < wid; ++w) off += s_scan[w];
+    const uint32_t ds = off + incl - c;
+    s_dstart[tid] = ds;
+    // LDS start of each wave's run of digit tid
+    uint32_t run = ds;
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) {
-      const uint32_t c = s_wcnt[w][tid];
+      const uint32_t cw = s_wcnt[w][tid];
       s_wcnt[w][tid] = run;
-      run += c;
+      run += cw;
     }
+    // global offset: counts of this digit in all earlier tiles, read
+    // LB_WINDOW tiles at a time (independent loads), summed in tile order up
+    // to the first inclusive word
+    uint64_t excl = 0;
+    if (tile > 0) {
+      int64_t t = tile - 1;
+      uint32_t spins = 0;
+      while (true) {
+        uint64_t w[LB_WINDOW];
+#pragma unroll
+        for (int j = 0; j < LB_WINDOW; ++j)
+          w[j] = t - j >= 0 ? __hip_atomic_load(status + (t - j) * 256 + tid, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int j = 0; j < LB_WINDOW; ++j) {
+          if (used != j || done) continue;
+          const uint32_t kind = (uint32_t)(w[j] >> 62);
+          if (kind == 0 || (uint32_t)((w[j] >> 56) & 63u) != tag) continue;  // not yet published
+          excl += w[j] & LB_VALUE;
+          used = j + 1;
+          done = kind == 2;
+        }
+        if (done) break;
+        t -= used;
+        if (t < 0) {  // tile 0 always publishes an inclusive word
+          atomicOr(err, 2u);
+          s_fail = 1;
+          break;
+        }
+        if (used == 0) {
+          if (++spins > LB_SPIN_LIMIT) {
+            atomicOr(err, 1u);
+            s_fail = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        } else {
+          spins = 0;
+        }
+      }
+      __hip_atomic_store(status + tile * 256 + tid, lb_word(2, tag, excl + c), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_base[tid] = dstart[tid] + excl;
   }
   __syncthreads();
+  if (s_fail) return;  // offsets unknown: write nothing (the host reports *err)
+  // LDS slot of each item (reused for the values)
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
     if (pr[it] != ~0u) {
-      const uint32_t lp = s_wcnt[wid][pr[it] & 255u] + (pr[it] >> 8);
-      s_keys[lp] = kr[it];
-      s_vals[lp] = vr[it];
+      pr[it] = s_wcnt[wid][pr[it] & 255u] + (pr[it] >> 8);
+      s_stage[pr[it]] = kr[it];
     }
   }
   __syncthreads();
-  for (int li = tid; li < nvalid; li += RS_THREADS) {
-    const uint64_t k = s_keys[li];
-    const uint32_t d = (uint32_t)((k >> shift) & 255u);
-    const uint64_t pos = s_base[d] + (uint64_t)(li - (int)s_dstart[d]);
-    kout[pos] = k;
-    vout[pos] = s_vals[li];
+  uint32_t dg[RS_ITEMS];  // digit of the staged element this thread writes out
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int li = it * RS_THREADS + tid;
+    if (li < nvalid) {
+      const uint64_t k = s_stage[li];
+      const uint32_t d = (uint32_t)((k >> shift) & 255u);
+      dg[it] = d;
+      kout[s_base[d] + (uint64_t)(li - (int)s_dstart[d])] = k;
+    }
   }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it)
+    if (pr[it] != ~0u) s_vstage[pr[it]] = vr[it];
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int li = it * RS_THREADS + tid;
+    if (li < nvalid) {
+      const uint32_t d = dg[it];
+      vout[s_base[d] + (uint64_t)(li - (int)s_dstart[d])] = s_vstage[li];
+    }
+  }
+}
+
+__global__ void k_rs_clear(uint64_t* __restrict__ p, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 template <typename V>
@@ -250,33 +367,55 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
   if (k_res) *k_res = keys;
   if (v_res) *v_res = vals;
   if (n <= 1 || bits <= 0) return LIGHTNE_OK;
+  const int npass = (bits + 7) / 8;
+  if (npass > RS_MAX_PASSES)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "radix sort of %d bits (at most %d)", bits, 8 * RS_MAX_PASSES);
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   if (!k2 || !v2) {
     k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
     v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
   }
-  auto* hist = (uint32_t*)ctx_buf(ctx, "rs_hist", 256 * ntiles * sizeof(uint32_t));
-  auto* offs = (uint64_t*)ctx_buf(ctx, "rs_offs", (256 * ntiles + 1) * sizeof(uint64_t));
-  if (!k2 || !v2 || !hist || !offs) return LIGHTNE_ERR_OOM;
+  // one zero-filled block: look-back words (ntiles x 256), digit totals and
+  // starts (npass x 256 each), tile counters, error flag
+  const int64_t nstat = ntiles * 256;
+  const int64_t nwords = nstat + 2 * RS_MAX_PASSES * 256 + RS_MAX_PASSES + 1;
+  auto* state = (uint64_t*)ctx_buf(ctx, "rs_state", nwords * sizeof(uint64_t));
+  if (!k2 || !v2 || !state) return LIGHTNE_ERR_OOM;
+  uint64_t* status = state;
+  auto* ghist = (unsigned long long*)(state + nstat);
+  uint64_t* dstart = state + nstat + RS_MAX_PASSES * 256;
+  auto* tile_ctr = (uint32_t*)(state + nstat + 2 * RS_MAX_PASSES * 256);
+  auto* err = (uint32_t*)(state + nwords - 1);
+  const hipStream_t s = ctx->stream;
+  hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)std::min<int64_t>(4096, (nwords + 255) / 256)), dim3(256), 0, s, state,
+                     nwords);
+  LN_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_rs_hist_all, dim3((unsigned)std::min<int64_t>(RS_HIST_BLOCKS, ntiles)), dim3(RS_THREADS), 0, s,
+                     keys, n, lo, npass, ghist);
+  LN_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_rs_digit_starts, dim3(npass), dim3(256), 0, s, (const unsigned long long*)ghist, dstart);
+  LN_LAUNCH(ctx);
   uint64_t *ka = keys, *kb = k2;
   V *va = vals, *vb = v2;
-  for (int shift = lo; shift < lo + bits; shift += 8) {
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, n, shift, hist,
-                       ntiles);
-    LN_LAUNCH(ctx);
-    LN_CHECK(scan_u32_to_u64(ctx, hist, offs, 256 * ntiles));
-    hipLaunchKernelGGL(k_rs_scatter<V>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, ctx->stream, ka, va, kb, vb, n,
-                       shift, (const uint64_t*)offs, (const uint32_t*)hist, ntiles);
+  for (int p = 0; p < npass; ++p) {
+    hipLaunchKernelGGL(k_rs_onesweep<V>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, lo + 8 * p,
+                       (const uint64_t*)(dstart + p * 256), status, tile_ctr + p, (uint32_t)(p + 1), err);
     LN_LAUNCH(ctx);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
+  uint32_t herr = 0;
+  LN_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(herr), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  if (herr)
+    return ctx_fail(ctx, LIGHTNE_ERR_HIP, "radix sort: tile look-back failed (code %u, %lld tiles)", herr,
+                    (long long)ntiles);
   if (k_res) {  // the caller reads the result where the last pass left it
     *k_res = ka;
     *v_res = va;
   } else if (ka != keys) {
-    LN_HIP(ctx, hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
-    LN_HIP(ctx, hipMemcpyAsync(vals, va, n * sizeof(V), hipMemcpyDeviceToDevice, ctx->stream));
+    LN_HIP(ctx, hipMemcpyAsync(keys, ka, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    LN_HIP(ctx, hipMemcpyAsync(vals, va, n * sizeof(V), hipMemcpyDeviceToDevice, s));
   }
   return LIGHTNE_OK;
 }
