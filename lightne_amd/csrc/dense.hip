@@ -193,7 +193,24 @@ __global__ void k_sum_parts(const double* __restrict__ part, int nparts, int64_t
 constexpr int PS_THREADS = 512;
 constexpr int PS_MAX_T2 = 16;  // k2 <= 256
 
-template <int T2>
+// one 16-column step t of X's slab row (lane: row li, columns 16t + 4lk..+3)
+// times rows 16t.. of the LDS copy of W; TRI (W upper triangular, k1 = k2):
+// output tile j only takes steps t <= j, the others multiply zeros
+template <int T2, bool TRI>
+__device__ __forceinline__ void ps_step(const float* X, int64_t row, bool rv, int64_t ldx, int t, int lk, int li,
+                                        const float* sW, int ldw, floatx4* acc) {
+  float4 xa = rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
+  const float xs[4] = {xa.x, xa.y, xa.z, xa.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float* wr = sW + (16 * t + 4 * lk + q) * ldw + li;
+#pragma unroll
+    for (int j = 0; j < T2; ++j)
+      if (!TRI || j >= t) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[q], wr[16 * j], acc[j], 0, 0, 0);
+  }
+}
+
+template <int T2, bool TRI>
 __global__ __launch_bounds__(PS_THREADS) void k_panel_small(const float* X, int64_t ldx, int64_t n,
                                                             int k1, const float* __restrict__ W, int ldW,
                                                             float* Out, int64_t ldo, int normalize) {
@@ -216,15 +233,11 @@ __global__ __launch_bounds__(PS_THREADS) void k_panel_small(const float* X, int6
     floatx4 acc[T2];
 #pragma unroll
     for (int j = 0; j < T2; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int t = 0; t < k1 / 16; ++t) {
-      float4 xa = rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float xs[4] = {xa.x, xa.y, xa.z, xa.w};
+    if constexpr (TRI) {  // k1 == 16 * T2: the steps unroll, so the skipped tiles are known at compile time
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float* wr = sW + (16 * t + 4 * lk + q) * ldw + li;
-#pragma unroll
-        for (int j = 0; j < T2; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[q], wr[16 * j], acc[j], 0, 0, 0);
-      }
+      for (int t = 0; t < T2; ++t) ps_step<T2, true>(X, row, rv, ldx, t, lk, li, sW, ldw, acc);
+    } else {
+      for (int t = 0; t < k1 / 16; ++t) ps_step<T2, false>(X, row, rv, ldx, t, lk, li, sW, ldw, acc);
     }
     // acc[j][v] = Out[slab*16 + 4*lk + v][16 j + li]
     if (normalize) {
@@ -451,7 +464,7 @@ int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int6
 }
 
 int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,
-                      int64_t k2, float* Out, int normalize_rows, float*) {
+                      int64_t k2, float* Out, int normalize_rows, float*, int upper_tri) {
   if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 16 * PS_MAX_T2 || ldx < k1 || ldx % 4)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: bad shape k1=%lld k2=%lld ldx=%lld",
                     (long long)k1, (long long)k2, (long long)ldx);
@@ -479,14 +492,20 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
   const int64_t waves_per_block = PS_THREADS / 64;
   const int64_t nblk = std::max<int64_t>(1, std::min<int64_t>((nslab + waves_per_block - 1) / waves_per_block,
                                                               (int64_t)ctx->num_cus * 4));
+  // an upper-triangular W skips its zero tiles when it is square and in one block
+  const bool tri = upper_tri && k1 == k2 && kb == k2;
   for (int64_t j0 = 0; j0 < k2; j0 += kb) {
     const int64_t kk = std::min(kb, k2 - j0);
     const size_t lds = (size_t)k1 * (kk + 4) * sizeof(float);
     const int T2 = (int)(kk / 16);
 #define PS_CASE(t)                                                                                               \
   case t:                                                                                                        \
-    hipLaunchKernelGGL(k_panel_small<t>, dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X, ldx, n,     \
-                       (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows);        \
+    if (tri)                                                                                                     \
+      hipLaunchKernelGGL((k_panel_small<t, true>), dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X,   \
+                         ldx, n, (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows); \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_panel_small<t, false>), dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X,  \
+                         ldx, n, (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows); \
     break;
     switch (T2) {
       PS_CASE(1) PS_CASE(2) PS_CASE(3) PS_CASE(4) PS_CASE(5) PS_CASE(6) PS_CASE(7) PS_CASE(8)

@@ -180,9 +180,10 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
              double* G_host);
 // Out(n x k2) = X(n x k1) W(k1 x k2) with W a host f64 matrix used as f32.
 // Optional row L2-normalisation epilogue (norm > eps).  Out may alias X only
-// when k2 == k1 (rows are staged through registers).
+// when k2 == k1 (rows are staged through registers).  upper_tri: W is upper
+// triangular (its zero tiles are skipped).
 int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,
-                      int64_t k2, float* Out, int normalize_rows, float* unused);
+                      int64_t k2, float* Out, int normalize_rows, float* unused, int upper_tri = 0);
 // emb = sqrt(S) U[:, :dim] with optional row normalisation (device kernel)
 int u_sigma_root_dev(lightne_ctx* ctx, const float* U, const float* S_host, int64_t n, int64_t rank, int64_t dim,
                      int normalize, float* emb);

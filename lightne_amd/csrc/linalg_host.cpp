@@ -17,32 +17,47 @@
 #include <numeric>
 #include <vector>
 
-bool chol_upper(const double* G, int k, double* R) {
+// Right-looking: row j of R from the updated trailing block, then that block
+// (upper triangle, row by row) loses the outer product of row j; every inner
+// loop runs along a contiguous row.
+bool chol_upper(const double* G, int k, double* R_out) {
+  std::vector<double> Wbuf(G, G + (size_t)k * k);
+  double* __restrict W = Wbuf.data();
+  double* __restrict R = R_out;
   std::fill(R, R + (size_t)k * k, 0.0);
   for (int j = 0; j < k; ++j) {
-    double d = G[(size_t)j * k + j];
-    for (int p = 0; p < j; ++p) d -= R[(size_t)p * k + j] * R[(size_t)p * k + j];
+    const double d = W[(size_t)j * k + j];
     if (!(d > 0.0) || !std::isfinite(d)) return false;
     const double rjj = std::sqrt(d);
-    R[(size_t)j * k + j] = rjj;
+    double* __restrict rj = R + (size_t)j * k;
+    const double* __restrict wj = W + (size_t)j * k;
+    rj[j] = rjj;
+    for (int i = j + 1; i < k; ++i) rj[i] = wj[i] / rjj;
     for (int i = j + 1; i < k; ++i) {
-      double s = G[(size_t)j * k + i];
-      for (int p = 0; p < j; ++p) s -= R[(size_t)p * k + j] * R[(size_t)p * k + i];
-      R[(size_t)j * k + i] = s / rjj;
+      const double f = rj[i];
+      double* __restrict wi = W + (size_t)i * k;
+      for (int l = i; l < k; ++l) wi[l] -= f * rj[l];
     }
   }
   return true;
 }
 
-void tri_upper_inverse(const double* R, int k, double* Rinv) {
-  std::fill(Rinv, Rinv + (size_t)k * k, 0.0);
-  for (int j = k - 1; j >= 0; --j) {
-    Rinv[(size_t)j * k + j] = 1.0 / R[(size_t)j * k + j];
-    for (int i = j - 1; i >= 0; --i) {
-      double s = 0.0;
-      for (int p = i + 1; p <= j; ++p) s += R[(size_t)i * k + p] * Rinv[(size_t)p * k + j];
-      Rinv[(size_t)i * k + j] = -s / R[(size_t)i * k + i];
+// Rows from the bottom up: row i of R^-1 is (e_i - sum_{p>i} R[i][p] row_p) /
+// R[i][i], an update along contiguous rows.
+void tri_upper_inverse(const double* R_in, int k, double* Rinv_out) {
+  const double* __restrict R = R_in;
+  double* __restrict X = Rinv_out;
+  std::fill(X, X + (size_t)k * k, 0.0);
+  for (int i = k - 1; i >= 0; --i) {
+    double* __restrict xi = X + (size_t)i * k;
+    xi[i] = 1.0;
+    for (int p = i + 1; p < k; ++p) {
+      const double f = R[(size_t)i * k + p];
+      const double* __restrict xp = X + (size_t)p * k;
+      for (int l = p; l < k; ++l) xi[l] -= f * xp[l];
     }
+    const double inv = 1.0 / R[(size_t)i * k + i];
+    for (int l = i; l < k; ++l) xi[l] *= inv;
   }
 }
 

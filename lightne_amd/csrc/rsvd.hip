@@ -79,7 +79,7 @@ int csr_mm64(lightne_ctx* ctx, lightne_csr* A, const double* X, int64_t k, doubl
 // Y <- Y W for a square host W (in place for FP32 panels; FP64 panels go
 // through tmp)
 int apply_small(lightne_ctx* ctx, float* Y, float*, int64_t n, int64_t k, const double* W) {
-  return panel_times_small(ctx, Y, n, k, k, W, k, Y, 0, nullptr);
+  return panel_times_small(ctx, Y, n, k, k, W, k, Y, 0, nullptr, 1);  // W = R^-1, upper triangular
 }
 int apply_small(lightne_ctx* ctx, double* Y, double* tmp, int64_t n, int64_t k, const double* W) {
   LN_CHECK(panel_times_small64(ctx, Y, n, k, W, k, tmp));
