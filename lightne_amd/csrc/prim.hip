@@ -5,6 +5,7 @@
 #include "lightne_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -134,6 +135,7 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_WAVES = RS_THREADS / 64;
 constexpr int RS_MAX_PASSES = 8;
+constexpr int RS_MAX_DIGITS = 512;  // 9-bit digits at most
 constexpr int RS_HIST_BLOCKS = 2048;
 
 // Look-back word of (tile, digit): kind in bits 62-63 (0 none yet, 1 the
@@ -148,58 +150,75 @@ __device__ __forceinline__ uint64_t lb_word(uint64_t kind, uint32_t tag, uint64_
   return (kind << 62) | ((uint64_t)tag << 56) | v;
 }
 
-// per-digit totals of every pass in one read of the keys
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, int lo,
-                                                            int npass, unsigned long long* __restrict__ ghist) {
-  __shared__ uint32_t s_h[RS_MAX_PASSES * 256];
-  for (int i = threadIdx.x; i < npass * 256; i += RS_THREADS) s_h[i] = 0;
+// the passes: digit p is bits [shift[p], shift[p] + width[p]) of the key
+struct RsPlan {
+  int npass;
+  int shift[RS_MAX_PASSES];
+  int width[RS_MAX_PASSES];
+};
+
+// per-digit totals of every pass in one read of the keys (pass p's counters
+// at ghist[p * RS_MAX_DIGITS ..])
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, RsPlan pl,
+                                                            unsigned long long* __restrict__ ghist) {
+  __shared__ uint32_t s_h[RS_MAX_PASSES * RS_MAX_DIGITS];
+  for (int i = threadIdx.x; i < pl.npass * RS_MAX_DIGITS; i += RS_THREADS) s_h[i] = 0;
   __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RS_THREADS) {
     const uint64_t k = keys[i];
-    for (int p = 0; p < npass; ++p) atomicAdd(&s_h[p * 256 + (uint32_t)((k >> (lo + 8 * p)) & 255u)], 1u);
+    for (int p = 0; p < pl.npass; ++p)
+      atomicAdd(&s_h[p * RS_MAX_DIGITS + (uint32_t)((k >> pl.shift[p]) & ((1u << pl.width[p]) - 1u))], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < npass * 256; i += RS_THREADS)
+  for (int i = threadIdx.x; i < pl.npass * RS_MAX_DIGITS; i += RS_THREADS)
     if (s_h[i]) atomicAdd(&ghist[i], (unsigned long long)s_h[i]);
 }
 
-// block p: exclusive scan of pass p's 256 digit totals
-__global__ __launch_bounds__(256) void k_rs_digit_starts(const unsigned long long* __restrict__ ghist,
-                                                         uint64_t* __restrict__ dstart) {
-  __shared__ uint64_t s_w[4];
+// block p: exclusive scan of pass p's digit totals (512 slots; the unused
+// ones of an 8-bit pass hold zeros)
+__global__ __launch_bounds__(RS_MAX_DIGITS) void k_rs_digit_starts(const unsigned long long* __restrict__ ghist,
+                                                                   uint64_t* __restrict__ dstart) {
+  __shared__ uint64_t s_w[RS_MAX_DIGITS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const uint64_t c = ghist[blockIdx.x * 256 + tid];
+  const uint64_t c = ghist[blockIdx.x * RS_MAX_DIGITS + tid];
   const uint64_t incl = wave_incl_scan(c);
   if (lane == 63) s_w[wid] = incl;
   __syncthreads();
   uint64_t off = 0;
   for (int w = 0; w < wid; ++w) off += s_w[w];
-  dstart[blockIdx.x * 256 + tid] = off + incl - c;
+  dstart[blockIdx.x * RS_MAX_DIGITS + tid] = off + incl - c;
 }
 
-// One stable pass over one 8-bit digit.  The block takes the next tile in
-// launch order (a counter, so every earlier tile belongs to a block that is
-// already running), ranks it (each wave ranks its own contiguous 1024
-// elements: peers by 8 ballots per item and a per-wave running count per
-// digit in LDS, so no block barrier per item), publishes the tile's digit
-// counts, and reads back over the earlier tiles' words until an inclusive
-// prefix, adding their counts.  The tile's keys, then its values, are staged
-// in digit order in one LDS buffer (75 KB per block, two blocks per CU) and
-// each digit run is written at its global offset with consecutive lanes on
-// consecutive addresses.  The look-back wait is bounded: past the limit it
-// raises *err and gives up, so the kernel always ends.
-template <typename V>
-__global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __restrict__ kin,
+// One stable pass over one RB-bit digit (RB = 8 or 9).  The block takes the
+// next tile in launch order (a counter, so every earlier tile belongs to a
+// block that is already running), ranks it (each wave ranks its own
+// contiguous 1024 elements: peers by RB ballots per item and a per-wave
+// running count per digit in LDS, so no block barrier per item), publishes
+// the tile's digit counts, and reads back over the earlier tiles' words until
+// an inclusive prefix, adding their counts.  The tile's keys, then its
+// values, are staged in digit order in one LDS buffer (at most 78 KB per
+// block, two blocks per CU) and each digit run is written at its global
+// offset with consecutive lanes on consecutive addresses.  The look-back wait
+// is bounded: past the limit it raises *err and writes nothing, so the kernel
+// always ends.
+template <typename V, int RB>
+__global__ __launch_bounds__(RS_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_rs_onesweep(const uint64_t* __restrict__ kin,
                                                             const V* __restrict__ vin, uint64_t* __restrict__ kout,
                                                             V* __restrict__ vout, int64_t n, int shift,
                                                             const uint64_t* __restrict__ dstart,
                                                             uint64_t* __restrict__ status, uint32_t* __restrict__ tile_ctr,
                                                             uint32_t tag, uint32_t* __restrict__ err) {
   static_assert(sizeof(V) <= sizeof(uint64_t), "values are staged in the key buffer");
+  constexpr int RD = 1 << RB;
+  constexpr uint32_t DM = RD - 1;
+  static_assert(RD <= RS_THREADS, "one thread per digit");
   __shared__ uint64_t s_stage[RS_TILE];
-  __shared__ uint64_t s_base[256];
-  __shared__ uint32_t s_dstart[256];
-  __shared__ uint32_t s_wcnt[RS_WAVES][256];
+  __shared__ uint64_t s_base[RD];
+  __shared__ uint32_t s_dstart[RD];
+  // a wave's count of one digit (at most 1024), later a tile offset (< 8192):
+  // 16 bits for 9-bit digits keep the block within half the LDS
+  using WC = typename std::conditional<RB == 9, uint16_t, uint32_t>::type;
+  __shared__ WC s_wcnt[RS_WAVES][RD];
   __shared__ uint32_t s_scan[RS_WAVES];
   __shared__ uint32_t s_tile, s_fail;
   V* s_vstage = reinterpret_cast<V*>(s_stage);
@@ -208,7 +227,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
     s_tile = atomicAdd(tile_ctr, 1u);
     s_fail = 0;
   }
-  for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
+  for (int i = tid; i < RS_WAVES * RD; i += RS_THREADS) (&s_wcnt[0][0])[i] = 0;
   __syncthreads();
   const int64_t tile = s_tile;
   const int64_t base = tile * RS_TILE;
@@ -216,17 +235,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t kr[RS_ITEMS];
   V vr[RS_ITEMS];
-  uint32_t pr[RS_ITEMS];  // digit | (position among the wave's elements of that digit) << 8
+  uint32_t pr[RS_ITEMS];  // digit | (position among the wave's elements of that digit) << RB
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
     const int li = wid * (RS_ITEMS * 64) + it * 64 + lane;
     const bool valid = li < nvalid;
     const uint64_t k = valid ? kin[base + li] : 0;
     const V v = valid ? vin[base + li] : V(0);
-    const uint32_t d = (uint32_t)((k >> shift) & 255u);
+    const uint32_t d = (uint32_t)((k >> shift) & DM);
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < RB; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t bal = __ballot(bit);
       peers &= bit ? bal : ~bal;
@@ -235,18 +254,18 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
     // every lane reads the wave's running count before the leader adds to it
     // (one wave's LDS operations complete in order)
     const uint32_t before = s_wcnt[wid][d];
-    if (valid && rank == 0) s_wcnt[wid][d] = before + (uint32_t)__popcll(peers);
+    if (valid && rank == 0) s_wcnt[wid][d] = (WC)(before + (uint32_t)__popcll(peers));
     kr[it] = k;
     vr[it] = v;
-    pr[it] = valid ? (d | ((before + rank) << 8)) : ~0u;
+    pr[it] = valid ? (d | ((before + rank) << RB)) : ~0u;
   }
   __syncthreads();
-  // thread d < 256 owns digit d: the tile's count, published at once
+  // thread d < RD owns digit d: the tile's count, published at once
   uint32_t c = 0;
-  if (tid < 256) {
+  if (tid < RD) {
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) c += s_wcnt[w][tid];
-    __hip_atomic_store(status + tile * 256 + tid, lb_word(tile == 0 ? 2 : 1, tag, c), __ATOMIC_RELAXED,
+    __hip_atomic_store(status + tile * RD + tid, lb_word(tile == 0 ? 2 : 1, tag, c), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
   // tile-local start of every digit: exclusive scan of the counts
@@ -258,7 +277,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
   }
   if (lane == 63) s_scan[wid] = incl;
   __syncthreads();
-  if (tid < 256) {
+  if (tid < RD) {
     uint32_t off = 0;
     for (int w = 0; w < wid; ++w) off += s_scan[w];
     const uint32_t ds = off + incl - c;
@@ -268,7 +287,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) {
       const uint32_t cw = s_wcnt[w][tid];
-      s_wcnt[w][tid] = run;
+      s_wcnt[w][tid] = (WC)run;
       run += cw;
     }
     // global offset: counts of this digit in all earlier tiles, read
@@ -282,7 +301,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
         uint64_t w[LB_WINDOW];
 #pragma unroll
         for (int j = 0; j < LB_WINDOW; ++j)
-          w[j] = t - j >= 0 ? __hip_atomic_load(status + (t - j) * 256 + tid, __ATOMIC_RELAXED,
+          w[j] = t - j >= 0 ? __hip_atomic_load(status + (t - j) * RD + tid, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT)
                             : 0ull;
         int used = 0;
@@ -314,7 +333,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
           spins = 0;
         }
       }
-      __hip_atomic_store(status + tile * 256 + tid, lb_word(2, tag, excl + c), __ATOMIC_RELAXED,
+      __hip_atomic_store(status + tile * RD + tid, lb_word(2, tag, excl + c), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
     s_base[tid] = dstart[tid] + excl;
@@ -325,19 +344,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
     if (pr[it] != ~0u) {
-      pr[it] = s_wcnt[wid][pr[it] & 255u] + (pr[it] >> 8);
+      pr[it] = (uint32_t)s_wcnt[wid][pr[it] & DM] + (pr[it] >> RB);
       s_stage[pr[it]] = kr[it];
     }
   }
   __syncthreads();
-  uint32_t dg[RS_ITEMS];  // digit of the staged element this thread writes out
+  uint32_t dg[RS_ITEMS / 2] = {};  // digits of the staged elements this thread writes out, two per register
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
     const int li = it * RS_THREADS + tid;
     if (li < nvalid) {
       const uint64_t k = s_stage[li];
-      const uint32_t d = (uint32_t)((k >> shift) & 255u);
-      dg[it] = d;
+      const uint32_t d = (uint32_t)((k >> shift) & DM);
+      dg[it >> 1] |= d << (16 * (it & 1));
       kout[s_base[d] + (uint64_t)(li - (int)s_dstart[d])] = k;
     }
   }
@@ -350,7 +369,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_onesweep(const uint64_t* __re
   for (int it = 0; it < RS_ITEMS; ++it) {
     const int li = it * RS_THREADS + tid;
     if (li < nvalid) {
-      const uint32_t d = dg[it];
+      const uint32_t d = (dg[it >> 1] >> (16 * (it & 1))) & 0xffffu;
       vout[s_base[d] + (uint64_t)(li - (int)s_dstart[d])] = s_vstage[li];
     }
   }
@@ -361,45 +380,68 @@ __global__ void k_rs_clear(uint64_t* __restrict__ p, int64_t count) {
     p[i] = 0;
 }
 
+// Digit widths: 8-bit passes, or 9-bit ones where that saves a pass (a key of
+// at most 45 bits: 42 = 9 + 9 + 8 + 8 + 8); lowest bits first
+RsPlan rs_plan(int lo, int bits) {
+  RsPlan pl{};
+  const int n8 = (bits + 7) / 8, n9 = (bits + 8) / 9;
+  pl.npass = std::min(n8, n9);
+  const int wide = pl.npass < n8 ? std::max(0, bits - 8 * pl.npass) : 0;  // passes that take 9 bits
+  int sh = lo;
+  for (int p = 0; p < pl.npass; ++p) {
+    pl.shift[p] = sh;
+    pl.width[p] = p < wide ? 9 : 8;
+    sh += pl.width[p];
+  }
+  return pl;
+}
+
 template <typename V>
 int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bits, uint64_t* k2, V* v2, int lo,
                     uint64_t** k_res, V** v_res) {
   if (k_res) *k_res = keys;
   if (v_res) *v_res = vals;
   if (n <= 1 || bits <= 0) return LIGHTNE_OK;
-  const int npass = (bits + 7) / 8;
-  if (npass > RS_MAX_PASSES)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "radix sort of %d bits (at most %d)", bits, 8 * RS_MAX_PASSES);
+  const RsPlan pl = rs_plan(lo, bits);
+  if (pl.npass > RS_MAX_PASSES || pl.shift[pl.npass - 1] + pl.width[pl.npass - 1] > 64)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "radix sort of bits %d..%d (at most %d passes of a 64-bit key)", lo,
+                    lo + bits, RS_MAX_PASSES);
   const int64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   if (!k2 || !v2) {
     k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
     v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
   }
-  // one zero-filled block: look-back words (ntiles x 256), digit totals and
-  // starts (npass x 256 each), tile counters, error flag
-  const int64_t nstat = ntiles * 256;
-  const int64_t nwords = nstat + 2 * RS_MAX_PASSES * 256 + RS_MAX_PASSES + 1;
+  // one zero-filled block: look-back words (ntiles x digits), digit totals
+  // and starts (passes x 512 each), tile counters, error flag
+  const int64_t nstat = ntiles * RS_MAX_DIGITS;
+  const int64_t nwords = nstat + 2 * RS_MAX_PASSES * RS_MAX_DIGITS + RS_MAX_PASSES + 1;
   auto* state = (uint64_t*)ctx_buf(ctx, "rs_state", nwords * sizeof(uint64_t));
   if (!k2 || !v2 || !state) return LIGHTNE_ERR_OOM;
   uint64_t* status = state;
   auto* ghist = (unsigned long long*)(state + nstat);
-  uint64_t* dstart = state + nstat + RS_MAX_PASSES * 256;
-  auto* tile_ctr = (uint32_t*)(state + nstat + 2 * RS_MAX_PASSES * 256);
+  uint64_t* dstart = state + nstat + RS_MAX_PASSES * RS_MAX_DIGITS;
+  auto* tile_ctr = (uint32_t*)(state + nstat + 2 * RS_MAX_PASSES * RS_MAX_DIGITS);
   auto* err = (uint32_t*)(state + nwords - 1);
   const hipStream_t s = ctx->stream;
   hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)std::min<int64_t>(4096, (nwords + 255) / 256)), dim3(256), 0, s, state,
                      nwords);
   LN_LAUNCH(ctx);
   hipLaunchKernelGGL(k_rs_hist_all, dim3((unsigned)std::min<int64_t>(RS_HIST_BLOCKS, ntiles)), dim3(RS_THREADS), 0, s,
-                     keys, n, lo, npass, ghist);
+                     keys, n, pl, ghist);
   LN_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_rs_digit_starts, dim3(npass), dim3(256), 0, s, (const unsigned long long*)ghist, dstart);
+  hipLaunchKernelGGL(k_rs_digit_starts, dim3(pl.npass), dim3(RS_MAX_DIGITS), 0, s, (const unsigned long long*)ghist,
+                     dstart);
   LN_LAUNCH(ctx);
   uint64_t *ka = keys, *kb = k2;
   V *va = vals, *vb = v2;
-  for (int p = 0; p < npass; ++p) {
-    hipLaunchKernelGGL(k_rs_onesweep<V>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n, lo + 8 * p,
-                       (const uint64_t*)(dstart + p * 256), status, tile_ctr + p, (uint32_t)(p + 1), err);
+  for (int p = 0; p < pl.npass; ++p) {
+    const uint64_t* ds = dstart + p * RS_MAX_DIGITS;
+    if (pl.width[p] == 9)
+      hipLaunchKernelGGL((k_rs_onesweep<V, 9>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n,
+                         pl.shift[p], ds, status, tile_ctr + p, (uint32_t)(p + 1), err);
+    else
+      hipLaunchKernelGGL((k_rs_onesweep<V, 8>), dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n,
+                         pl.shift[p], ds, status, tile_ctr + p, (uint32_t)(p + 1), err);
     LN_LAUNCH(ctx);
     std::swap(ka, kb);
     std::swap(va, vb);
