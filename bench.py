@@ -412,7 +412,7 @@ def main():
         "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
         "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
         "table_capacity": sp["table_capacity"], "table_slot_bytes": 8 if sp["table_layout"] == 2 else 16,
-        "csr_row_ranges": sp["csr_parts"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
+        "csr_row_ranges": sp["csr_parts"], "csr_value_cache": bool(sp["value_cache"]), "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "adjoint": adjoint, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -120,6 +120,11 @@ typedef struct {
   int32_t csr_parts;        /* 0 auto: the CSR is built by row ranges only
                                when one global sort does not fit; k >= 2:
                                at least k ranges                           */
+  int32_t value_cache;      /* pipeline only: 0 auto (the count pass caches
+                               each slot's CSR value where memory allows), 1
+                               always (the workspace grows for it), -1 never;
+                               the CSR is the same either way               */
+  int32_t reserved;
 } lightne_sparsify_params;
 
 typedef struct {
@@ -140,6 +145,9 @@ typedef struct {
   double t_sample, t_table, t_csr;
   uint32_t table_layout;   /* 1 = 16-byte slots, 2 = 8-byte slots          */
   uint32_t csr_parts;      /* row ranges the CSR was built in (1 = one sort) */
+  uint32_t value_cache;    /* 1: the CSR values came from the count pass's
+                              per-slot cache, not a second read of degrees  */
+  uint32_t reserved;
 } lightne_sparsify_stats;
 
 /* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */

@@ -53,6 +53,7 @@ class SparsifyParams(C.Structure):
         ("walk_len", C.c_uint64), ("negative", C.c_uint64), ("sample", C.c_int32), ("upper", C.c_int32),
         ("sample_ratio", C.c_float), ("mem_ratio", C.c_float), ("tablesz", C.c_uint64), ("seed", C.c_uint64),
         ("max_table_bytes", C.c_uint64), ("table_layout", C.c_int32), ("csr_parts", C.c_int32),
+        ("value_cache", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -64,7 +65,8 @@ class SparsifyStats(C.Structure):
         ("walk_steps", C.c_uint64), ("distinct_keys", C.c_uint64),
         ("M_exact", C.c_double), ("factor", C.c_double), ("kept_upper", C.c_uint64), ("nnz", C.c_uint64),
         ("t_sample", C.c_double), ("t_table", C.c_double), ("t_csr", C.c_double),
-        ("table_layout", C.c_uint32), ("csr_parts", C.c_uint32),
+        ("table_layout", C.c_uint32), ("csr_parts", C.c_uint32), ("value_cache", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

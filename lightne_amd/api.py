@@ -250,13 +250,15 @@ def sample_scalars(n: int, m: int, walk_len: int = 10, sample_ratio: float = 1.0
 
 
 def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                     max_table_bytes, table_layout=0, csr_parts=0) -> SparsifyParams:
+                     max_table_bytes, table_layout=0, csr_parts=0, value_cache=0) -> SparsifyParams:
     """``table_layout``: 0 auto, 1 16-byte slots, 2 8-byte slots (n <= 2^26).
-    ``csr_parts``: 0 auto, k >= 2 builds the CSR in at least k row ranges."""
+    ``csr_parts``: 0 auto, k >= 2 builds the CSR in at least k row ranges.
+    ``value_cache`` (pipeline): 0 auto, 1 always, -1 never cache the count
+    pass's per-slot CSR values for the emission (same CSR either way)."""
     return SparsifyParams(walk_len=walk_len, negative=negative, sample=int(bool(sample)), upper=int(bool(upper)),
                           sample_ratio=sample_ratio, mem_ratio=mem_ratio, tablesz=table_size,
                           seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes,
-                          table_layout=table_layout, csr_parts=csr_parts)
+                          table_layout=table_layout, csr_parts=csr_parts, value_cache=value_cache)
 
 
 @dataclasses.dataclass
@@ -422,14 +424,14 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
            random_project_only: bool = False, seed: int = 0, svd_seed: int = 777, power_iters: int = 1,
            precision: int = FP32, max_table_bytes: int = 0, sparse_project: bool = False,
            sparse_project_s: float = 100.0, table_layout: int = 0, csr_parts: int = 0,
-           ctx: Optional[Context] = None):
+           value_cache: int = 0, ctx: Optional[Context] = None):
     """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats).
     ``sparse_project_s`` defaults to 100 as LightNE.cc:50 does."""
     del walks_per_edge, analyze
     torch = _torch()
     ctx = ctx or context()
     sp = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                          max_table_bytes, table_layout, csr_parts)
+                          max_table_bytes, table_layout, csr_parts, value_cache)
     rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
                     random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed,
                     sparse_project=int(bool(sparse_project)), sparse_project_s=sparse_project_s)

@@ -425,6 +425,11 @@ struct TruncArgs {
   int colbits;
   uint64_t* skey;
   float* sval;
+  // per-slot value cache: the count pass writes every slot's CSR value
+  // ((float)log(adj) when kept, else 0) to lw_out; an emission pass given
+  // lw_in reads it instead of the two degrees
+  float* lw_out;
+  const float* lw_in;
 };
 
 // PathEmbed.h:203-210: adjusted weight of a key, evaluated in the reference's
@@ -456,19 +461,28 @@ constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in 
 
 // CSR entries of a loaded slot whose row lies in [r0, r1): 0, 1 or 2 (eu:
 // row u, ev: row v); trunc-log keeps a key iff log(adj) > 0
-// (PathEmbed.h:203-217).  Slots with neither endpoint in the range skip the
-// degree reads.
+// (PathEmbed.h:203-217), and the entry's value is (float)log(adj) (lwv,
+// computed when want_value).  Slots with neither endpoint in the range skip
+// the degree reads; with the value cache there are none (a kept key's cached
+// value is > 0: log(adj) >= log(1 + 2^-52) > 0 stays positive as a float).
 __device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const RawSlot& r, uint32_t r0, uint32_t r1,
-                                         uint32_t& u, uint32_t& v, double& adj, bool& eu, bool& ev) {
+                                         uint32_t& u, uint32_t& v, float& lwv, bool want_value, bool& eu, bool& ev) {
   uint64_t cnt;
   eu = ev = false;
   if (!raw_get(a.t, r, u, v, cnt)) return 0;
   const bool iu = u >= r0 && u < r1;
   const bool iv = !a.upper && u != v && v >= r0 && v < r1;
   if (!iu && !iv) return 0;
-  adj = trunc_adj(a, i, u, v, cnt);
-  // log(adj) > 0  <=>  adj > 1 for any faithful log
-  if (!(adj > 1.0)) return 0;
+  if (a.lw_in) {
+    const float l = a.lw_in[i];
+    if (!(l > 0.f)) return 0;
+    lwv = l;
+  } else {
+    const double adj = trunc_adj(a, i, u, v, cnt);
+    // log(adj) > 0  <=>  adj > 1 for any faithful log
+    if (!(adj > 1.0)) return 0;
+    if (want_value) lwv = (float)log(adj);
+  }
   eu = iu;
   ev = iv;
   return (int)iu + (int)iv;
@@ -498,11 +512,12 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
       uint32_t u, v;
-      double adj;
+      float lwv = 0.f;
       bool eu, ev;
-      const int c = emit_slot(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, adj, eu, ev);
+      const int c = emit_slot(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, lwv, a.lw_out != nullptr, eu, ev);
       e += c;
       kk += c > 0;
+      if (a.lw_out && ib + j < b1) a.lw_out[ib + j] = c > 0 ? lwv : 0.f;
       if (HIST) {
         if (eu) atomicAdd(&s_h[u >> shift], 1u);
         if (ev) atomicAdd(&s_h[v >> shift], 1u);
@@ -560,11 +575,12 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
     uint32_t uu[TR_ITEMS], vv[TR_ITEMS];
-    double adj[TR_ITEMS];
+    float lw[TR_ITEMS];
     bool eu[TR_ITEMS], ev[TR_ITEMS];
     int c = 0;
 #pragma unroll
-    for (int j = 0; j < TR_ITEMS; ++j) c += emit_slot(a, ib + j, rs[j], r0, r1, uu[j], vv[j], adj[j], eu[j], ev[j]);
+    for (int j = 0; j < TR_ITEMS; ++j)
+      c += emit_slot(a, ib + j, rs[j], r0, r1, uu[j], vv[j], lw[j], true, eu[j], ev[j]);
     // block-wide exclusive scan of the threads' counts
     int incl = c;
 #pragma unroll
@@ -584,15 +600,14 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
       if (!eu[j] && !ev[j]) continue;
-      const float lw = (float)log(adj[j]);
       if (eu[j]) {
         a.skey[pos] = ((uint64_t)(uu[j] - r0) << a.colbits) | vv[j];
-        a.sval[pos] = lw;
+        a.sval[pos] = lw[j];
         ++pos;
       }
       if (ev[j]) {
         a.skey[pos] = ((uint64_t)(vv[j] - r0) << a.colbits) | uu[j];
-        a.sval[pos] = lw;
+        a.sval[pos] = lw[j];
         ++pos;
       }
     }
@@ -860,6 +875,19 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
   const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
   uint32_t* bh = nullptr;  // per-block row histograms, when taken
+  // Value cache: the count pass leaves every slot's CSR value in a float
+  // array right behind the table in the workspace (context-owned builds
+  // only), so the emission passes read it instead of two degrees at random
+  // places.  Written whenever the workspace holds it beside the table, or as
+  // a buffer of its own for a single-sort build that surely fits beside it;
+  // the layout chosen after the count keeps it only where it needs no extra
+  // memory (value_cache = 1: always, growing the workspace; -1: never).
+  const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
+  const size_t lw_bytes = al(tv.cap * sizeof(float));
+  const bool lw_force = p->value_cache > 0;
+  bool lw_allowed = ctx_owned && p->value_cache >= 0;
+  float* lw = nullptr;
+  bool lw_in_ws = false;  // else a buffer of its own (single-sort builds only)
   // totals (PathEmbed.h:175-200), then per-block counts of the kept entries
   // over contiguous slot ranges and their scan; repeated after a workspace
   // regrowth re-ran the (deterministic) sampler
@@ -880,6 +908,35 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     ta.factor = st.factor;
     ta.upper = p->upper;
     ta.colbits = colbits;
+    lw = nullptr;
+    lw_in_ws = false;
+    if (lw_allowed) {
+      const size_t nb = 2 * (size_t)distinct + 1;  // nnz <= 2 x kept <= 2 x distinct
+      const size_t csr_b = al((g->n + 1) * 8) + 2 * al(nb * 4);
+      const size_t ws_b = ctx->bufs["ws"].bytes;
+      size_t fb = 0, tb = 0;
+      LN_HIP(ctx, hipMemGetInfo(&fb, &tb));
+      const size_t mg = ((size_t)3 << 30) + tb / 100;
+      const size_t av = fb + ctx_held_bytes(ctx) > mg ? fb + ctx_held_bytes(ctx) - mg : 0;
+      const size_t tmp_b = (al(nb * 8) + nb * 4 <= tab_bytes) ? 0 : al(nb * 8) + al(nb * 4);
+      const size_t single_b = std::max(tab_bytes + al(nb * 8), front_reserve) + csr_b + tmp_b;
+      const bool single = p->csr_parts < 2 && single_b <= av;  // one sort will fit
+      if (single) {
+        // behind the table when the keys and the CSR still fit behind it,
+        // else a buffer of its own when memory holds both
+        if (std::max(tab_bytes + lw_bytes + al(nb * 8), front_reserve) + csr_b <= ws_b) {
+          lw = (float*)((char*)ctx->bufs["ws"].p + tab_bytes);
+          lw_in_ws = true;
+        } else if (lw_force || single_b + lw_bytes + tb / 8 <= av) {  // an eighth of the device kept for stages 2-3
+          lw = (float*)ctx_buf(ctx, "s1_lw", lw_bytes);
+          if (!lw) ctx->err.clear();  // no room after all: the emission reads the degrees
+        }
+      } else if (tab_bytes + lw_bytes <= ws_b) {
+        lw = (float*)((char*)ctx->bufs["ws"].p + tab_bytes);  // row ranges: behind the table
+        lw_in_ws = true;
+      }
+    }
+    ta.lw_out = lw;
     const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
     if (hist) {
       bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
@@ -906,7 +963,6 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
   // tables that may need a build by row ranges take the row histograms now
   LN_CHECK(count_pass(p->csr_parts >= 2 || tv.cap >= (1ull << 30)));
   const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
-  const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
   const size_t csr_bytes = al((g->n + 1) * 8) + 2 * al(nz * 4);
   // Memory: one global sort needs the table, 8-byte keys, the CSR and (unless
   // they fit in the dead table) 12 bytes of ping-pong buffers per entry.  When
@@ -932,10 +988,16 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     // allocation.  Otherwise the keys (and a context-owned CSR) are buffers of
     // their own.
     const size_t ws_bytes = ctx->bufs["ws"].bytes;
-    const size_t front = std::max(tab_bytes + al(nz * 8), front_reserve);
+    // auto: keep an in-workspace cache only where the keys and the CSR still
+    // fit the workspace behind it
+    if (lw && lw_in_ws && !lw_force &&
+        std::max(tab_bytes + lw_bytes + al(nz * 8), front_reserve) + csr_bytes > ws_bytes)
+      lw = nullptr;
+    const size_t lwb = lw && lw_in_ws ? lw_bytes : 0;
+    const size_t front = std::max(tab_bytes + lwb + al(nz * 8), front_reserve);
     const bool in_ws = ctx_owned && front + csr_bytes <= ws_bytes;
     char* ws = (char*)ctx->bufs["ws"].p;  // the table starts the workspace
-    uint64_t* skey = in_ws ? (uint64_t*)(ws + tab_bytes) : (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
+    uint64_t* skey = in_ws ? (uint64_t*)(ws + tab_bytes + lwb) : (uint64_t*)ctx_buf(ctx, "s1_skey", nz * 8);
     if (!skey) return LIGHTNE_ERR_OOM;
     if (in_ws) {
       char* back = ws + (ws_bytes - csr_bytes);
@@ -957,6 +1019,9 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     }
     ta.skey = skey;
     ta.sval = val;
+    ta.lw_out = nullptr;
+    ta.lw_in = lw;
+    st.value_cache = lw != nullptr;
     const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
     hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
                        (const uint64_t*)blk_off);
@@ -1000,34 +1065,44 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
       return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row histogram holds %llu entries, the count pass %lld",
                       (unsigned long long)hsum, (long long)nnz);
     // entries per range: forced count, or what the memory left beside the
-    // table and the CSR allows (24 bytes per entry: keys, values, ping-pong).
-    // Context-owned, the workspace is rebuilt as [table | ranges | CSR];
-    // otherwise the range buffers use the workspace's room behind the table
-    // or memory of their own.
+    // table (and value cache) and the CSR allows (24 bytes per entry: keys,
+    // values, ping-pong).  Context-owned, the workspace is rebuilt as
+    // [table | cache | ranges | CSR]; otherwise the range buffers use the
+    // workspace's room behind the table or memory of their own.
     const size_t ws_now = ctx->bufs["ws"].bytes;
-    size_t part_room;
-    if (ctx_owned)
-      part_room = avail > tab_bytes + csr_bytes ? avail - tab_bytes - csr_bytes : 0;
-    else
-      part_room = std::max(ws_now - tab_bytes, avail > ws_now + csr_bytes ? avail - ws_now - csr_bytes : 0);
-    uint64_t cap_e;
-    if (p->csr_parts >= 2) {
-      cap_e = ((uint64_t)nnz + p->csr_parts - 1) / p->csr_parts;
-    } else {
-      cap_e = part_room > 4096 ? (part_room - 4096) / 24 : 0;
-      cap_e = std::min<uint64_t>(cap_e, ((uint64_t)nnz + 3) / 4);  // at least 4 ranges: memory for later stages
-    }
-    cap_e = std::max<uint64_t>(cap_e, hmax);
-    cap_e = std::max<uint64_t>(cap_e, 1);
-    if (p->csr_parts < 2 && 24 * cap_e + 4096 > part_room)
-      return ctx_fail(ctx, LIGHTNE_ERR_OOM,
-                      "sparsifier of %lld entries: %.1f GB of table and %.1f GB of CSR leave %.1f GB for sort "
-                      "buffers, the densest row range needs %.1f GB",
-                      (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
-    // ranges of whole buckets, at most 2^22 rows each (6 radix passes at n = 2^26)
+    // ranges of whole buckets, at most 2^22 rows each (6 radix passes at
+    // n = 2^26), sized for the room left beside the table, the value cache
+    // (lwb bytes) and the CSR
     std::vector<std::pair<int, int>> rng;  // bucket ranges
     std::vector<uint64_t> rcnt;
-    {
+    size_t pe = 1, part_bytes = 0;
+    // in_ws: the ranges must fit the workspace as it is
+    auto plan = [&](size_t lwb, bool in_ws) -> int {
+      size_t part_room;
+      if (ctx_owned)
+        part_room = avail > tab_bytes + lwb + csr_bytes ? avail - tab_bytes - lwb - csr_bytes : 0;
+      else
+        part_room = std::max(ws_now - tab_bytes, avail > ws_now + csr_bytes ? avail - ws_now - csr_bytes : 0);
+      if (in_ws) {
+        const size_t fixed = std::max(tab_bytes + lwb, front_reserve) + csr_bytes;
+        part_room = std::min(part_room, ws_now > fixed ? ws_now - fixed : 0);
+      }
+      uint64_t cap_e;
+      if (p->csr_parts >= 2) {
+        cap_e = ((uint64_t)nnz + p->csr_parts - 1) / p->csr_parts;
+      } else {
+        cap_e = part_room > 4096 ? (part_room - 4096) / 24 : 0;
+        cap_e = std::min<uint64_t>(cap_e, ((uint64_t)nnz + 3) / 4);  // at least 4 ranges: memory for later stages
+      }
+      cap_e = std::max<uint64_t>(cap_e, hmax);
+      cap_e = std::max<uint64_t>(cap_e, 1);
+      if (p->csr_parts < 2 && 24 * cap_e + 4096 > part_room)
+        return ctx_fail(ctx, LIGHTNE_ERR_OOM,
+                        "sparsifier of %lld entries: %.1f GB of table and %.1f GB of CSR leave %.1f GB for sort "
+                        "buffers, the densest row range needs %.1f GB",
+                        (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
+      rng.clear();
+      rcnt.clear();
       const int max_buckets = std::max(1, (1 << 22) >> shift);
       int lo = 0;
       uint64_t acc = 0;
@@ -1042,19 +1117,41 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
       }
       rng.push_back({lo, nbuck});
       rcnt.push_back(acc);
+      uint64_t pmax = 0;
+      for (uint64_t c : rcnt) pmax = std::max(pmax, c);
+      pe = (size_t)std::max<uint64_t>(pmax, 1);
+      part_bytes = 2 * al(pe * 8) + 2 * al(pe * 4);
+      return LIGHTNE_OK;
+    };
+    // keep the value cache only inside the workspace and where the plan with
+    // it fits the workspace as it is (no regrowth, which would re-run the
+    // sampler)
+    if (lw && !lw_in_ws) lw = nullptr;
+    bool keep_lw = ctx_owned && ((lw != nullptr) || (lw_force && lw_allowed));
+    if (keep_lw) {
+      const int rc = plan(lw_bytes, !lw_force);
+      if (rc != LIGHTNE_OK) {
+        if (lw_force) return rc;
+        keep_lw = false;
+      } else if (!lw_force && std::max(tab_bytes + lw_bytes + part_bytes, front_reserve) + csr_bytes > ws_now) {
+        keep_lw = false;
+      }
     }
+    if (!keep_lw) {
+      lw = nullptr;
+      lw_allowed = false;  // a regrowth's count pass must not place it either
+      ctx->err.clear();
+      LN_CHECK(plan(0, false));
+    }
+    const size_t lwb = keep_lw ? lw_bytes : 0;
     st.csr_parts = (uint32_t)rng.size();
-    uint64_t pmax = 0;
-    for (uint64_t c : rcnt) pmax = std::max(pmax, c);
-    const size_t pe = (size_t)std::max<uint64_t>(pmax, 1);
-    const size_t part_bytes = 2 * al(pe * 8) + 2 * al(pe * 4);
-    // Context-owned: [table | range buffers | ... | CSR] in the workspace,
-    // which grows (and the deterministic sampler runs again) when it is too
-    // small.  Otherwise the range buffers follow the table when they fit and
-    // the CSR arrays belong to the handle.
+    // Context-owned: [table | value cache | range buffers | ... | CSR] in the
+    // workspace, which grows (and the deterministic sampler runs again) when
+    // it is too small.  Otherwise the range buffers follow the table when
+    // they fit and the CSR arrays belong to the handle.
     char* pbuf = nullptr;
     if (ctx_owned) {
-      const size_t need = std::max(tab_bytes + part_bytes, front_reserve) + csr_bytes;
+      const size_t need = std::max(tab_bytes + lwb + part_bytes, front_reserve) + csr_bytes;
       if (ctx->bufs["ws"].bytes < need) {
         if (p->csr_parts < 2 && need > avail)
           return ctx_fail(ctx, LIGHTNE_ERR_OOM, "stage 1 needs a %.1f GB workspace, %.1f GB are available",
@@ -1062,14 +1159,16 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
         if (!ctx_buf(ctx, "ws", need)) return LIGHTNE_ERR_OOM;
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
-        LN_CHECK(count_pass(true));
+        LN_CHECK(count_pass(true));  // places the cache behind the table iff keep_lw
+        if (keep_lw && !(lw && lw_in_ws))
+          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the value cache did not fit the grown workspace");
         if (st.nnz != keep.nnz || st.distinct_keys != keep.distinct_keys)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different table on its second run");
         st.csr_parts = keep.csr_parts;
       }
       char* ws = (char*)ctx->bufs["ws"].p;
       const size_t ws_bytes = ctx->bufs["ws"].bytes;
-      pbuf = ws + tab_bytes;
+      pbuf = ws + tab_bytes + lwb;
       char* back = ws + (ws_bytes - csr_bytes);
       rp = (int64_t*)back;
       col = (int32_t*)(back + al((g->n + 1) * 8));
@@ -1092,6 +1191,9 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     auto* vtmp = (float*)(pbuf + 2 * al(pe * 8) + al(pe * 4));
     ta.skey = pkey;
     ta.sval = pval;
+    ta.lw_out = nullptr;
+    ta.lw_in = lw;
+    st.value_cache = lw != nullptr;
     int64_t base = 0;
     for (size_t r = 0; r < rng.size(); ++r) {
       const uint64_t r0 = (uint64_t)rng[r].first << shift;

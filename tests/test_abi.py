@@ -35,6 +35,33 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= exported
 
 
+_STRUCTS = [("lightne_graph", _lib.Graph), ("lightne_sparsify_params", _lib.SparsifyParams),
+            ("lightne_sparsify_stats", _lib.SparsifyStats), ("lightne_rsvd_params", _lib.RsvdParams),
+            ("lightne_rsvd_stats", _lib.RsvdStats), ("lightne_prop_params", _lib.PropParams),
+            ("lightne_prop_stats", _lib.PropStats), ("lightne_netsmf_stats", _lib.NetsmfStats)]
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror has the C struct's size and field offsets (a C
+    program built against include/lightne.h prints them)."""
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "lightne.h"', "int main(void) {"]
+    for cname, py in _STRUCTS:
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(H.ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                          check=True).stdout.splitlines())
+    for cname, py in _STRUCTS:
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
+
+
 def test_library_is_a_gfx950_hip_binary():
     out = subprocess.run(["strings", _lib.LIB_PATH], capture_output=True, text=True).stdout
     assert "gfx950" in out

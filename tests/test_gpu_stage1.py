@@ -221,6 +221,28 @@ def test_netsmf_row_ranges_in_a_fresh_context(gpu):
         ctx2.close()
 
 
+@pytest.mark.parametrize("parts", [0, 6])
+def test_value_cache_gives_the_same_embedding(gpu, parts):
+    """The count pass's per-slot value cache gives the pipeline the CSR (so the
+    embedding) that reading the degrees again gives, bit for bit, in one sort
+    and by row ranges; auto mode gives the same."""
+    import torch
+
+    hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
+    g = hg.to_device()
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
+    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, csr_parts=parts, ctx=gpu)
+    e0, S0, st0 = L.NetSMF(g, value_cache=-1, **kw)
+    e1, S1, st1 = L.NetSMF(g, value_cache=1, **kw)
+    assert st0["sparsify"]["value_cache"] == 0 and st1["sparsify"]["value_cache"] == 1
+    assert st1["sparsify"]["nnz"] == st0["sparsify"]["nnz"]
+    np.testing.assert_array_equal(S1, S0)
+    assert torch.equal(e1, e0)
+    e2, S2, _ = L.NetSMF(g, **kw)
+    np.testing.assert_array_equal(S2, S0)
+    assert torch.equal(e2, e0)
+
+
 def test_csr_prefix_copy(gpu):
     """The entries of the first rows holding a fraction of the nonzeros come to
     the host alone (row_ptr whole); out-of-range counts are errors."""
