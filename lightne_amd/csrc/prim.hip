@@ -411,9 +411,12 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
     k2 = (uint64_t*)ctx_buf(ctx, "rs_keys", n * sizeof(uint64_t));
     v2 = (V*)ctx_buf(ctx, "rs_vals", n * sizeof(V));
   }
-  // one zero-filled block: look-back words (ntiles x digits), digit totals
-  // and starts (passes x 512 each), tile counters, error flag
-  const int64_t nstat = ntiles * RS_MAX_DIGITS;
+  // one zero-filled block: look-back words (ntiles x the widest pass's
+  // digits), digit totals and starts (passes x 512 each), tile counters,
+  // error flag
+  int maxw = 8;
+  for (int p = 0; p < pl.npass; ++p) maxw = std::max(maxw, pl.width[p]);
+  const int64_t nstat = ntiles * ((int64_t)1 << maxw);
   const int64_t nwords = nstat + 2 * RS_MAX_PASSES * RS_MAX_DIGITS + RS_MAX_PASSES + 1;
   auto* state = (uint64_t*)ctx_buf(ctx, "rs_state", nwords * sizeof(uint64_t));
   if (!k2 || !v2 || !state) return LIGHTNE_ERR_OOM;
