@@ -163,6 +163,8 @@ int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t
 int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp = nullptr,
                      float* v_tmp = nullptr, int lo_bit = 0, uint64_t** k_res = nullptr, float** v_res = nullptr);
 int radix_sort_pairs_u32(lightne_ctx* ctx, uint64_t* keys, uint32_t* vals, int64_t n, int bits);
+// digit passes radix_sort_pairs makes over a key of `bits` bits
+int radix_sort_passes(int bits);
 
 // ---------------------------------------------------------------------------
 // SpMM (spmm.hip)

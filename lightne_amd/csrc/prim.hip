@@ -470,6 +470,7 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
 int scan_u32_to_u64(lightne_ctx* ctx, const uint32_t* in, uint64_t* out, int64_t n) {
   return scan_impl<uint32_t>(ctx, in, out, n, 0);
 }
+int radix_sort_passes(int bits) { return bits > 0 ? rs_plan(0, bits).npass : 0; }
 int radix_sort_pairs(lightne_ctx* ctx, uint64_t* keys, float* vals, int64_t n, int bits, uint64_t* k_tmp,
                      float* v_tmp, int lo_bit, uint64_t** k_res, float** v_res) {
   return radix_sort_impl<float>(ctx, keys, vals, n, bits, k_tmp, v_tmp, lo_bit, k_res, v_res);

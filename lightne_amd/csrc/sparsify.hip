@@ -1017,8 +1017,26 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
       col = (int32_t*)mcol.p;
       val = (float*)mval.p;
     }
+    // The sort ping-pongs between (keys, values) and two temporaries, which
+    // reuse the dead table when they fit there.  Otherwise they are buffers
+    // of their own, alive during the emission too: with an odd number of
+    // passes the emission then writes the values into the temporary, so the
+    // last pass leaves them in the CSR's value array (no copy).
+    const size_t kbytes = al(nz * 8);
+    uint64_t* k_tmp = nullptr;
+    float* v_tmp = nullptr;
+    float* v_emit = val;
+    if (kbytes + nz * 4 <= tab_bytes) {
+      k_tmp = (uint64_t*)ws;
+      v_tmp = (float*)(ws + kbytes);
+    } else if (radix_sort_passes(2 * colbits) % 2 == 1) {
+      k_tmp = (uint64_t*)ctx_buf(ctx, "rs_keys", nz * 8);
+      v_emit = (float*)ctx_buf(ctx, "rs_vals", nz * 4);
+      if (!k_tmp || !v_emit) return LIGHTNE_ERR_OOM;
+      v_tmp = val;
+    }
     ta.skey = skey;
-    ta.sval = val;
+    ta.sval = v_emit;
     ta.lw_out = nullptr;
     ta.lw_in = lw;
     st.value_cache = lw != nullptr;
@@ -1026,21 +1044,12 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
                        (const uint64_t*)blk_off);
     LN_LAUNCH(ctx);
-    // the table is dead once the kept entries are out: the sort's ping-pong
-    // buffers reuse its memory when they fit there
-    const size_t kbytes = al(nz * 8);
-    uint64_t* k_tmp = nullptr;
-    float* v_tmp = nullptr;
-    if (kbytes + nz * 4 <= tab_bytes) {
-      k_tmp = (uint64_t*)ws;
-      v_tmp = (float*)(ws + kbytes);
-    }
     // rows sorted by column: a key's table slot depends on the order concurrent
     // inserts claimed slots, so the column order is what makes the matrix (and
     // every summation over a row) deterministic
     uint64_t* kres = nullptr;
     float* vres = nullptr;
-    LN_CHECK(radix_sort_pairs(ctx, skey, val, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
+    LN_CHECK(radix_sort_pairs(ctx, skey, v_emit, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
     hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, nnz,
                        g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val, (int64_t)0);
     LN_LAUNCH(ctx);
