@@ -158,20 +158,43 @@ struct RsPlan {
 };
 
 // per-digit totals of every pass in one read of the keys (pass p's counters
-// at ghist[p * RS_MAX_DIGITS ..])
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, RsPlan pl,
+// at ghist[p * RS_MAX_DIGITS ..]).  Each wave counts into its own LDS copy
+// (the high digits of row-major keys are skewed: shared counters would
+// serialise on the hub rows' digit), four keys per thread in flight.
+constexpr int HA_THREADS = 256;
+constexpr int HA_WAVES = HA_THREADS / 64;
+constexpr int HA_ITEMS = 4;
+
+__global__ __launch_bounds__(HA_THREADS) void k_rs_hist_all(const uint64_t* __restrict__ keys, int64_t n, RsPlan pl,
                                                             unsigned long long* __restrict__ ghist) {
-  __shared__ uint32_t s_h[RS_MAX_PASSES * RS_MAX_DIGITS];
-  for (int i = threadIdx.x; i < pl.npass * RS_MAX_DIGITS; i += RS_THREADS) s_h[i] = 0;
+  extern __shared__ uint32_t s_hw[];  // [HA_WAVES][npass * RS_MAX_DIGITS]
+  const int tid = threadIdx.x, wid = tid >> 6;
+  const int W = pl.npass * RS_MAX_DIGITS;
+  for (int i = tid; i < HA_WAVES * W; i += HA_THREADS) s_hw[i] = 0;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * RS_THREADS) {
-    const uint64_t k = keys[i];
-    for (int p = 0; p < pl.npass; ++p)
-      atomicAdd(&s_h[p * RS_MAX_DIGITS + (uint32_t)((k >> pl.shift[p]) & ((1u << pl.width[p]) - 1u))], 1u);
+  uint32_t* my = s_hw + wid * W;
+  const int64_t step = (int64_t)gridDim.x * HA_THREADS * HA_ITEMS;
+  for (int64_t b = (int64_t)blockIdx.x * HA_THREADS * HA_ITEMS + tid; b < n; b += step) {
+    uint64_t k[HA_ITEMS];
+#pragma unroll
+    for (int j = 0; j < HA_ITEMS; ++j) {
+      const int64_t i = b + (int64_t)j * HA_THREADS;
+      k[j] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < HA_ITEMS; ++j) {
+      if (b + (int64_t)j * HA_THREADS >= n) continue;
+      for (int p = 0; p < pl.npass; ++p)
+        atomicAdd(&my[p * RS_MAX_DIGITS + (uint32_t)((k[j] >> pl.shift[p]) & ((1u << pl.width[p]) - 1u))], 1u);
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < pl.npass * RS_MAX_DIGITS; i += RS_THREADS)
-    if (s_h[i]) atomicAdd(&ghist[i], (unsigned long long)s_h[i]);
+  for (int i = tid; i < W; i += HA_THREADS) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < HA_WAVES; ++w) t += s_hw[w * W + i];
+    if (t) atomicAdd(&ghist[i], (unsigned long long)t);
+  }
 }
 
 // block p: exclusive scan of pass p's digit totals (512 slots; the unused
@@ -429,8 +452,11 @@ int radix_sort_impl(lightne_ctx* ctx, uint64_t* keys, V* vals, int64_t n, int bi
   hipLaunchKernelGGL(k_rs_clear, dim3((unsigned)std::min<int64_t>(4096, (nwords + 255) / 256)), dim3(256), 0, s, state,
                      nwords);
   LN_LAUNCH(ctx);
-  hipLaunchKernelGGL(k_rs_hist_all, dim3((unsigned)std::min<int64_t>(RS_HIST_BLOCKS, ntiles)), dim3(RS_THREADS), 0, s,
-                     keys, n, pl, ghist);
+  hipLaunchKernelGGL(k_rs_hist_all,
+                     dim3((unsigned)std::min<int64_t>(RS_HIST_BLOCKS, (n + HA_THREADS * HA_ITEMS - 1) /
+                                                                          (HA_THREADS * HA_ITEMS))),
+                     dim3(HA_THREADS), (size_t)HA_WAVES * pl.npass * RS_MAX_DIGITS * sizeof(uint32_t), s, keys, n, pl,
+                     ghist);
   LN_LAUNCH(ctx);
   hipLaunchKernelGGL(k_rs_digit_starts, dim3(pl.npass), dim3(RS_MAX_DIGITS), 0, s, (const unsigned long long*)ghist,
                      dstart);
