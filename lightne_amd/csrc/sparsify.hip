@@ -624,12 +624,18 @@ __global__ void k_csr_finish(const uint64_t* __restrict__ skey, int64_t nnz, int
                              const float* __restrict__ vin = nullptr, float* __restrict__ vout = nullptr,
                              int64_t base = 0) {
   const uint64_t cmask = (colbits >= 64) ? ~0ull : ((1ull << colbits) - 1);
+  const int lane = threadIdx.x & 63;
+  // a wave covers 64 consecutive i (blockDim is a multiple of 64), so the
+  // previous key comes from the lane below; lane 0 loads its own
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= nnz; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = (i < nnz) ? (int64_t)(skey[i] >> colbits) : n;
-    const int64_t rp = (i == 0) ? -1 : (int64_t)(skey[i - 1] >> colbits);
+    const uint64_t k = (i < nnz) ? skey[i] : 0ull;
+    uint64_t kp = __shfl_up(k, 1, 64);
+    if (lane == 0) kp = (i > 0) ? skey[i - 1] : 0ull;
+    const int64_t r = (i < nnz) ? (int64_t)(k >> colbits) : n;
+    const int64_t rp = (i == 0) ? -1 : (int64_t)(kp >> colbits);
     for (int64_t rr = rp + 1; rr <= r && rr <= n; ++rr) row_ptr[rr] = base + i;
     if (i < nnz) {
-      col[i] = (int32_t)(skey[i] & cmask);
+      col[i] = (int32_t)(k & cmask);
       if (vin) vout[i] = vin[i];
     }
   }
