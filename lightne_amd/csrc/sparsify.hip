@@ -458,6 +458,14 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
 
 constexpr int HIST_BUCKETS = 4096;
 constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in flight
+static_assert(TR_ITEMS == 4, "the value cache is written as one float4 per thread");
+
+// slots per block of the trunc-log passes: a multiple of TR_ITEMS, so every
+// thread's TR_ITEMS slots start 16-byte aligned in the value cache
+inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
+  const uint64_t pb = (cap + nblk - 1) / nblk;
+  return (pb + TR_ITEMS - 1) / TR_ITEMS * TR_ITEMS;
+}
 
 // CSR entries of a loaded slot whose row lies in [r0, r1): 0, 1 or 2 (eu:
 // row u, ev: row v); trunc-log keeps a key iff log(adj) > 0
@@ -509,6 +517,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
     RawSlot rs[TR_ITEMS];
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
+    float lwo[TR_ITEMS];
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
       uint32_t u, v;
@@ -517,10 +526,21 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
       const int c = emit_slot(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, lwv, a.lw_out != nullptr, eu, ev);
       e += c;
       kk += c > 0;
-      if (a.lw_out && ib + j < b1) a.lw_out[ib + j] = c > 0 ? lwv : 0.f;
+      lwo[j] = c > 0 ? lwv : 0.f;
       if (HIST) {
         if (eu) atomicAdd(&s_h[u >> shift], 1u);
         if (ev) atomicAdd(&s_h[v >> shift], 1u);
+      }
+    }
+    // the value cache: one 16-byte store per thread (a wave writes 1 KB
+    // contiguously; ib is a multiple of TR_ITEMS), scalar stores at the tail
+    if (a.lw_out) {
+      if (ib + TR_ITEMS <= b1) {
+        *reinterpret_cast<float4*>(a.lw_out + ib) = make_float4(lwo[0], lwo[1], lwo[2], lwo[3]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < TR_ITEMS; ++j)
+          if (ib + j < b1) a.lw_out[ib + j] = lwo[j];
       }
     }
   }
@@ -943,7 +963,7 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
       }
     }
     ta.lw_out = lw;
-    const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+    const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     if (hist) {
       bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
       if (!bh) return LIGHTNE_ERR_OOM;
@@ -1046,7 +1066,7 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
     ta.lw_out = nullptr;
     ta.lw_in = lw;
     st.value_cache = lw != nullptr;
-    const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+    const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
                        (const uint64_t*)blk_off);
     LN_LAUNCH(ctx);
@@ -1217,7 +1237,7 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
       uint64_t* kres = pkey;
       float* vres = pval;
       if (cnt > 0) {
-        const uint64_t per_block = (tv.cap + nblk - 1) / nblk;
+        const uint64_t per_block = trunc_per_block(tv.cap, nblk);
         hipLaunchKernelGGL(k_range_counts, dim3((nblk + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh,
                            rng[r].first, rng[r].second, nblk, blk_emit);
         LN_LAUNCH(ctx);
