@@ -19,6 +19,8 @@
 //                    the symmetric expansion (PathEmbed.h:278-311): kept keys emit
 //                    (row, col) sort keys and float values; per-block counts, a
 //                    scan, then writes in slot order (no contended atomics).
+//                    Where memory allows, the count pass also caches every
+//                    slot's value, so the emission reads no degrees.
 //  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
 //                    row_ptr by boundary detection (no atomics, deterministic).
 //  by row ranges : when the sort buffers for every entry do not fit beside
