@@ -105,6 +105,22 @@ def test_c1_counts_and_csr_bit_exact(gpu):
         np.testing.assert_array_equal(x, y)
 
 
+def test_sparse_graph_csr_sorted_beside_the_table(gpu):
+    """A sparse graph whose sort buffers do not fit the dead table (so the sort
+    gets buffers of its own) and whose 26-bit keys take an odd number of radix
+    passes (the emission then writes the values into the sort's temporary):
+    the CSR still equals the oracle's."""
+    hg = L.chung_lu(8000, 2.1, 4.0, 200.0, 11)
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 10.0)
+    csr, st = L.generate_trunc_log_matrix(hg.to_device(), 5, walk_len=10, sample_ratio=ratio, ctx=gpu)
+    assert st["table_layout"] == 2 and st["nnz"] * 12 > st["table_capacity"] * 8  # keys + values exceed the table
+    keys, cnt, wgh, _ = O.sample(hg.offsets, hg.edges, 5, 10, ratio, 0)
+    ref = O.trunc_log_csr(hg.offsets, keys, wgh)
+    assert st["nnz"] == csr.nnz == ref[3]["nnz"]
+    _compare_csr(csr, ref)
+    csr.free()
+
+
 def test_c2_counts_bit_exact(gpu):
     """Config 2: Chung-Lu n=1.14M, m=2.99M, T=10, M = 20 m (10 per directed edge)."""
     hg = L.chung_lu(1_140_000, 2.0, 5.25, 2500.0, 2)
