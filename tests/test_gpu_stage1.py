@@ -237,17 +237,18 @@ def test_netsmf_row_ranges_in_a_fresh_context(gpu):
         ctx2.close()
 
 
-@pytest.mark.parametrize("parts", [0, 6])
-def test_value_cache_gives_the_same_embedding(gpu, parts):
+@pytest.mark.parametrize("parts,sample", [(0, False), (6, False), (0, True), (6, True)])
+def test_value_cache_gives_the_same_embedding(gpu, parts, sample):
     """The count pass's per-slot value cache gives the pipeline the CSR (so the
     embedding) that reading the degrees again gives, bit for bit, in one sort
-    and by row ranges; auto mode gives the same."""
+    and by row ranges, with plain and effective-resistance (-sample 1) weights;
+    auto mode gives the same."""
     import torch
 
     hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
     g = hg.to_device()
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
-    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, csr_parts=parts, ctx=gpu)
+    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, csr_parts=parts, sample=sample, ctx=gpu)
     e0, S0, st0 = L.NetSMF(g, value_cache=-1, **kw)
     e1, S1, st1 = L.NetSMF(g, value_cache=1, **kw)
     assert st0["sparsify"]["value_cache"] == 0 and st1["sparsify"]["value_cache"] == 1
