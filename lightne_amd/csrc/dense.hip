@@ -102,11 +102,13 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ 
                                                          double* __restrict__ part) {
   constexpr int ROWS = GramRows<TI>::R;
   constexpr int E = 16 / sizeof(TI);  // elements per 16-B vector
-  extern __shared__ float4 s_raw[];
-  TI* s_mem = reinterpret_cast<TI*>(s_raw);
+  // the slab is staged in LDS as FP64 (FP32 panels converted once per element
+  // on the way in, not once per use)
+  extern __shared__ double2 s_raw2[];
+  double* s_mem = reinterpret_cast<double*>(s_raw2);
   const bool same = X == Y;
-  TI* sX = s_mem;
-  TI* sY = same ? s_mem : s_mem + ROWS * k1;
+  double* sX = s_mem;
+  double* sY = same ? s_mem : s_mem + ROWS * k1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   int2 pr[PPW];
@@ -148,19 +150,27 @@ __global__ __launch_bounds__(GR_THREADS) void k_gram_f64(const TI* __restrict__ 
 #pragma unroll
     for (int t = 0; t < GR_MAX_V; ++t) {
       const int f = threadIdx.x + t * GR_THREADS;
-      if (f < nxv + nyv) s_raw[f] = buf[t];
+      if (f < nxv + nyv) {
+        if constexpr (sizeof(TI) == 4) {
+          s_raw2[2 * f] = make_double2((double)buf[t].x, (double)buf[t].y);
+          s_raw2[2 * f + 1] = make_double2((double)buf[t].z, (double)buf[t].w);
+        } else {
+          const double2 d = *reinterpret_cast<const double2*>(&buf[t]);
+          s_raw2[f] = d;
+        }
+      }
     }
     __syncthreads();
     if (r0 + ROWS < r_end) fetch(r0 + ROWS);
 #pragma unroll 2
     for (int s = 0; s < ROWS; s += 4) {
-      const TI* xr = sX + (s + lk) * k1 + li;
-      const TI* yr = sY + (s + lk) * k2 + li;
+      const double* xr = sX + (s + lk) * k1 + li;
+      const double* yr = sY + (s + lk) * k2 + li;
       double av[PPW], bv[PPW];
 #pragma unroll
       for (int q = 0; q < PPW; ++q) {
-        av[q] = (double)xr[pa[q]];
-        bv[q] = (double)yr[pb[q]];
+        av[q] = xr[pa[q]];
+        bv[q] = yr[pb[q]];
       }
 #pragma unroll
       for (int q = 0; q < PPW; ++q) acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], acc[q], 0, 0, 0);
@@ -418,7 +428,7 @@ int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1,
   if (!padded.empty())
     LN_HIP(ctx, hipMemcpyAsync(dpairs, padded.data(), padded.size() * sizeof(int2), hipMemcpyHostToDevice,
                                ctx->stream));
-  const size_t lds = (size_t)ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(TI);
+  const size_t lds = (size_t)ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(double);  // staged as FP64
   if (n > 0) {
     // each launch owns a disjoint set of output tiles
     for (size_t L = 0; L < launch_np.size(); ++L) {
