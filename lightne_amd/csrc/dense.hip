@@ -207,9 +207,8 @@ constexpr int PS_MAX_T2 = 16;  // k2 <= 256
 // times rows 16t.. of the LDS copy of W; TRI (W upper triangular, k1 = k2):
 // output tile j only takes steps t <= j, the others multiply zeros
 template <int T2, bool TRI>
-__device__ __forceinline__ void ps_step(const float* X, int64_t row, bool rv, int64_t ldx, int t, int lk, int li,
-                                        const float* sW, int ldw, floatx4* acc) {
-  float4 xa = rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
+__device__ __forceinline__ void ps_step(const float4 xa, int t, int lk, int li, const float* sW, int ldw,
+                                        floatx4* acc) {
   const float xs[4] = {xa.x, xa.y, xa.z, xa.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -243,11 +242,26 @@ __global__ __launch_bounds__(PS_THREADS) void k_panel_small(const float* X, int6
     floatx4 acc[T2];
 #pragma unroll
     for (int j = 0; j < T2; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    if constexpr (TRI) {  // k1 == 16 * T2: the steps unroll, so the skipped tiles are known at compile time
+    if constexpr (TRI) {
+      // k1 == 16 * T2: the slab row's k1 columns are loaded up front (all
+      // loads in flight together), then multiplied; the steps unroll, so the
+      // zero tiles skipped are known at compile time
+      constexpr int MAXT = 16;
+      const int nt = k1 / 16;
+      float4 xa[MAXT];
 #pragma unroll
-      for (int t = 0; t < T2; ++t) ps_step<T2, true>(X, row, rv, ldx, t, lk, li, sW, ldw, acc);
+      for (int t = 0; t < MAXT; ++t)
+        if (t < nt)
+          xa[t] = rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int t = 0; t < MAXT; ++t)
+        if (t < nt) ps_step<T2, true>(xa[t], t, lk, li, sW, ldw, acc);
     } else {
-      for (int t = 0; t < k1 / 16; ++t) ps_step<T2, false>(X, row, rv, ldx, t, lk, li, sW, ldw, acc);
+      for (int t = 0; t < k1 / 16; ++t) {
+        const float4 xa =
+            rv ? *reinterpret_cast<const float4*>(X + row * ldx + 16 * t + 4 * lk) : make_float4(0.f, 0.f, 0.f, 0.f);
+        ps_step<T2, false>(xa, t, lk, li, sW, ldw, acc);
+      }
     }
     // acc[j][v] = Out[slab*16 + 4*lk + v][16 j + li]
     if (normalize) {
@@ -508,14 +522,15 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
     const int64_t kk = std::min(kb, k2 - j0);
     const size_t lds = (size_t)k1 * (kk + 4) * sizeof(float);
     const int T2 = (int)(kk / 16);
-#define PS_CASE(t)                                                                                               \
-  case t:                                                                                                        \
-    if (tri)                                                                                                     \
-      hipLaunchKernelGGL((k_panel_small<t, true>), dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X,   \
-                         ldx, n, (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows); \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_panel_small<t, false>), dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X,  \
-                         ldx, n, (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows); \
+#define PS_LAUNCH(t, tri_)                                                                                      \
+  hipLaunchKernelGGL((k_panel_small<t, tri_>), dim3((unsigned)nblk), dim3(PS_THREADS), lds, ctx->stream, X, ldx, n, \
+                     (int)k1, (const float*)(dW + j0), (int)k2, dst + j0, (int64_t)k2, normalize_rows)
+#define PS_CASE(t)           \
+  case t:                    \
+    if (tri)                 \
+      PS_LAUNCH(t, true);    \
+    else                     \
+      PS_LAUNCH(t, false);   \
     break;
     switch (T2) {
       PS_CASE(1) PS_CASE(2) PS_CASE(3) PS_CASE(4) PS_CASE(5) PS_CASE(6) PS_CASE(7) PS_CASE(8)
@@ -524,6 +539,7 @@ int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, i
         return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel_times_small: k2 too large");
     }
 #undef PS_CASE
+#undef PS_LAUNCH
     LN_LAUNCH(ctx);
   }
   if (dst != Out)
