@@ -25,7 +25,7 @@
 
 /* pbbslib/utilities.h:154-159 -- splitmix64 finalizer used as hash64_2.
  * pinned by: Python big-int restatement + golden values from the
- * reference-header harness (tests/test_oracle_golden.py). */
+ * reference-header harness (tests/test_oracle.py). */
 uint64_t orc_hash64_2(uint64_t x) {
   x = (x ^ (x >> 30)) * UINT64_C(0xbf58476d1ce4e5b9);
   x = (x ^ (x >> 27)) * UINT64_C(0x94d049bb133111eb);
@@ -379,22 +379,30 @@ void orc_csr_free(orc_csr* c) {
 /* ---------------------------------------------------------------------------
  * Sparse x dense products (the MKL mkl_sparse_?_mm calls of
  * mklredsvd.cc:235,271 and SpectralPropagation.h:101,116,153,168), row-major
- * X (n x k), Y = alpha*A*X + beta*Y, rows summed in CSR order.
- * pinned by: scipy.sparse CSR products on the same inputs (library routine).
+ * X (n x k), Y = alpha*A*X + beta*Y, rows summed in CSR order.  Columns are
+ * processed in blocks of ORC_KB so any k works (the reference's -rank 4096).
+ * pinned by: scipy.sparse CSR products on the same inputs, alpha/beta forms and
+ * k > ORC_KB included (tests/test_oracle.py::test_spmm_oracle_vs_scipy,
+ * test_spmm_f32_oracle_vs_scipy).
  * ------------------------------------------------------------------------- */
+#define ORC_KB 256
+
 void orc_spmm_f64(uint64_t n, const int64_t* rp, const int32_t* col, const float* val, const double* X,
                   uint64_t k, double alpha, double beta, double* Y, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
   for (uint64_t i = 0; i < n; ++i) {
-    double acc[512];
-    for (uint64_t c = 0; c < k; ++c) acc[c] = 0.0;
-    for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
-      double a = (double)val[j];
-      const double* x = X + (uint64_t)col[j] * k;
-      for (uint64_t c = 0; c < k; ++c) acc[c] += a * x[c];
+    double acc[ORC_KB];
+    for (uint64_t c0 = 0; c0 < k; c0 += ORC_KB) {
+      uint64_t kb = k - c0 < ORC_KB ? k - c0 : ORC_KB;
+      for (uint64_t c = 0; c < kb; ++c) acc[c] = 0.0;
+      for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+        double a = (double)val[j];
+        const double* x = X + (uint64_t)col[j] * k + c0;
+        for (uint64_t c = 0; c < kb; ++c) acc[c] += a * x[c];
+      }
+      double* y = Y + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) y[c] = alpha * acc[c] + (beta != 0.0 ? beta * y[c] : 0.0);
     }
-    double* y = Y + i * k;
-    for (uint64_t c = 0; c < k; ++c) y[c] = alpha * acc[c] + (beta != 0.0 ? beta * y[c] : 0.0);
   }
 }
 
@@ -402,15 +410,18 @@ void orc_spmm_f32(uint64_t n, const int64_t* rp, const int32_t* col, const float
                   uint64_t k, float alpha, float beta, float* Y, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
   for (uint64_t i = 0; i < n; ++i) {
-    float acc[512];
-    for (uint64_t c = 0; c < k; ++c) acc[c] = 0.0f;
-    for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
-      float a = val[j];
-      const float* x = X + (uint64_t)col[j] * k;
-      for (uint64_t c = 0; c < k; ++c) acc[c] += a * x[c];
+    float acc[ORC_KB];
+    for (uint64_t c0 = 0; c0 < k; c0 += ORC_KB) {
+      uint64_t kb = k - c0 < ORC_KB ? k - c0 : ORC_KB;
+      for (uint64_t c = 0; c < kb; ++c) acc[c] = 0.0f;
+      for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+        float a = val[j];
+        const float* x = X + (uint64_t)col[j] * k + c0;
+        for (uint64_t c = 0; c < kb; ++c) acc[c] += a * x[c];
+      }
+      float* y = Y + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) y[c] = alpha * acc[c] + (beta != 0.0f ? beta * y[c] : 0.0f);
     }
-    float* y = Y + i * k;
-    for (uint64_t c = 0; c < k; ++c) y[c] = alpha * acc[c] + (beta != 0.0f ? beta * y[c] : 0.0f);
   }
 }
 
@@ -419,47 +430,58 @@ void orc_spmm_f32(uint64_t n, const int64_t* rp, const int32_t* col, const float
  * with the two coefficients formed as the reference forms them (double
  * expression, stored as FP).  Graph CSR (off, adj) without self loops.
  * Y = alpha * M X + beta * Y.  Double precision throughout.
- * pinned by: dense numpy M built from the same formula, cross-checked against
- * spectrum bounds [-mu, 2-mu). */
+ * pinned by: the explicit scipy M of oracle.py:prop_matrix (itself pinned by
+ * M 1 = -mu 1 and the closed-form spectra of a cycle and a star), on random
+ * graphs with alpha/beta forms (tests/test_oracle.py::test_prop_apply_vs_explicit_matrix). */
 void orc_prop_apply_f64(uint64_t n, const uint64_t* off, const uint32_t* adj, double mu, const double* X,
                         uint64_t k, double alpha, double beta, double* Y, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
   for (uint64_t i = 0; i < n; ++i) {
-    double acc[512];
+    double acc[ORC_KB];
     double deg = (double)(off[i + 1] - off[i] + 1);
     double dg = 1.0 - mu - 1.0 / deg, of = -1.0 / deg;
-    const double* xi = X + i * k;
-    for (uint64_t c = 0; c < k; ++c) acc[c] = dg * xi[c];
-    for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
-      const double* x = X + (uint64_t)adj[j] * k;
-      for (uint64_t c = 0; c < k; ++c) acc[c] += of * x[c];
+    for (uint64_t c0 = 0; c0 < k; c0 += ORC_KB) {
+      uint64_t kb = k - c0 < ORC_KB ? k - c0 : ORC_KB;
+      const double* xi = X + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) acc[c] = dg * xi[c];
+      for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
+        const double* x = X + (uint64_t)adj[j] * k + c0;
+        for (uint64_t c = 0; c < kb; ++c) acc[c] += of * x[c];
+      }
+      double* y = Y + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) y[c] = alpha * acc[c] + (beta != 0.0 ? beta * y[c] : 0.0);
     }
-    double* y = Y + i * k;
-    for (uint64_t c = 0; c < k; ++c) y[c] = alpha * acc[c] + (beta != 0.0 ? beta * y[c] : 0.0);
   }
 }
 
 /* Same operator in FP32 with coefficients rounded to float as the reference
- * stores them (SpectralPropagation.h:62,66,72).  CPU baseline kernel. */
+ * stores them (SpectralPropagation.h:62,66,72).  CPU baseline kernel.
+ * pinned by: the FP64 form above at FP32 tolerance, and its coefficients equal
+ * to (float)(1.0 - mu - 1.0/deg) and (float)(-1.0/deg) exactly on unit vectors
+ * (tests/test_oracle.py::test_prop_apply_vs_explicit_matrix). */
 void orc_prop_apply_f32(uint64_t n, const uint64_t* off, const uint32_t* adj, float mu, const float* X,
                         uint64_t k, float alpha, float beta, float* Y, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
   for (uint64_t i = 0; i < n; ++i) {
-    float acc[512];
+    float acc[ORC_KB];
     float deg = (float)(off[i + 1] - off[i] + 1);
     float dg = (float)(1.0 - (double)mu - 1.0 / (double)deg), of = (float)(-1.0 / (double)deg);
-    const float* xi = X + i * k;
-    for (uint64_t c = 0; c < k; ++c) acc[c] = dg * xi[c];
-    for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
-      const float* x = X + (uint64_t)adj[j] * k;
-      for (uint64_t c = 0; c < k; ++c) acc[c] += of * x[c];
+    for (uint64_t c0 = 0; c0 < k; c0 += ORC_KB) {
+      uint64_t kb = k - c0 < ORC_KB ? k - c0 : ORC_KB;
+      const float* xi = X + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) acc[c] = dg * xi[c];
+      for (uint64_t j = off[i]; j < off[i + 1]; ++j) {
+        const float* x = X + (uint64_t)adj[j] * k + c0;
+        for (uint64_t c = 0; c < kb; ++c) acc[c] += of * x[c];
+      }
+      float* y = Y + i * k + c0;
+      for (uint64_t c = 0; c < kb; ++c) y[c] = alpha * acc[c] + (beta != 0.0f ? beta * y[c] : 0.0f);
     }
-    float* y = Y + i * k;
-    for (uint64_t c = 0; c < k; ++c) y[c] = alpha * acc[c] + (beta != 0.0f ? beta * y[c] : 0.0f);
   }
 }
 
-/* SpectralPropagation.h:210-221 neighbour aggregation z_i = y_i + sum_{j in N(i)} y_j. */
+/* SpectralPropagation.h:210-221 neighbour aggregation z_i = y_i + sum_{j in N(i)} y_j.
+ * pinned by: scipy (A + I) @ Y on random graphs (tests/test_oracle.py::test_aggregate_vs_scipy). */
 void orc_aggregate_f64(uint64_t n, const uint64_t* off, const uint32_t* adj, const double* Yin, uint64_t k,
                        double* Z, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)

@@ -296,6 +296,34 @@ def spmm_f32(rp, col, val, X: np.ndarray, threads: int = 0) -> np.ndarray:
     return Y
 
 
+def prop_apply(offsets, edges, X: np.ndarray, mu: float, alpha: float = 1.0, beta: float = 0.0, Y=None,
+               fp32: bool = False, threads: int = 0) -> np.ndarray:
+    """Y = alpha * M X + beta * Y with M applied implicitly from the graph
+    (orc_prop_apply_f64 / _f32; SpectralPropagation.h:51-75)."""
+    lib = load()
+    dt = np.float32 if fp32 else np.float64
+    off = np.ascontiguousarray(offsets, np.uint64)
+    adj = np.ascontiguousarray(edges, np.uint32)
+    X = np.ascontiguousarray(X, dt)
+    Y = np.zeros_like(X) if Y is None else np.ascontiguousarray(Y, dt).copy()
+    fn = lib.orc_prop_apply_f32 if fp32 else lib.orc_prop_apply_f64
+    fn(off.shape[0] - 1, _p(off), _p(adj), mu, _p(X), X.shape[1], alpha, beta, _p(Y),
+       threads or int(lib.orc_max_threads()))
+    return Y
+
+
+def aggregate(offsets, edges, Y: np.ndarray, threads: int = 0) -> np.ndarray:
+    """z = (A + I) y (orc_aggregate_f64; SpectralPropagation.h:210-221)."""
+    lib = load()
+    off = np.ascontiguousarray(offsets, np.uint64)
+    adj = np.ascontiguousarray(edges, np.uint32)
+    Y = np.ascontiguousarray(Y, np.float64)
+    Z = np.empty_like(Y)
+    lib.orc_aggregate_f64(off.shape[0] - 1, _p(off), _p(adj), _p(Y), Y.shape[1], _p(Z),
+                          threads or int(lib.orc_max_threads()))
+    return Z
+
+
 # ---------------------------------------------------------------------------
 # the ne_zhang_et_al matrix (PathEmbed.h:410-444)
 # ---------------------------------------------------------------------------

@@ -265,3 +265,128 @@ def test_sparse_projection_distribution():
     # different seeds draw different matrices; the same seed the same one
     assert not np.array_equal(O.sparse_projection(500, 32, 1, 4.0), O.sparse_projection(500, 32, 2, 4.0))
     assert np.array_equal(O.sparse_projection(500, 32, 1, 4.0), O.sparse_projection(500, 32, 1, 4.0))
+
+
+# ---------------------------------------------------------------------------
+# propagation operator M = (1 - mu) I - D~^-1 A~ (SpectralPropagation.h:51-75)
+# pinned by closed forms that a D-for-D~, sign or diagonal mistake fails
+# ---------------------------------------------------------------------------
+def _random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    A = np.triu(rng.random((n, n)) < p, 1)
+    pairs = list(zip(*np.nonzero(A)))
+    off, edg = _sym_graph(n, pairs)
+    # shuffle each adjacency list: the operator must not depend on neighbour order
+    for i in range(n):
+        seg = edg[off[i]:off[i + 1]]
+        rng.shuffle(seg)
+    return off, edg
+
+
+@pytest.mark.parametrize("graph", ["cl200", "blogcatalog"])
+@pytest.mark.parametrize("mu", [0.2, 0.0, 0.7])
+def test_prop_matrix_rows_sum_to_minus_mu(graph, mu):
+    """M 1 = (1 - mu) 1 - D~^-1 (A + I) 1 = (1 - mu) - (d + 1)/(d + 1) = -mu for
+    every row (D in place of D~, or a dropped self loop, breaks it)."""
+    off, edg = H.golden_graph(graph)
+    n = len(off) - 1
+    M = O.prop_matrix(off, edg, mu)
+    np.testing.assert_allclose(M @ np.ones(n), -mu * np.ones(n), atol=1e-12)
+    # the row-stochastic part: every off-diagonal entry of row i is -1/(d_i + 1)
+    deg = np.diff(off)
+    Mc = M.tocsr()
+    i = int(np.argmax(deg))
+    row = Mc.getrow(i).toarray().ravel()
+    assert row[i] == 1.0 - mu - 1.0 / (deg[i] + 1)
+    nb = np.asarray(edg[off[i]:off[i + 1]], np.int64)
+    np.testing.assert_array_equal(row[nb], -1.0 / (deg[i] + 1))
+    assert np.count_nonzero(row) == deg[i] + 1
+
+
+@pytest.mark.parametrize("n", [7, 12, 31])
+def test_prop_matrix_cycle_spectrum(n):
+    """Cycle C_n: D~ = 3 I, eig(A) = 2 cos(2 pi j / n), so
+    eig(M) = 1 - mu - (1 + 2 cos(2 pi j / n)) / 3 exactly."""
+    mu = 0.2
+    off, edg = _sym_graph(n, [(i, (i + 1) % n) for i in range(n)])
+    M = O.prop_matrix(off, edg, mu).toarray()
+    got = np.sort(np.linalg.eigvals(M).real)
+    j = np.arange(n)
+    want = np.sort(1 - mu - (1 + 2 * np.cos(2 * np.pi * j / n)) / 3)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+@pytest.mark.parametrize("m", [3, 8, 40])
+def test_prop_matrix_star_spectrum(m):
+    """Star K_{1,m}: D~^-1 A~ has eigenvalues 1, 1/(m+1) - 1/2 and 1/2 (x m-1),
+    so eig(M) = {-mu, 3/2 - mu - 1/(m+1), 1/2 - mu (m-1 times)}."""
+    mu = 0.2
+    off, edg = _sym_graph(m + 1, [(0, i) for i in range(1, m + 1)])
+    M = O.prop_matrix(off, edg, mu).toarray()
+    got = np.sort(np.linalg.eigvals(M).real)
+    want = np.sort(np.array([-mu, 1.5 - mu - 1.0 / (m + 1)] + [0.5 - mu] * (m - 1)))
+    np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+def test_prop_matrix_spectrum_range():
+    """M is similar to the symmetric (1 - mu) I - D~^-1/2 A~ D~^-1/2, whose
+    normalised part has spectrum in (-1, 1]: eig(M) in [-mu, 2 - mu), with -mu
+    attained (the constant vector)."""
+    off, edg = H.golden_graph("toy")
+    mu = 0.2
+    M = O.prop_matrix(off, edg, mu).toarray()
+    deg = np.diff(off) + 1.0
+    S = np.diag(np.sqrt(deg)) @ M @ np.diag(1 / np.sqrt(deg))
+    np.testing.assert_allclose(S, S.T, atol=1e-14)
+    ev = np.linalg.eigvalsh(0.5 * (S + S.T))
+    assert abs(ev.min() + mu) < 1e-12 and ev.max() < 2 - mu
+
+
+@pytest.mark.parametrize("k", [1, 16, 300])
+def test_prop_apply_vs_explicit_matrix(k):
+    """The implicit C operator (FP64 and FP32 forms, alpha/beta) equals the
+    pinned explicit M applied by scipy."""
+    off, edg = _random_graph(120, 0.08, 5)
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((120, k))
+    Y0 = rng.standard_normal((120, k))
+    M = O.prop_matrix(off, edg, float(np.float32(0.2)))
+    for alpha, beta in [(1.0, 0.0), (0.5, -2.0), (-1.0, 1.0)]:
+        got = O.prop_apply(off, edg, X, float(np.float32(0.2)), alpha, beta, Y0, threads=2)
+        np.testing.assert_allclose(got, alpha * (M @ X) + beta * Y0, rtol=1e-13, atol=1e-13)
+        g32 = O.prop_apply(off, edg, X, 0.2, alpha, beta, Y0, fp32=True, threads=2)
+        np.testing.assert_allclose(g32, alpha * (M @ X) + beta * Y0, rtol=2e-5, atol=2e-5)
+    # FP32 coefficients are the reference's double expressions rounded to float
+    E = np.eye(120)[:, :1]
+    col = O.prop_apply(off, edg, E, 0.2, fp32=True, threads=1).ravel()
+    d0 = float(off[1] - off[0]) + 1.0
+    assert col[0] == np.float32(1.0 - float(np.float32(0.2)) - 1.0 / d0)
+    for j in edg[off[0]:off[1]]:
+        dj = float(off[j + 1] - off[j]) + 1.0
+        assert col[j] == np.float32(-1.0 / dj)
+
+
+def test_aggregate_vs_scipy():
+    import scipy.sparse as sp
+
+    off, edg = _random_graph(150, 0.05, 9)
+    n = 150
+    rows = np.repeat(np.arange(n), np.diff(off))
+    A = sp.csr_matrix((np.ones(len(edg)), (rows, edg.astype(np.int64))), shape=(n, n))
+    Y = np.random.default_rng(1).standard_normal((n, 20))
+    np.testing.assert_allclose(O.aggregate(off, edg, Y, threads=2), (A + sp.eye(n)) @ Y, rtol=1e-13, atol=1e-13)
+
+
+def test_spmm_f32_oracle_vs_scipy():
+    """FP32 oracle SpMM (the CPU baseline kernel) against scipy in FP64, and the
+    FP64 one at k above the oracle's column block."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(2)
+    A = sp.random(400, 400, density=0.03, random_state=3, format="csr", dtype=np.float32)
+    for k in [8, 144, 600]:
+        X = rng.standard_normal((400, k)).astype(np.float32)
+        ref = A.astype(np.float64) @ X.astype(np.float64)
+        np.testing.assert_allclose(O.spmm_f32(A.indptr, A.indices, A.data, X, threads=2), ref, rtol=1e-5, atol=1e-5)
+        np.testing.assert_allclose(O.spmm_f64(A.indptr, A.indices, A.data, X.astype(np.float64), threads=2), ref,
+                                   rtol=1e-12, atol=1e-12)
