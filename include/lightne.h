@@ -62,6 +62,12 @@ extern "C" {
 #define LIGHTNE_BF16 1
 #define LIGHTNE_FP64 2 /* randomized SVD only: FP64 panels, products and Grams (accuracy reference) */
 #define LIGHTNE_BF16_POWER 3 /* as LIGHTNE_BF16, and the randomized SVD's power iterations also gather BF16 */
+/* randomized SVD only: FP64 panels, products and Grams in the FP32 form's
+ * schedule (thin SVD of B = A Q from its Gram) with two n x rank FP64 panels
+ * instead of four; its outputs (U_out of lightne_rsvd, emb_out of
+ * lightne_netsmf) may be host memory.  The accuracy reference where the
+ * LIGHTNE_FP64 form does not fit. */
+#define LIGHTNE_FP64_LEAN 4
 
 typedef struct lightne_ctx lightne_ctx;
 typedef struct lightne_table lightne_table;

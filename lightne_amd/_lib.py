@@ -26,6 +26,7 @@ FP32 = 0
 BF16 = 1
 FP64 = 2  # randomized SVD only (accuracy reference)
 BF16_POWER = 3  # BF16, and the power iterations gather BF16 too
+FP64_LEAN = 4  # randomized SVD only: FP64 in two panels (the FP32 form's schedule); output may be host memory
 
 _ERR_NAMES = {
     OK: "OK", ERR_BAD_ARG: "BAD_ARG", ERR_OOM: "OOM", ERR_TABLE_FULL: "TABLE_FULL", ERR_HIP: "HIP",

@@ -27,14 +27,36 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from ._lib import (BF16, BF16_POWER, FP32, FP64, Graph, LightNEError, NetsmfStats, PropParams, PropStats,
-                   RsvdParams, RsvdStats, SparsifyParams, SparsifyStats, check, load)
+from ._lib import (BF16, BF16_POWER, FP32, FP64, FP64_LEAN, Graph, LightNEError, NetsmfStats, PropParams,
+                   PropStats, RsvdParams, RsvdStats, SparsifyParams, SparsifyStats, check, load)
 
 
 def _torch():
     import torch  # imported lazily: host-only helpers work without torch
 
     return torch
+
+
+def _bad(msg: str):
+    raise LightNEError(_lib.ERR_BAD_ARG, msg)
+
+
+def _need(t, name: str, dtype, shape=None, device: Optional[int] = None):
+    """A device tensor the library reads or writes through a raw pointer:
+    it must be a contiguous torch tensor of ``dtype`` (and ``shape``) on the
+    context's device, or the C side would read the wrong bytes."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        _bad(f"{name}: expected a torch tensor on the device, got {type(t).__name__}")
+    if t.device.type != "cuda" or (device is not None and (t.device.index or 0) != device):
+        _bad(f"{name}: tensor on {t.device}, expected cuda:{device if device is not None else 0}")
+    if t.dtype != dtype:
+        _bad(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if not t.is_contiguous():
+        _bad(f"{name}: tensor is not contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        _bad(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return t
 
 
 def _stats_dict(s) -> dict:
@@ -87,6 +109,10 @@ class DeviceGraph:
     ith_rule: int = _lib.ITH_PLAIN
 
     def desc(self) -> Graph:
+        torch = _torch()
+        _need(self.offsets, "graph offsets", torch.int64, (self.n + 1,))
+        if self.m:
+            _need(self.edges, "graph edges", torch.int32, (self.m,), self.offsets.device.index or 0)
         return Graph(self.n, self.m, self.offsets.data_ptr(), self.edges.data_ptr() if self.m else None,
                      self.ith_rule, 0)
 
@@ -191,26 +217,59 @@ def rmat(scale: int, edge_factor: float, a: float, b: float, c: float, seed: int
 # ---------------------------------------------------------------------------
 # context
 # ---------------------------------------------------------------------------
+class _StreamOrdered:
+    """The library's entry points, each called with the context's stream
+    ordered against torch's current stream on the device at the time of the
+    call: the context stream first waits for the work queued on the current
+    stream (which produced the inputs), and the current stream then waits for
+    the call's work (which produced the outputs).  Work on another torch
+    stream therefore never races with the library."""
+
+    def __init__(self, lib, ctx: "Context"):
+        self._lib = lib
+        self._ctx = ctx
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name)
+        ctx = self._ctx
+
+        def call(*args):
+            torch = _torch()
+            cur = torch.cuda.current_stream(ctx.device)
+            other = cur.cuda_stream != ctx.stream.cuda_stream
+            if other:
+                ctx.stream.wait_stream(cur)
+            rc = fn(*args)
+            if other:
+                cur.wait_stream(ctx.stream)
+            return rc
+
+        return call
+
+
 class Context:
-    """One library context bound to a device and torch's current stream on it."""
+    """One library context bound to a device and a stream on it (torch's
+    current stream when the context is created); calls made while another
+    torch stream is current are ordered against it (``_StreamOrdered``)."""
 
     def __init__(self, device: int = 0):
         torch = _torch()
         if not torch.cuda.is_available():
             raise LightNEError(_lib.ERR_NO_DEVICE, "no GPU visible to torch")
-        self.lib = load()
+        self.raw = load()
+        self.lib = _StreamOrdered(self.raw, self)
         self.device = device
         with torch.cuda.device(device):
             self.stream = torch.cuda.current_stream(device)
         self.ptr = C.c_void_p()
-        check(self.lib.lightne_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(self.ptr)))
+        check(self.raw.lightne_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(self.ptr)))
 
     def check(self, rc: int) -> None:
         check(rc, self.ptr)
 
     def close(self) -> None:
         if self.ptr:
-            self.lib.lightne_ctx_destroy(self.ptr)
+            self.raw.lightne_ctx_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
@@ -308,18 +367,26 @@ class CSR:
         ctx = ctx or context()
         keep = []
 
-        def ptr(a, dt):
+        torch = _torch()
+        tdt = {np.int64: torch.int64, np.int32: torch.int32, np.float32: torch.float32}
+
+        def ptr(a, dt, name, length):
             if isinstance(a, np.ndarray):
                 a = np.ascontiguousarray(a, dtype=dt)
+                if a.shape != (length,):
+                    _bad(f"CSR.from_arrays: {name} has shape {a.shape}, expected ({length},)")
                 keep.append(a)
                 return a.ctypes.data
-            return a.data_ptr()
+            return _need(a, f"CSR.from_arrays: {name}", tdt[dt], (length,), ctx.device).data_ptr()
 
         n = int(len(row_ptr) - 1)
         nnz = int(len(col))
+        if n <= 0:
+            _bad("CSR.from_arrays: row_ptr needs n + 1 >= 2 entries")
         h = C.c_void_p()
-        ctx.check(ctx.lib.lightne_csr_create(ctx.ptr, n, nnz, ptr(row_ptr, np.int64), ptr(col, np.int32),
-                                             ptr(val, np.float32), C.byref(h)))
+        ctx.check(ctx.lib.lightne_csr_create(ctx.ptr, n, nnz, ptr(row_ptr, np.int64, "row_ptr", n + 1),
+                                             ptr(col, np.int32, "col", nnz), ptr(val, np.float32, "val", nnz),
+                                             C.byref(h)))
         return cls(ctx, h)
 
     def to_numpy(self, nnz_fraction: Optional[float] = None):
@@ -393,13 +460,19 @@ class MKLRedSVD:
         self.S = None
         self.stats = None
 
-    def run(self):
+    def run(self, out=None):
+        """``out`` (n x rank float32) receives U: a device tensor, or with the
+        FP64_LEAN form also a host numpy array."""
         torch = _torch()
         n, k = self.csr.n, self.params.rank
-        U = torch.empty((n, k), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        if out is None:
+            U = torch.empty((n, k), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        else:
+            U = out
         S = np.zeros(k, np.float32)
         st = RsvdStats()
-        self.ctx.check(self.ctx.lib.lightne_rsvd(self.ctx.ptr, self.csr.handle, C.byref(self.params), _dev(U),
+        uptr = _out_ptr(U, (n, k), "MKLRedSVD.run: out", self.ctx, self.params.precision == FP64_LEAN)
+        self.ctx.check(self.ctx.lib.lightne_rsvd(self.ctx.ptr, self.csr.handle, C.byref(self.params), uptr,
                                                  S.ctypes.data, C.byref(st)))
         self.matU = U
         self.S = None if self.params.random_project_only else S
@@ -407,10 +480,24 @@ class MKLRedSVD:
         return self
 
 
+def _out_ptr(out, shape, name: str, ctx: Context, host_ok: bool) -> int:
+    """Pointer of an output buffer: a device tensor, or (where the entry point
+    accepts host memory) a writable C-contiguous float32 numpy array."""
+    torch = _torch()
+    if isinstance(out, np.ndarray):
+        if not host_ok:
+            _bad(f"{name}: must be a device tensor")
+        if out.dtype != np.float32 or not out.flags.c_contiguous or not out.flags.writeable or out.shape != shape:
+            _bad(f"{name}: host output must be a writable C-contiguous float32 array of shape {shape}")
+        return out.ctypes.data
+    return _need(out, name, torch.float32, shape, ctx.device).data_ptr()
+
+
 def compute_u_sigma_root(U, S, n: int, rank: int, dim: int, normalize: bool, ctx: Optional[Context] = None):
     """emb[:, j] = sqrt(S_j) U[:, j], optional row normalisation (PathEmbed.h:371-408)."""
     torch = _torch()
     ctx = ctx or context()
+    _need(U, "compute_u_sigma_root: U", torch.float32, (n, rank), ctx.device)
     emb = torch.empty((n, dim), dtype=torch.float32, device=U.device)
     s = None if S is None else np.ascontiguousarray(S, dtype=np.float32)
     ctx.check(ctx.lib.lightne_u_sigma_root(ctx.ptr, _dev(U), s.ctypes.data if s is not None else None, n, rank, dim,
@@ -424,9 +511,11 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
            random_project_only: bool = False, seed: int = 0, svd_seed: int = 777, power_iters: int = 1,
            precision: int = FP32, max_table_bytes: int = 0, sparse_project: bool = False,
            sparse_project_s: float = 100.0, table_layout: int = 0, csr_parts: int = 0,
-           value_cache: int = 0, ctx: Optional[Context] = None):
+           value_cache: int = 0, out=None, ctx: Optional[Context] = None):
     """Stages 1 + 2 (PathEmbed.h:494-563): returns (emb n x dim, S, stats).
-    ``sparse_project_s`` defaults to 100 as LightNE.cc:50 does."""
+    ``sparse_project_s`` defaults to 100 as LightNE.cc:50 does.  ``out``
+    (n x dim float32) receives emb: a device tensor, or with the FP64_LEAN
+    form also a host numpy array."""
     del walks_per_edge, analyze
     torch = _torch()
     ctx = ctx or context()
@@ -435,12 +524,13 @@ def NetSMF(graph: DeviceGraph, walks_per_edge: int = 100, walk_len: int = 10, up
     rp = RsvdParams(rank=rank, power_iters=power_iters, precision=precision,
                     random_project_only=int(bool(random_project_only)), upper=int(bool(upper)), seed=svd_seed,
                     sparse_project=int(bool(sparse_project)), sparse_project_s=sparse_project_s)
-    emb = torch.empty((graph.n, dim), dtype=torch.float32, device=f"cuda:{ctx.device}")
+    emb = out if out is not None else torch.empty((graph.n, dim), dtype=torch.float32, device=f"cuda:{ctx.device}")
+    eptr = _out_ptr(emb, (graph.n, dim), "NetSMF: out", ctx, precision == FP64_LEAN)
     S = np.zeros(rank, np.float32)
     st = NetsmfStats()
     g = graph.desc()
     ctx.check(ctx.lib.lightne_netsmf(ctx.ptr, C.byref(g), C.byref(sp), C.byref(rp), dim, int(bool(normalize)),
-                                     _dev(emb), S.ctypes.data, C.byref(st)))
+                                     eptr, S.ctypes.data, C.byref(st)))
     return emb, S, _stats_dict(st)
 
 
@@ -479,7 +569,9 @@ def propagate(a, graph: DeviceGraph, order: int = 10, mu: float = 0.2, theta: fl
     ``a`` (n x dim float32 on the device) is not modified.  Returns (y, z, stats)."""
     torch = _torch()
     ctx = ctx or context()
-    a = a.contiguous()
+    if getattr(a, "ndim", 0) != 2:
+        _bad("propagate: a must be an n x dim tensor")
+    _need(a, "propagate: a", torch.float32, (graph.n, a.shape[1]), ctx.device)
     n, dim = a.shape
     y = torch.empty_like(a)
     z = torch.empty_like(a) if aggregate else None
@@ -496,7 +588,7 @@ def get_embedding_dense(mat, n: int, rank: int, dim: int, ctx: Optional[Context]
     Returns (emb, S)."""
     torch = _torch()
     ctx = ctx or context()
-    mat = mat.contiguous()
+    _need(mat, "get_embedding_dense: mat", torch.float32, (n, rank), ctx.device)
     emb = torch.empty((n, dim), dtype=torch.float32, device=mat.device)
     S = np.zeros(rank, np.float32)
     ctx.check(ctx.lib.lightne_embed_dense(ctx.ptr, _dev(mat), n, rank, dim, _dev(emb), S.ctypes.data))
@@ -512,11 +604,10 @@ def chebyshev_expansion(a, graph: DeviceGraph, rank: int, dim: int, order: int =
     if rank != dim:
         raise LightNEError(_lib.ERR_BAD_ARG, "chebyshev_expansion: the runner passes rank == dim (LightNE.cc:114)")
     ctx = ctx or context()
-    a = a.contiguous()
-    n = a.shape[0]
+    n = graph.n
+    _need(a, "chebyshev_expansion: a", torch.float32, (n, dim), ctx.device)
     emb = out if out is not None else torch.empty((n, dim), dtype=torch.float32, device=a.device)
-    if emb.shape != (n, dim) or emb.dtype != torch.float32 or not emb.is_contiguous():
-        raise LightNEError(_lib.ERR_BAD_ARG, "chebyshev_expansion: out must be a contiguous n x dim float32 tensor")
+    _need(emb, "chebyshev_expansion: out", torch.float32, (n, dim), ctx.device)
     p = PropParams(order=order, mu=mu, theta=s, precision=precision)
     st = PropStats()
     g = graph.desc()
@@ -530,11 +621,16 @@ def chebyshev_expansion(a, graph: DeviceGraph, rank: int, dim: int, order: int =
 # ---------------------------------------------------------------------------
 def dense_gram(X, Y=None, ctx: Optional[Context] = None) -> np.ndarray:
     """X^T Y in FP64 (device FP32 panels)."""
+    torch = _torch()
     ctx = ctx or context()
     sym = Y is None
     Y = X if sym else Y
+    if getattr(X, "ndim", 0) != 2 or getattr(Y, "ndim", 0) != 2:
+        _bad("dense_gram: X and Y must be 2-D tensors")
     n, k1 = X.shape
     k2 = Y.shape[1]
+    _need(X, "dense_gram: X", torch.float32, (n, k1), ctx.device)
+    _need(Y, "dense_gram: Y", torch.float32, (n, k2), ctx.device)
     G = np.zeros((k1, k2), np.float64)
     ctx.check(ctx.lib.lightne_dense_gram(ctx.ptr, _dev(X), _dev(Y), n, k1, k2, int(sym), G.ctypes.data))
     return G
@@ -544,11 +640,17 @@ def dense_panel_gemm(X, W: np.ndarray, normalize_rows: bool = False, out=None, c
     """X (n x k1, device FP32) times W (k1 x k2 host) -> device FP32 n x k2."""
     torch = _torch()
     ctx = ctx or context()
+    if getattr(X, "ndim", 0) != 2:
+        _bad("dense_panel_gemm: X must be a 2-D tensor")
     n, k1 = X.shape
+    _need(X, "dense_panel_gemm: X", torch.float32, (n, k1), ctx.device)
     W = np.ascontiguousarray(W, np.float64)
+    if W.ndim != 2 or W.shape[0] != k1:
+        _bad(f"dense_panel_gemm: W must be {k1} x k2, got {W.shape}")
     k2 = W.shape[1]
     if out is None:
         out = torch.empty((n, k2), dtype=torch.float32, device=X.device)
+    _need(out, "dense_panel_gemm: out", torch.float32, (n, k2), ctx.device)
     ctx.check(ctx.lib.lightne_dense_panel_gemm(ctx.ptr, _dev(X), n, k1, W.ctypes.data, k2, _dev(out),
                                                int(bool(normalize_rows))))
     return out
@@ -556,8 +658,12 @@ def dense_panel_gemm(X, W: np.ndarray, normalize_rows: bool = False, out=None, c
 
 def dense_cholqr(Y, ctx: Optional[Context] = None):
     """CholeskyQR2 in place: returns (Q (= Y), R, passes)."""
+    torch = _torch()
     ctx = ctx or context()
+    if getattr(Y, "ndim", 0) != 2:
+        _bad("dense_cholqr: Y must be a 2-D tensor")
     n, k = Y.shape
+    _need(Y, "dense_cholqr: Y", torch.float32, (n, k), ctx.device)
     R = np.zeros((k, k), np.float64)
     passes = C.c_int32()
     ctx.check(ctx.lib.lightne_dense_cholqr(ctx.ptr, _dev(Y), n, k, R.ctypes.data, C.byref(passes)))
@@ -578,8 +684,10 @@ def propagate_backward(a, grad_out, graph: DeviceGraph, order: int = 10, mu: flo
     dL/dy (dL/dz) -- the hand-written adjoint of the Chebyshev recurrence."""
     torch = _torch()
     ctx = ctx or context()
-    a = a.contiguous()
-    g_out = grad_out.contiguous().to(torch.float32)
+    if getattr(a, "ndim", 0) != 2:
+        _bad("propagate_backward: a must be an n x dim tensor")
+    _need(a, "propagate_backward: a", torch.float32, (graph.n, a.shape[1]), ctx.device)
+    g_out = _need(grad_out, "propagate_backward: grad_out", torch.float32, tuple(a.shape), ctx.device)
     n, dim = a.shape
     grad_a = torch.empty_like(a)
     gm, gt = C.c_double(), C.c_double()

@@ -312,6 +312,148 @@ static int rsvd64(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p
   return LIGHTNE_OK;
 }
 
+// Lean FP64 form: the FP32 form's schedule (thin SVD of B = A Q from its Gram)
+// with FP64 panels, products and Grams in two n x k FP64 panels -- half the
+// explicit form's four, so that it fits beside a sparsifier of 1.5e10 nonzeros
+// at n = 2^26.  The triangular applies Y R^-1 run over row blocks through a
+// small staging buffer instead of a third panel.  On return `out` (host or
+// device, row-major n x cols) holds B V[:, :cols] S^-pw rounded to FP32, row
+// normalised when asked, or Y = A Omega with random_project_only.
+static int rsvd64_lean(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p, int q, int64_t cols,
+                       double pw, int normalize, float* out, std::vector<double>& S, lightne_rsvd_stats* stp) {
+  const int64_t n = A->n, k = p->rank;
+  lightne_rsvd_stats& st = *stp;
+  st = lightne_rsvd_stats{};
+  hipStream_t s = ctx->stream;
+  const size_t pd = (size_t)n * k * sizeof(double);
+  const int64_t blk_rows = std::min<int64_t>(n, (int64_t)1 << 20);
+  double *P0 = nullptr, *P1 = nullptr, *T = nullptr;  // Omega, then B; Y / Q; row-block staging
+  {
+    WsCarve wc;
+    wc.add((void**)&P0, pd);
+    wc.add((void**)&P1, pd);
+    wc.add((void**)&T, (size_t)blk_rows * k * sizeof(double));
+    LN_CHECK(wc.commit(ctx));
+  }
+  hipEvent_t* e = ctx->ev;
+  auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
+  // Y <- Y W for a square host W, one row block at a time through T
+  auto apply_rows = [&](double* Y, const double* W) -> int {
+    for (int64_t r0 = 0; r0 < n; r0 += blk_rows) {
+      const int64_t rows = std::min(blk_rows, n - r0);
+      LN_CHECK(panel_times_small64(ctx, Y + r0 * k, rows, k, W, k, T));
+      LN_HIP(ctx, hipMemcpyAsync(Y + r0 * k, T, (size_t)rows * k * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    return LIGHTNE_OK;
+  };
+  double t_spmm = 0, t_qr = 0;
+  LN_HIP(ctx, hipEventRecord(e[0], s));
+  if (!p->sparse_project) {
+    // the sketch's FP32 values (BF16-representable, exact in double) drawn
+    // into the second half of P1's bytes, widened into P0
+    float* tmp32 = reinterpret_cast<float*>(P1) + (size_t)n * k;
+    LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, tmp32));
+    LN_CHECK(f32_to_f64(ctx, tmp32, P0, n * k));
+  }
+  LN_HIP(ctx, hipEventRecord(e[1], s));
+  LN_CHECK(csr_plan(ctx, A));
+  LN_HIP(ctx, hipEventRecord(e[2], s));
+  if (p->sparse_project)
+    LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, true));
+  else
+    LN_CHECK(csr_mm64(ctx, A, P0, k, P1));
+  LN_HIP(ctx, hipEventRecord(e[3], s));
+  LN_HIP(ctx, hipEventSynchronize(e[3]));
+  st.t_sketch = lap(e[0], e[1]);
+  t_spmm += lap(e[2], e[3]);
+  st.spmm_calls = 1;
+  float* stage = reinterpret_cast<float*>(P0);  // FP32 result staging (P0's bytes hold 2 n k floats)
+  if (p->random_project_only) {
+    LN_CHECK(f64_to_f32(ctx, P1, stage, n * k));
+    LN_HIP(ctx, hipMemcpyAsync(out, stage, (size_t)n * k * sizeof(float), hipMemcpyDefault, s));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+    S.clear();
+    st.t_spmm = t_spmm;
+    st.t_total = lap(e[0], e[3]);
+    return LIGHTNE_OK;
+  }
+  std::vector<double> G(k * k), Gs(k * k), Rs(k * k), R(k * k), Rinv(k * k), w(k), V(k * k, 0.0);
+  for (int it = 0; it < q; ++it) {
+    LN_HIP(ctx, hipEventRecord(e[4], s));
+    // CholeskyQR2 in FP64 (the same passes as cholqr2_t)
+    for (int pass = 0; pass < 4; ++pass) {
+      LN_CHECK(gram_f64(ctx, P1, P1, n, k, k, 1, G.data()));
+      double dev = 0.0;
+      for (int64_t i = 0; i < k; ++i)
+        for (int64_t j = 0; j < k; ++j) dev = std::max(dev, std::fabs(G[i * k + j] - (i == j ? 1.0 : 0.0)));
+      if (pass >= 2 && dev < 1e-5) break;
+      std::vector<double> d(k);
+      for (int64_t i = 0; i < k; ++i) d[i] = G[i * k + i] > 0.0 ? 1.0 / std::sqrt(G[i * k + i]) : 1.0;
+      for (int64_t i = 0; i < k; ++i)
+        for (int64_t j = 0; j < k; ++j) Gs[i * k + j] = d[i] * G[i * k + j] * d[j];
+      if (!chol_upper(Gs.data(), (int)k, Rs.data()))
+        return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "FP64 CholeskyQR: Gram matrix is not positive definite");
+      for (int64_t i = 0; i < k; ++i)
+        for (int64_t j = 0; j < k; ++j) R[i * k + j] = Rs[i * k + j] / d[j];
+      tri_upper_inverse(R.data(), (int)k, Rinv.data());
+      LN_CHECK(apply_rows(P1, Rinv.data()));
+      if (pass >= 1 && dev < 1e-1) break;
+    }
+    st.qr_calls++;
+    LN_HIP(ctx, hipEventRecord(e[5], s));
+    LN_CHECK(csr_mm64(ctx, A, P1, k, P0));  // A Q over the dead sketch
+    st.spmm_calls++;
+    LN_HIP(ctx, hipEventRecord(e[6], s));
+    LN_HIP(ctx, hipEventSynchronize(e[6]));
+    t_qr += lap(e[4], e[5]);
+    t_spmm += lap(e[5], e[6]);
+    if (it + 1 < q) std::swap(P0, P1);
+  }
+  // B = P0: B^T B = V diag(S^2) V^T, out = B V[:, :cols] S^-pw
+  LN_HIP(ctx, hipEventRecord(e[7], s));
+  LN_CHECK(gram_f64(ctx, P0, P0, n, k, k, 1, G.data()));
+  auto t0 = std::chrono::steady_clock::now();
+  if (!sym_eig(G.data(), (int)k, w.data(), V.data()))
+    return ctx_fail(ctx, LIGHTNE_ERR_NUMERIC, "lightne_rsvd: eigensolver did not converge");
+  st.t_small_svd = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  S.resize(k);
+  for (int64_t j = 0; j < k; ++j) S[j] = std::sqrt(std::max(w[j], 0.0));
+  // W padded to k columns (k_panel64 needs widths in multiples of 16)
+  const int64_t cw = (cols + 15) / 16 * 16;
+  std::vector<double> W(k * cw, 0.0);
+  for (int64_t j = 0; j < cols; ++j) {
+    if (!(S[j] > 1e-12 * S[0])) continue;
+    const double f = std::pow(S[j], -pw);
+    for (int64_t i = 0; i < k; ++i) W[i * cw + j] = V[i * k + j] * f;
+  }
+  // P1 (Q) is dead: its bytes stage the FP32 rows
+  float* res = reinterpret_cast<float*>(P1);
+  float* res2 = res + (size_t)n * cols;
+  for (int64_t r0 = 0; r0 < n; r0 += blk_rows) {
+    const int64_t rows = std::min(blk_rows, n - r0);
+    LN_CHECK(panel_times_small64(ctx, P0 + r0 * k, rows, k, W.data(), cw, T));
+    if (cw == cols) {
+      LN_CHECK(f64_to_f32(ctx, T, res + r0 * cols, rows * cols));
+    } else {
+      LN_CHECK(f64_to_f32(ctx, T, res2, rows * cw));
+      LN_HIP(ctx, hipMemcpy2DAsync(res + r0 * cols, cols * sizeof(float), res2, cw * sizeof(float),
+                                   cols * sizeof(float), rows, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  if (normalize) {
+    LN_CHECK(u_sigma_root_dev(ctx, res, nullptr, n, cols, cols, 1, res2));
+    res = res2;
+  }
+  LN_HIP(ctx, hipMemcpyAsync(out, res, (size_t)n * cols * sizeof(float), hipMemcpyDefault, s));
+  LN_HIP(ctx, hipEventRecord(e[8], s));
+  LN_HIP(ctx, hipEventSynchronize(e[8]));
+  st.t_spmm = t_spmm;
+  st.t_qr = t_qr;
+  st.t_gemm = lap(e[7], e[8]) - st.t_small_svd;
+  st.t_total = lap(e[0], e[8]);
+  return LIGHTNE_OK;
+}
+
 // FP32 / BF16 schedule up to B = A Q and the eigendecomposition of its FP64
 // Gram.  On return *B_out is B (n x k, in the workspace) with S (descending)
 // and V (k x k, row-major) such that B^T B = V diag(S^2) V^T; with
@@ -429,7 +571,7 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
                     (long long)k);
   if (k > n) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank exceeds n");
   if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_FP64 &&
-      p->precision != LIGHTNE_BF16_POWER)
+      p->precision != LIGHTNE_BF16_POWER && p->precision != LIGHTNE_FP64_LEAN)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
   const int q = std::max(1, (int)p->power_iters);
   CallScope scope(ctx);
@@ -448,6 +590,15 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
     return rc;
   }
   if (p->precision == LIGHTNE_FP64) return rsvd64(ctx, A, p, q, U_out, S_host, stats);
+  if (p->precision == LIGHTNE_FP64_LEAN) {
+    lightne_rsvd_stats st{};
+    std::vector<double> S;
+    LN_CHECK(rsvd64_lean(ctx, A, p, q, k, 1.0, 0, U_out, S, &st));
+    if (S_host && !p->random_project_only)
+      for (int64_t j = 0; j < k; ++j) S_host[j] = (float)S[j];
+    if (stats) *stats = st;
+    return LIGHTNE_OK;
+  }
   lightne_rsvd_stats st{};
   const float* B = nullptr;
   std::vector<double> S, V;
@@ -529,6 +680,20 @@ static int svd_embed(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
     if (rc != LIGHTNE_OK) return rc;
     return u_sigma_root_dev(ctx, U, rp->random_project_only ? nullptr : S.data(), n, k, dim, normalize, emb_out);
   }
+  if (rp->precision == LIGHTNE_FP64_LEAN) {
+    if (k > n || k % 16 || k <= 0 || k > 256 || rp->upper || rp->random_project_only) {
+      lightne_csr_free(A);
+      return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG,
+                      "embedding: the lean FP64 form needs a rank in [16, 256] (multiple of 16), full storage and "
+                      "the whole randomized SVD");
+    }
+    std::vector<double> Sd;
+    rc = rsvd64_lean(ctx, A, rp, std::max(1, (int)rp->power_iters), dim, 0.5, normalize, emb_out, Sd, rst);
+    lightne_csr_free(A);
+    if (rc != LIGHTNE_OK) return rc;
+    for (int64_t j = 0; j < k; ++j) S[j] = (float)Sd[j];
+    return LIGHTNE_OK;
+  }
   // emb = B (V S^-1/2)[:, :dim], row-normalised in the same pass
   if (k > n || k % 16 || k <= 0 || k > 256 || rp->upper ||
       (rp->precision != LIGHTNE_FP32 && rp->precision != LIGHTNE_BF16 && rp->precision != LIGHTNE_BF16_POWER)) {
@@ -579,7 +744,11 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   // stage 2 carves its panels from the workspace front (rsvd_core)
   const size_t pnl = ((size_t)g->n * rp->rank * sizeof(float) + 255) & ~(size_t)255;
   const bool bf = rp->precision == LIGHTNE_BF16 || rp->precision == LIGHTNE_BF16_POWER;
-  const size_t front = rp->precision == LIGHTNE_FP64 ? 4 * 2 * pnl : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
+  const size_t blk64 = ((size_t)std::min<int64_t>(g->n, (int64_t)1 << 20) * rp->rank * sizeof(double) + 255) &
+                       ~(size_t)255;
+  const size_t front = rp->precision == LIGHTNE_FP64        ? 4 * 2 * pnl
+                       : rp->precision == LIGHTNE_FP64_LEAN ? 2 * 2 * pnl + blk64
+                                                            : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
   LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front));
   if (upper) st.sparsify.nnz = st.sparsify.kept_upper;
   auto t1 = std::chrono::steady_clock::now();
