@@ -286,6 +286,17 @@ int lightne_ne_zhang_matrix(lightne_ctx* ctx, const lightne_graph* g, lightne_cs
 /* G_host (k1 x k2, FP64) = X^T Y; symmetric != 0 requires X == Y, k1 == k2 */
 int lightne_dense_gram(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
                        int32_t symmetric, double* G_host);
+/* As lightne_dense_gram, but allocating no device memory: no counterpart in
+ * the reference; for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Its
+ * temporaries (tile-pair list, per-block partial sums, the summed Gram) live in
+ * a 48 MiB device array of the library, one per device, held until the process
+ * ends; k1 and k2 must be multiples of 16 in [16, 256] (partial sums of up to
+ * 94 blocks at 256 x 256; widths up to 144 use the same blocks, and so give the
+ * same bits, as lightne_dense_gram).  Calls that overlap on one device share
+ * the array and race: the caller serialises them. */
+int lightne_dense_gram_noalloc(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                               int32_t symmetric, double* G_host);
 /* Out (n x k2) = X (n x k1) W_host (k1 x k2, used as FP32); optional row L2
  * normalisation; Out may alias X when k1 == k2 */
 int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, const double* W_host,

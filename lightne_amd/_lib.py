@@ -146,6 +146,7 @@ _SIGS = {
                                    C.POINTER(RsvdStats)]),
     "lightne_ne_zhang_matrix": (C.c_int, [_P, C.POINTER(Graph), _PP]),
     "lightne_dense_gram": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, C.c_int32, _P]),
+    "lightne_dense_gram_noalloc": (C.c_int, [_P, _P, _P, _I64, _I64, _I64, C.c_int32, _P]),
     "lightne_dense_panel_gemm": (C.c_int, [_P, _P, _I64, _I64, _P, _I64, _P, C.c_int32]),
     "lightne_dense_cholqr": (C.c_int, [_P, _P, _I64, _I64, _P, C.POINTER(C.c_int32)]),
     "lightne_graph_read_adj": (C.c_int, [C.c_char_p, _PI64, _PI64, _PP, _PP]),

@@ -147,9 +147,15 @@ int lightne_ctx_create(int device, void* stream, lightne_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
-      delete ctx;
+      lightne_ctx_destroy(ctx);
       return global_fail(LIGHTNE_ERR_HIP, "lightne_ctx_create: hipEventCreate failed");
     }
+  }
+  // the allocation-free entry points' scratch array: its address on this device
+  if (dense_scratch_setup(ctx) != LIGHTNE_OK) {
+    std::string msg = ctx->err;
+    lightne_ctx_destroy(ctx);
+    return global_fail(LIGHTNE_ERR_HIP, "lightne_ctx_create: %s", msg.c_str());
   }
   *out = ctx;
   return LIGHTNE_OK;

@@ -17,6 +17,8 @@
 //                      (compute_u_sigma_root, PathEmbed.h:390-406).
 #include "lightne_internal.h"
 
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -403,10 +405,54 @@ int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t coun
   return LIGHTNE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free Gram (lightne_dense_gram_noalloc): the pair list, the
+// per-block partial sums and the reduced sum live in this file-scope device
+// array instead of context buffers.  Each device holds its own copy from the
+// first use until the process ends; its device address is taken once per
+// device at context set-up (dense_scratch_setup) and only that address is
+// used from the host.  Every call on a device shares the array: calls that
+// overlap on one device race.  Nothing in it needs clearing: every entry a
+// call reads was written earlier in the same call.
+// ---------------------------------------------------------------------------
+constexpr size_t SCRATCH_BYTES = (size_t)48 << 20;
+constexpr size_t SCR_PAIRS_BYTES = 4096 * sizeof(int2);
+constexpr size_t SCR_SUM_BYTES = 256 * 256 * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
 namespace {
+constexpr int SCR_MAX_DEVICES = 64;
+std::mutex g_scr_mu;
+void* g_scr_ptr[SCR_MAX_DEVICES] = {};
+}  // namespace
+
+int dense_scratch_setup(lightne_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= SCR_MAX_DEVICES)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "scratch: device %d beyond %d", ctx->device, SCR_MAX_DEVICES);
+  std::lock_guard<std::mutex> g(g_scr_mu);
+  if (!g_scr_ptr[ctx->device]) {
+    void* p = nullptr;
+    LN_HIP(ctx, hipSetDevice(ctx->device));
+    LN_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return ctx_fail(ctx, LIGHTNE_ERR_HIP, "scratch: no device address for the scratch array");
+    g_scr_ptr[ctx->device] = p;
+  }
+  ctx->scratch = g_scr_ptr[ctx->device];
+  return LIGHTNE_OK;
+}
+
+namespace {
+// device memory a Gram call uses in place of context buffers
+struct GramScratch {
+  int2* pairs;
+  double* sum;
+  double* part;
+  int max_blocks;
+};
+
 template <typename TI>
 int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
-              double* G_host) {
+              double* G_host, const GramScratch* scr = nullptr) {
   constexpr int ROWS = GramRows<TI>::R;
   if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 256)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: widths must be multiples of 16 in [16, 256]");
@@ -430,15 +476,26 @@ int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1,
     for (int i = 0; i < ppw * GR_WAVES; ++i) padded.push_back(i < np ? pairs[p0 + i] : make_int2(0, 0));
   }
   int nblk = std::max(1, ctx->num_cus);
+  if (scr) nblk = std::max(1, std::min(nblk, scr->max_blocks));
   int64_t rpb = (n + nblk - 1) / nblk;
   rpb = ((rpb + ROWS - 1) / ROWS) * ROWS;
   if (rpb == 0) rpb = ROWS;
   nblk = (int)std::max<int64_t>(1, (n + rpb - 1) / rpb);
   const int64_t len = k1 * k2;
-  auto* dpairs = (int2*)ctx_buf(ctx, "gram_pairs", std::max<size_t>(padded.size(), 1) * sizeof(int2));
-  auto* part = (double*)ctx_buf(ctx, "gram_part", (size_t)nblk * len * sizeof(double));
-  auto* gsum = (double*)ctx_buf(ctx, "gram_sum", len * sizeof(double));
-  if (!dpairs || !part || !gsum) return LIGHTNE_ERR_OOM;
+  int2* dpairs;
+  double *part, *gsum;
+  if (scr) {
+    if (padded.size() * sizeof(int2) > SCR_PAIRS_BYTES)
+      return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: %zu tile pairs exceed the scratch", padded.size());
+    dpairs = scr->pairs;
+    part = scr->part;
+    gsum = scr->sum;
+  } else {
+    dpairs = (int2*)ctx_buf(ctx, "gram_pairs", std::max<size_t>(padded.size(), 1) * sizeof(int2));
+    part = (double*)ctx_buf(ctx, "gram_part", (size_t)nblk * len * sizeof(double));
+    gsum = (double*)ctx_buf(ctx, "gram_sum", len * sizeof(double));
+    if (!dpairs || !part || !gsum) return LIGHTNE_ERR_OOM;
+  }
   if (!padded.empty())
     LN_HIP(ctx, hipMemcpyAsync(dpairs, padded.data(), padded.size() * sizeof(int2), hipMemcpyHostToDevice,
                                ctx->stream));
@@ -463,8 +520,9 @@ int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1,
       LN_LAUNCH(ctx);
     }
   } else {
-    LN_HIP(ctx, hipMemsetAsync(part, 0, len * sizeof(double), ctx->stream));
-    nblk = 1;
+    // no rows: G = 0 (written on the host, no device fill)
+    std::fill(G_host, G_host + len, 0.0);
+    return LIGHTNE_OK;
   }
   hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, ctx->stream, part, nblk, len,
                      gsum);
@@ -485,6 +543,24 @@ int gram_f64(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_
 int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
              double* G_host) {
   return gram_impl<double>(ctx, X, Y, n, k1, k2, symmetric, G_host);
+}
+
+int gram_f64_noalloc(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                     int symmetric, double* G_host) {
+  if (!ctx->scratch) return ctx_fail(ctx, LIGHTNE_ERR_HIP, "gram: the scratch array was not set up");
+  if (k1 % 16 || k2 % 16 || k1 <= 0 || k2 <= 0 || k1 > 256 || k2 > 256)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG,
+                    "dense_gram_noalloc: widths must be multiples of 16 in [16, 256] (the %zu-byte scratch holds "
+                    "the partial sums of at least %zu blocks at 256 x 256)", SCRATCH_BYTES,
+                    (SCRATCH_BYTES - SCR_PAIRS_BYTES - SCR_SUM_BYTES) / SCR_SUM_BYTES);
+  char* base = (char*)ctx->scratch;  // the device address of `scratch` on this device
+  GramScratch scr;
+  scr.pairs = (int2*)base;
+  scr.sum = (double*)(base + SCR_PAIRS_BYTES);
+  scr.part = (double*)(base + SCR_PAIRS_BYTES + SCR_SUM_BYTES);
+  const size_t part_cap = SCRATCH_BYTES - SCR_PAIRS_BYTES - SCR_SUM_BYTES;
+  scr.max_blocks = (int)std::min<size_t>(1 << 20, part_cap / ((size_t)k1 * k2 * sizeof(double)));
+  return gram_impl<float>(ctx, X, Y, n, k1, k2, symmetric, G_host, &scr);
 }
 
 int panel_times_small(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, int64_t ldx, const double* W_host,

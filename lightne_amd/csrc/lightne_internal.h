@@ -30,6 +30,7 @@ struct lightne_ctx {
   int depth = 0;       // nesting of library calls in progress
   hipEvent_t ev[16] = {};
   int num_cus = 256;
+  void* scratch = nullptr;  // device address of dense.hip's static scratch array on this device
 };
 
 struct lightne_table {
@@ -194,6 +195,12 @@ int f32_to_bf16(lightne_ctx* ctx, const float* in, __hip_bfloat16* out, int64_t 
 // and the conversions at its ends
 int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int64_t k1, int64_t k2, int symmetric,
              double* G_host);
+// the FP32-panel Gram with every temporary in dense.hip's static scratch array
+// (no device allocation); dense_scratch_setup takes the array's device address
+// for ctx->device once per device
+int gram_f64_noalloc(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                     int symmetric, double* G_host);
+int dense_scratch_setup(lightne_ctx* ctx);
 int panel_times_small64(lightne_ctx* ctx, const double* X, int64_t n, int64_t k1, const double* W_host, int64_t k2,
                         double* Out);
 int f32_to_f64(lightne_ctx* ctx, const float* in, double* out, int64_t count);

@@ -631,6 +631,15 @@ int lightne_dense_gram(lightne_ctx* ctx, const float* X, const float* Y, int64_t
   return gram_f64(ctx, X, Y, n, k1, k2, symmetric, G_host);
 }
 
+int lightne_dense_gram_noalloc(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
+                               int32_t symmetric, double* G_host) {
+  if (!ctx || !X || !Y || !G_host || n < 0)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_dense_gram_noalloc: bad argument");
+  CallScope scope(ctx);
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  return gram_f64_noalloc(ctx, X, Y, n, k1, k2, symmetric, G_host);
+}
+
 int lightne_dense_panel_gemm(lightne_ctx* ctx, const float* X, int64_t n, int64_t k1, const double* W_host,
                              int64_t k2, float* Out, int32_t normalize_rows) {
   if (!ctx || !X || !W_host || !Out || n < 0)
