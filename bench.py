@@ -480,9 +480,9 @@ def main():
 
     bf16 = None
     if not args.skip_bf16 or not args.skip_fp64:
-        _, S32, pro32, _, _, _ = run_pipeline(L, g, cfg, ratio, L.FP32, ctx, after_stage2=keep32)
-        keep["pro"] = to_host(pro32) if big else pro32
-        del pro32
+        r32 = run_pipeline(L, g, cfg, ratio, L.FP32, ctx, after_stage2=keep32)
+        keep["pro"] = to_host(r32[2]) if big else r32[2]
+        del r32  # with the in-place propagation emb and pro are one tensor
         torch.cuda.empty_cache()
     if not args.skip_bf16:
         bf16 = {}
@@ -492,7 +492,9 @@ def main():
             chk["stage2_bit_identical_to_fp32"] = bool(np.array_equal(S_, keep["S"])) and same_bits(e, keep["emb"])
 
         run_pipeline(L, g, cfg, ratio, L.BF16, ctx)  # warm
-        _, S16, pro16, stb, pstb, tb = run_pipeline(L, g, cfg, ratio, L.BF16, ctx, after_stage2=cmp16)
+        r16 = run_pipeline(L, g, cfg, ratio, L.BF16, ctx, after_stage2=cmp16)
+        pro16, stb, pstb, tb = r16[2], r16[3], r16[4], r16[5]
+        del r16
         log(f"bf16 step: {tb:.3f}s")
         bf16.update({"seconds": round(tb, 4), "speedup_vs_fp32": round((ms_per_step / 1e3) / tb, 3),
                      "stage2_bit_identical_to_fp32": chk["stage2_bit_identical_to_fp32"],
@@ -502,7 +504,9 @@ def main():
         del pro16
         if cfg["q"] > 1:
             run_pipeline(L, g, cfg, ratio, L.BF16_POWER, ctx)
-            _, Sp, prp, _, _, tp = run_pipeline(L, g, cfg, ratio, L.BF16_POWER, ctx)
+            rp_ = run_pipeline(L, g, cfg, ratio, L.BF16_POWER, ctx)
+            prp, tp = rp_[2], rp_[5]
+            del rp_
             bf16["bf16_power"] = {"seconds": round(tp, 4), "speedup_vs_fp32": round((ms_per_step / 1e3) / tp, 3),
                                   "final_angle_vs_fp32_rad": principal_angle(prp, keep["pro"])}
             del prp

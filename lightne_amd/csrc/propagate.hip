@@ -71,12 +71,13 @@ extern "C" double lightne_bessel_i(int k, double x) { return bessel_i(k, x); }
 // Workspace propagate_forward carves (from offset 0).  FP32: two iterates,
 // the intermediate M T_i and the accumulator (the new iterate overwrites
 // T_{i-1} element by element in the epilogue that reads it).  BF16: the same
-// in BF16 plus an FP32 accumulator and one FP32 panel for the FP32 first step,
-// whose intermediate spans the second iterate and the intermediate.
+// in BF16 plus an FP32 accumulator and two FP32 panels for the FP32 first two
+// steps (T_1, T_2), whose intermediate spans the second iterate and the
+// intermediate.
 size_t prop_ws_bytes(int64_t n, int64_t dim, bool bf) {
   const size_t pe = (size_t)n * dim;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return bf ? 3 * al(pe * 2) + 2 * al(pe * 4) : 4 * al(pe * 4);
+  return bf ? 3 * al(pe * 2) + 3 * al(pe * 4) : 4 * al(pe * 4);
 }
 
 // forward recurrence; y == NULL writes y over the accumulator (internal use)
@@ -90,13 +91,16 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   void* B[2];
   void* tmp = nullptr;
   float* conv = nullptr;
-  float* x32 = nullptr;
+  float *x32 = nullptr, *x32b = nullptr;
   WsCarve wc;
   wc.add(&B[0], pe * esz);
   wc.add(&B[1], pe * esz);
   wc.add(&tmp, pe * esz);
   wc.add((void**)&conv, pe * sizeof(float));
-  if (bf) wc.add((void**)&x32, pe * sizeof(float));
+  if (bf) {
+    wc.add((void**)&x32, pe * sizeof(float));
+    wc.add((void**)&x32b, pe * sizeof(float));
+  }
   LN_CHECK(wc.commit(ctx));
   // FP32 first step: M a into tmp32, T_1 into lx1_32
   float* tmp32 = bf ? (float*)B[1] : (float*)tmp;  // BF16: spans B[1] and tmp
@@ -105,14 +109,8 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   std::vector<float> c;
   cheb_coeffs(p->order, (double)p->theta, c);
   hipStream_t s = ctx->stream;
-  // Lx0 = a (as the gathered type)
-  const void* lx0;
-  if (bf) {
-    LN_CHECK(f32_to_bf16(ctx, a, (__hip_bfloat16*)B[0], (int64_t)pe));
-    lx0 = B[0];
-  } else {
-    lx0 = a;
-  }
+  // Lx0 = a: the FP32 steps read the caller's a
+  const void* lx0 = a;
   SpmmArgs base;
   base.n = n;
   base.k = dim;
@@ -124,9 +122,11 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   base.conv = conv;
   int calls = 0;
   LN_HIP(ctx, hipEventRecord(ctx->ev[13], s));
-  // The first step (T_0 -> T_1, the two largest weights c_0, c_1) always
-  // gathers FP32: in the BF16 form it is the refinement pass; later iterates
-  // are gathered in the panel type.
+  // The first two steps (T_1 and T_2, which carry the three largest weights
+  // c_0 = I_0, c_1 = -2 I_1, c_2 = 2 I_2; c_3 is 2 I_3(0.5) = 5e-3) always
+  // gather and store FP32: in the BF16 form they are the refinement pass, and
+  // the BF16 roundings (2^-9) of the later iterates enter y weighted by
+  // |c_i| <= 5e-3.  Later iterates are gathered in the panel type.
   // tmp = M a   (:101-111)
   {
     SpmmArgs t = base;
@@ -137,7 +137,7 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
     ++calls;
   }
   // Lx1 = 0.5 M tmp - a ; conv = c0 a + c1 Lx1   (:114-148)
-  void* lx1 = B[1];
+  void* lx1 = lx1_32;
   {
     SpmmArgs t = base;
     t.X = tmp32;
@@ -148,39 +148,51 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
     LN_CHECK(spmm_run(ctx, P, t, PANEL_F32, EPI_CHEB1, true));
     ++calls;
   }
-  if (bf) LN_CHECK(f32_to_bf16(ctx, lx1_32, (__hip_bfloat16*)lx1, (int64_t)pe));
   float* yout = y;
   if (K <= 2) {
     hipLaunchKernelGGL(k_y_from_conv, dim3(grid_blocks((int64_t)pe, 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
     LN_LAUNCH(ctx);
   }
   for (int64_t i = 2; i < K; ++i) {
+    // the BF16 form's second FP32 step: T_2 from the FP32 T_1 and a
+    const bool f32_step = bf && i == 2;
+    const int xti = f32_step ? PANEL_F32 : xt;
+    void* tmpi = f32_step ? (void*)tmp32 : tmp;
     // T_{i} overwrites T_{i-2} (never the caller's a)
-    void* lx2 = (lx0 == (const void*)a) ? B[0] : const_cast<void*>(lx0);
+    void* lx2 = f32_step ? (void*)x32b : (lx0 == (const void*)a) ? B[0] : const_cast<void*>(lx0);
     {
       SpmmArgs t = base;  // tmp = M Lx1  (:153-163)
       t.X = lx1;
-      t.out = tmp;
+      t.out = tmpi;
       t.alpha = 1.f;
-      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_STORE, true));
+      LN_CHECK(spmm_run(ctx, P, t, xti, EPI_STORE, true));
       ++calls;
     }
     SpmmArgs t = base;  // Lx2 = M tmp - Lx0 - 2 Lx1 ; conv += c_i Lx2  (:166-197)
-    t.X = tmp;
+    t.X = tmpi;
     t.alpha = 1.f;
     t.c1 = c[i];
     t.lx1 = lx1;
     t.lx0 = lx0;
     if (i + 1 < K) {
       t.lx_out = lx2;
-      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB, true));
+      LN_CHECK(spmm_run(ctx, P, t, xti, EPI_CHEB, true));
     } else {
       t.y_out = yout;
-      LN_CHECK(spmm_run(ctx, P, t, xt, EPI_CHEB_LAST, true));
+      LN_CHECK(spmm_run(ctx, P, t, xti, EPI_CHEB_LAST, true));
     }
     ++calls;
-    lx0 = lx1;
-    lx1 = lx2;
+    if (f32_step && i + 1 < K) {
+      // later steps gather BF16: T_1 and T_2 rounded into the BF16 iterates
+      // (tmp32, spanning B[1] and tmp, is dead)
+      LN_CHECK(f32_to_bf16(ctx, (const float*)lx1, (__hip_bfloat16*)B[0], (int64_t)pe));
+      LN_CHECK(f32_to_bf16(ctx, (const float*)lx2, (__hip_bfloat16*)B[1], (int64_t)pe));
+      lx0 = B[0];
+      lx1 = B[1];
+    } else {
+      lx0 = lx1;
+      lx1 = lx2;
+    }
   }
   LN_HIP(ctx, hipEventRecord(ctx->ev[14], s));
   if (z) {

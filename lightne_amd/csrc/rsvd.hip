@@ -327,6 +327,25 @@ static int rsvd64_lean(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_para
   hipStream_t s = ctx->stream;
   const size_t pd = (size_t)n * k * sizeof(double);
   const int64_t blk_rows = std::min<int64_t>(n, (int64_t)1 << 20);
+  // its own work split with 8192-nnz chunks: FP64 partial sums of the rows
+  // split over several chunks take nparts x k x 8 bytes (29.7 GB with the
+  // default chunks on a 1.5e10-nonzero sparsifier, 2 GB with these)
+  struct Plan64 {
+    SpmmPlan P;
+    ~Plan64() { spmm_plan_free(P); }
+  } plan;
+  LN_CHECK(spmm_plan_build(ctx, A->row_ptr, n, plan.P, 8192));
+  auto mm64 = [&](const double* X, double* Y) -> int {
+    SpmmArgs a;
+    a.n = n;
+    a.k = k;
+    a.rp = A->row_ptr;
+    a.col = A->col;
+    a.val = A->val;
+    a.X = X;
+    a.out = Y;
+    return spmm_run64(ctx, plan.P, a);
+  };
   double *P0 = nullptr, *P1 = nullptr, *T = nullptr;  // Omega, then B; Y / Q; row-block staging
   {
     WsCarve wc;
@@ -356,12 +375,11 @@ static int rsvd64_lean(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_para
     LN_CHECK(f32_to_f64(ctx, tmp32, P0, n * k));
   }
   LN_HIP(ctx, hipEventRecord(e[1], s));
-  LN_CHECK(csr_plan(ctx, A));
   LN_HIP(ctx, hipEventRecord(e[2], s));
   if (p->sparse_project)
     LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, true));
   else
-    LN_CHECK(csr_mm64(ctx, A, P0, k, P1));
+    LN_CHECK(mm64(P0, P1));
   LN_HIP(ctx, hipEventRecord(e[3], s));
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
@@ -401,7 +419,7 @@ static int rsvd64_lean(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_para
     }
     st.qr_calls++;
     LN_HIP(ctx, hipEventRecord(e[5], s));
-    LN_CHECK(csr_mm64(ctx, A, P1, k, P0));  // A Q over the dead sketch
+    LN_CHECK(mm64(P1, P0));  // A Q over the dead sketch
     st.spmm_calls++;
     LN_HIP(ctx, hipEventRecord(e[6], s));
     LN_HIP(ctx, hipEventSynchronize(e[6]));

@@ -367,12 +367,15 @@ __global__ void k_plan_fill(const int64_t* __restrict__ rp, int64_t n, int chunk
 }  // namespace
 
 // Work split for one CSR, built on the device: rows longer than SHORT_NNZ are
-// cut into CHUNK_NNZ chunks; rows with several chunks get partial slots.
-int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P) {
+// cut into chunks of chunk_nnz (CHUNK_NNZ unless given); rows with several
+// chunks get partial slots (nparts x k accumulators per product).
+int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P, int chunk_nnz) {
+  if (chunk_nnz <= 0) chunk_nnz = CHUNK_NNZ;
+  if (chunk_nnz < SHORT_NNZ) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: chunk below the short-row length");
   spmm_plan_free(P);
   P.n = n;
   P.short_nnz = SHORT_NNZ;
-  P.chunk_nnz = CHUNK_NNZ;
+  P.chunk_nnz = chunk_nnz;
   if (n >= ((int64_t)1 << 31)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: more than 2^31 rows");
   hipStream_t s = ctx->stream;
   auto* nch = (uint32_t*)ctx_buf(ctx, "plan_nch", std::max<int64_t>(n, 1) * 4);
@@ -383,7 +386,7 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
   auto* poff = (uint64_t*)ctx_buf(ctx, "plan_poff", (n + 1) * 8);
   if (!nch || !mul || !par || !coff || !moff || !poff) return LIGHTNE_ERR_OOM;
   const unsigned nb = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(256), 0, s, rp_dev, n, SHORT_NNZ, CHUNK_NNZ, nch, mul, par);
+  hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(256), 0, s, rp_dev, n, SHORT_NNZ, chunk_nnz, nch, mul, par);
   LN_LAUNCH(ctx);
   LN_CHECK(scan_u32_to_u64(ctx, nch, coff, n));
   LN_CHECK(scan_u32_to_u64(ctx, mul, moff, n));
@@ -407,7 +410,7 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
   LN_CHECK(al((void**)&P.chunk_slot, P.nchunks * sizeof(int32_t)));
   LN_CHECK(al((void**)&P.multi_row, P.nmulti * sizeof(int32_t)));
   LN_CHECK(al((void**)&P.multi_first, (P.nmulti + 1) * sizeof(int64_t)));
-  hipLaunchKernelGGL(k_plan_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, CHUNK_NNZ, (const uint64_t*)coff,
+  hipLaunchKernelGGL(k_plan_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, chunk_nnz, (const uint64_t*)coff,
                      (const uint64_t*)moff, (const uint64_t*)poff, P.chunk_start, P.chunk_row, P.chunk_slot,
                      P.multi_row, P.multi_first);
   LN_LAUNCH(ctx);

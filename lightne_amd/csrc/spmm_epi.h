@@ -59,7 +59,8 @@ struct SpmmArgs {
   float* partial = nullptr;
 };
 
-int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P);
+// chunk_nnz <= 0: the default chunk length
+int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan& P, int chunk_nnz = 0);
 void spmm_plan_free(SpmmPlan& P);
 int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, bool impl);
 // FP64 form: out(double) = A X(double), explicit CSR values
