@@ -159,12 +159,14 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
     bounded sample, reported as an estimate with every factor named:
       * sampler + trunc-log/CSR on every (1/frac)-th source vertex, scaled by
         (all directed edges / sampled directed edges);
-      * randomized SVD: both sparse products on the rows holding the first
+      * randomized SVD: the sparse products on the rows holding the first
         `frac` of the sparsifier's nonzeros (scaled by nnz), the tall QRs and
-        GEMMs on those rows (scaled by rows), the k x k SVD once;
-      * propagation: the 19 sparse products on the rows holding `frac` of
-        the operator's nonzeros (scaled by nnz), the vector updates and the
-        tall SVD on those rows (scaled by rows)."""
+        GEMMs on the first `frac` of the rows (scaled by rows; dense work
+        does not depend on the values), the k x k SVD once;
+      * propagation: one of the 19 sparse products on the rows holding `frac`
+        of the operator's nonzeros (scaled by nnz and by 19), the vector
+        updates and the tall SVD on the first `frac` of the rows (scaled by
+        rows)."""
     from oracle import mkl_ref as K
     from oracle import oracle as O
 
@@ -201,6 +203,7 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
     # stage 2
     r2 = n if full else int(np.searchsorted(rp, frac * nnz))
     nnz2 = int(rp[r2])
+    rr = n if full else max(1, min(n, int(frac * n)))  # rows of the dense sample
     Om = O.gaussian(777, 0, n * k).reshape(n, k)
     Pm = O.gaussian(777, 1, k * k).reshape(k, k)
     if use_mkl:
@@ -208,20 +211,24 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
         t0 = time.perf_counter()
         Y = A.mm(Om)
         t_sp = time.perf_counter() - t0
+        if not full:  # the dense steps on the first rr rows of a panel of the same shape
+            Y = np.ascontiguousarray(Om[:rr])
         t0 = time.perf_counter()
         K.gram_schmidt(Y)
         t_rows = time.perf_counter() - t0
         for _ in range(q - 1):
             t0 = time.perf_counter()
-            Yf = Y if full else Om  # a row sample multiplies an n x k panel of the same shape
-            Y = A.mm(Yf)
+            Y2 = A.mm(Y if full else Om)  # a row sample multiplies an n x k panel of the same shape
             t_sp += time.perf_counter() - t0
+            Y = Y2 if full else np.ascontiguousarray(Om[:rr])
             t0 = time.perf_counter()
             K.gram_schmidt(Y)
             t_rows += time.perf_counter() - t0
         t0 = time.perf_counter()
         B = A.mm(Y if full else Om)
         t_sp += time.perf_counter() - t0
+        if not full:
+            B = np.ascontiguousarray(Om[:rr])
         t0 = time.perf_counter()
         Z = K.sgemm(B, Pm)
         K.gram_schmidt(Z)
@@ -254,10 +261,10 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
         U = Z @ Uc
         t_rows += time.perf_counter() - t0
         del Y, Q, Z, Cm
-    s2n, s2r = nnz / max(nnz2, 1), n / max(r2, 1)
+    s2n, s2r = nnz / max(nnz2, 1), n / max(rr, 1)
     t_svd = t_sp * s2n + t_rows * s2r + t_small
     if not full:
-        note.append(f"rSVD sparse products on {nnz2}/{nnz} nnz ({t_sp:.2f} s, x{s2n:.2f}), QR/GEMM on {r2}/{n} rows "
+        note.append(f"rSVD sparse products on {nnz2}/{nnz} nnz ({t_sp:.2f} s, x{s2n:.2f}), QR/GEMM on {rr}/{n} rows "
                     f"({t_rows:.2f} s, x{s2r:.2f}), k x k SVD once")
     log(f"cpu: rsvd {t_svd:.2f}s")
     del rp, col, val
@@ -288,23 +295,23 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
             t0 = time.perf_counter()
             M.mm(Xd, 1.0, 0.0, Yr, cols=d)
             t_p = time.perf_counter() - t0
-            ar = np.ascontiguousarray(Om[:r3, :d])
+            ar = np.ascontiguousarray(Om[:rr, :d])
             t0 = time.perf_counter()
             conv = np.zeros_like(ar)
             Lx2 = ar.copy()
-            for i in range(2, cfg["K"]):
+            for i in range(2, cfg["K"]):  # the recurrence's per-step vector work (SpectralPropagation.h:164-197)
                 np.copyto(Lx2, ar)
-                K.saxpy(-2.0, Yr, Lx2)
+                K.saxpy(-2.0, ar, Lx2)
                 K.saxpy(float(np.float32(c[i])), Lx2, conv)
             t_u = time.perf_counter() - t0
             t0 = time.perf_counter()
             K.gesvd_left(ar.copy())
             t_tall = time.perf_counter() - t0
-            s3n, s3r = nnz3_all / max(nnz3, 1), n / max(r3, 1)
+            s3n, s3r = nnz3_all / max(nnz3, 1), n / max(rr, 1)
             t_prop = t_p * s3n * (2 * cfg["K"] - 1) + t_u * s3r
             t_tall *= s3r
             note.append(f"propagation: 1 of {2 * cfg['K'] - 1} products on {nnz3}/{nnz3_all} operator nnz "
-                        f"({t_p:.2f} s, x{s3n:.2f} x{2 * cfg['K'] - 1}), updates and tall SVD on {r3}/{n} rows "
+                        f"({t_p:.2f} s, x{s3n:.2f} x{2 * cfg['K'] - 1}), updates and tall SVD on {rr}/{n} rows "
                         f"(x{s3r:.2f})")
             del Yr, ar, conv, Lx2
         M.close()
