@@ -525,10 +525,8 @@ def main():
         # lean two-panel form writing its embedding to host memory
         prec = L.FP64_LEAN if big else L.FP64
         if big:
-            ctx.close()
-            L.api._CTX.pop(0, None)
+            ctx.release()  # the FP32 pipeline's workspace
             torch.cuda.empty_cache()
-            ctx = L.context(0)
         log("fp64 reference ...")
         try:
             out64 = np.empty((n, d), np.float32) if big else None
@@ -537,10 +535,7 @@ def main():
             torch.cuda.synchronize()
             S64 = S64.astype(np.float64)
             if big:
-                ctx.close()
-                L.api._CTX.pop(0, None)
-                torch.cuda.empty_cache()
-                ctx = L.context(0)
+                ctx.release()
             ang = principal_angle(keep["emb"], emb64)
             sv = float(np.max(np.abs(keep["S"][:d] - S64[:d]) / S64[:d]))
             fp64 = {"form": "lean FP64 (two panels)" if big else "explicit FP64 (the reference's Z/C/U schedule)",
@@ -591,10 +586,8 @@ def main():
         parts = None
         if cfg["cpu_frac"] < 1.0:
             # the rows of the sparsifier the sampled CPU products use
-            ctx.close()
-            L.api._CTX.pop(0, None)
+            ctx.release()
             torch.cuda.empty_cache()
-            ctx = L.context(0)
             csr, _ = L.generate_trunc_log_matrix(g, 1, walk_len=cfg["T"], sample_ratio=ratio, ctx=ctx)
             parts = csr.to_numpy(nnz_fraction=cfg["cpu_frac"])
             csr.free()

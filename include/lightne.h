@@ -79,6 +79,11 @@ typedef struct lightne_csr lightne_csr;
 int lightne_ctx_create(int device, void* stream, lightne_ctx** out);
 int lightne_ctx_destroy(lightne_ctx* ctx);
 int lightne_ctx_synchronize(lightne_ctx* ctx);
+/* Free the device memory the context keeps between calls (its workspace and
+ * named scratch buffers; the next call allocates what it needs again).  No
+ * reference counterpart: the reference frees its buffers at the end of each
+ * stage. */
+int lightne_ctx_release(lightne_ctx* ctx);
 /* Text of the last error on this context ("" if none). */
 const char* lightne_last_error(const lightne_ctx* ctx);
 /* Text of the last error raised outside any context (create, graph IO). */

@@ -117,6 +117,7 @@ _SIGS = {
     "lightne_ctx_create": (C.c_int, [C.c_int, _P, _PP]),
     "lightne_ctx_destroy": (C.c_int, [_P]),
     "lightne_ctx_synchronize": (C.c_int, [_P]),
+    "lightne_ctx_release": (C.c_int, [_P]),
     "lightne_sample_scalars": (C.c_int, [_I64, _I64, C.POINTER(SparsifyParams), C.POINTER(SparsifyStats)]),
     "lightne_sparsify_counts": (C.c_int, [_P, C.POINTER(Graph), C.POINTER(SparsifyParams), _PP,
                                           C.POINTER(SparsifyStats)]),

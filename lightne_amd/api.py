@@ -267,6 +267,12 @@ class Context:
     def check(self, rc: int) -> None:
         check(rc, self.ptr)
 
+    def release(self) -> None:
+        """Free the device memory the context keeps between calls (workspace,
+        scratch); the next call allocates again."""
+        if self.ptr:
+            self.check(self.lib.lightne_ctx_release(self.ptr))
+
     def close(self) -> None:
         if self.ptr:
             self.raw.lightne_ctx_destroy(self.ptr)

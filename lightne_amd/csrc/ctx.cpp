@@ -173,6 +173,16 @@ int lightne_ctx_destroy(lightne_ctx* ctx) {
   return LIGHTNE_OK;
 }
 
+int lightne_ctx_release(lightne_ctx* ctx) {
+  if (!ctx) return global_fail(LIGHTNE_ERR_BAD_ARG, "ctx is NULL");
+  LN_HIP(ctx, hipSetDevice(ctx->device));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  ctx->bufs.clear();
+  return LIGHTNE_OK;
+}
+
 int lightne_ctx_synchronize(lightne_ctx* ctx) {
   if (!ctx) return global_fail(LIGHTNE_ERR_BAD_ARG, "ctx is NULL");
   LN_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -44,3 +44,20 @@ def sign_align(X, ref):
     s = np.sign(np.sum(np.asarray(X, np.float64) * np.asarray(ref, np.float64), axis=0))
     s[s == 0] = 1.0
     return X * s[None, :]
+
+
+def same_bits(A, B, chunk=1 << 20) -> bool:
+    """A and B (torch device tensors or host arrays, float32) hold the same
+    bits, compared in row chunks on the device."""
+    import torch
+
+    def dev(X, r):
+        x = X[r:r + chunk]
+        return torch.from_numpy(np.ascontiguousarray(x)).cuda() if isinstance(x, np.ndarray) else x
+
+    if tuple(A.shape) != tuple(B.shape):
+        return False
+    for r in range(0, A.shape[0], chunk):
+        if not torch.equal(dev(A, r).view(torch.int32), dev(B, r).view(torch.int32)):
+            return False
+    return True
