@@ -169,6 +169,7 @@ int lightne_ctx_destroy(lightne_ctx* ctx) {
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  wide_handle_free(ctx);
   delete ctx;
   return LIGHTNE_OK;
 }

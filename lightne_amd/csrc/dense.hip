@@ -72,6 +72,25 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
   }
 }
 
+// the same n x k sketch written into a panel of row stride ld: element e of
+// the stream goes to row e / k, column e % k
+__global__ void k_gaussian_ld(uint64_t seed, uint32_t stream, int64_t n, int64_t k, int64_t ld,
+                              float* __restrict__ out) {
+  const int64_t count = n * k;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * p < count;
+       p += (int64_t)gridDim.x * blockDim.x) {
+  uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+  const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * log(u1));
+  const double t = 6.283185307179586476925286766559 * u2;
+  const int64_t e0 = 2 * p, e1 = 2 * p + 1;
+  out[(e0 / k) * ld + e0 % k] = round_bf16((float)(r * cos(t)));
+  if (e1 < count) out[(e1 / k) * ld + e1 % k] = round_bf16((float)(r * sin(t)));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Gram: rows are staged through LDS in 32-KB slabs; every wave owns a
 // fixed subset of the 16x16 output tiles and walks the slab 4 rows at a time.
@@ -394,6 +413,16 @@ void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* ou
     out[2 * p] = round_bf16((float)(r * std::cos(t)));
     if (2 * p + 1 < count) out[2 * p + 1] = round_bf16((float)(r * std::sin(t)));
   }
+}
+
+int gaussian_fill_ld(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t n, int64_t k, int64_t ld,
+                     float* out) {
+  if (n <= 0 || k <= 0) return LIGHTNE_OK;
+  const int64_t pairs = (n * k + 1) / 2;
+  hipLaunchKernelGGL(k_gaussian_ld, dim3(grid_blocks(pairs, 256)), dim3(256), 0, ctx->stream, seed, stream, n, k,
+                     ld, out);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
 }
 
 int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out) {

@@ -31,6 +31,7 @@ struct lightne_ctx {
   hipEvent_t ev[16] = {};
   int num_cus = 256;
   void* scratch = nullptr;  // device address of dense.hip's static scratch array on this device
+  void* blas = nullptr;     // rocblas_handle of the wide-rank stage 2 (wide.hip), created on first use
 };
 
 struct lightne_table {
@@ -176,6 +177,9 @@ enum PanelType { PANEL_F32 = 0, PANEL_BF16 = 1 };
 // dense panels (dense.hip)
 // ---------------------------------------------------------------------------
 int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out);
+// the same values as gaussian_fill(count = n k) in an n x k panel of row stride ld
+int gaussian_fill_ld(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t n, int64_t k, int64_t ld,
+                     float* out);
 void gaussian_fill_host(uint64_t seed, uint32_t stream, int64_t count, float* out);
 // G(k1 x k2, f64, host) = X^T Y for X: n x k1, Y: n x k2 (row-major f32 device
 // panels); symmetric=1 when X == Y (only the upper tiles are computed).
@@ -201,6 +205,24 @@ int gram_f64(lightne_ctx* ctx, const double* X, const double* Y, int64_t n, int6
 int gram_f64_noalloc(lightne_ctx* ctx, const float* X, const float* Y, int64_t n, int64_t k1, int64_t k2,
                      int symmetric, double* G_host);
 int dense_scratch_setup(lightne_ctx* ctx);
+
+// ---------------------------------------------------------------------------
+// stage 2 for sketch widths outside the 16-wide tiling (wide.hip)
+// ---------------------------------------------------------------------------
+// true when rank k takes the wide path (k > 256 or k not a multiple of 16)
+inline bool rsvd_is_wide(int64_t k) { return k > 256 || k % 16 != 0; }
+// the randomized SVD with the sketch padded to a multiple of 16 columns:
+// out (row-major n x cols, row stride ldo, device) = B V[:, :cols] S^-pw (row
+// normalised when asked), or Y = A Omega (cols = k) with random_project_only;
+// S receives the k singular values, descending
+int rsvd_wide(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p, int64_t cols, double pw,
+              int normalize, float* out, int64_t ldo, std::vector<double>& S, lightne_rsvd_stats* st);
+// workspace front the wide path carves (two padded panels and the BF16 shadow)
+size_t rsvd_wide_front(int64_t n, int64_t k, int precision);
+void wide_handle_free(lightne_ctx* ctx);
+// get_embedding_dense of mat (n x rank, row stride ld) for any rank and dim
+int embed_dense_wide(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t ld, int64_t dim,
+                     float* emb_out, float* S_host);
 int panel_times_small64(lightne_ctx* ctx, const double* X, int64_t n, int64_t k1, const double* W_host, int64_t k2,
                         double* Out);
 int f32_to_f64(lightne_ctx* ctx, const float* in, double* out, int64_t count);

@@ -46,8 +46,7 @@ __global__ void k_y_from_conv(const float* __restrict__ a, const float* __restri
 int check_prop(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim, const lightne_prop_params* p) {
   if (!ctx || !g || !a || !p) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: NULL argument");
   if (g->n <= 0 || !g->offsets || (g->m > 0 && !g->edges)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: bad graph");
-  if (dim <= 0 || dim % 16 || dim > 1024)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: dim must be a multiple of 16 in [16, 1024]");
+  if (dim <= 0 || dim > 1024) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: dim must be in [1, 1024]");
   if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_BF16_POWER)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "propagate: FP32, BF16 or BF16_POWER iterates");
   return LIGHTNE_OK;
@@ -220,8 +219,10 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
 // panel product with the row-normalisation epilogue.  mat is not modified.
 int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim, float* emb_out,
                     float* S_host) {
-  if (rank % 16 || dim % 16 || dim > rank || rank > 256 || n < rank)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embed_dense: rank, dim must be multiples of 16, dim <= rank <= 256, n >= rank");
+  if (dim <= 0 || dim > rank || n < rank)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embed_dense: need 1 <= dim <= rank <= n");
+  if (rsvd_is_wide(rank) || dim % 16)  // wide.hip: rocBLAS / rocSOLVER on FP64 row blocks
+    return embed_dense_wide(ctx, mat, n, rank, rank, dim, emb_out, S_host);
   std::vector<double> G(rank * rank), V(rank * rank), w(rank), S(rank);
   LN_CHECK(gram_f64(ctx, mat, mat, n, rank, rank, 1, G.data()));
   if (!sym_eig(G.data(), (int)rank, w.data(), V.data()))
@@ -242,6 +243,23 @@ int embed_dense_dev(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank,
   return LIGHTNE_OK;
 }
 
+// A dim that is not a multiple of 16 runs on a copy padded with zero columns:
+// the operator acts on each column alone, so the padding stays zero and the
+// first dim columns are the result.  Returns the padded copy (context buffer)
+// and its width.
+static int pad_columns(lightne_ctx* ctx, const float* a, int64_t n, int64_t dim, const char* name, const float** ap,
+                       int64_t* dp) {
+  *dp = (dim + 15) / 16 * 16;
+  *ap = a;
+  if (*dp == dim) return LIGHTNE_OK;
+  auto* buf = (float*)ctx_buf(ctx, name, (size_t)n * *dp * sizeof(float));
+  if (!buf) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemset2DAsync(buf + dim, *dp * 4, 0, (*dp - dim) * 4, n, ctx->stream));
+  LN_HIP(ctx, hipMemcpy2DAsync(buf, *dp * 4, a, dim * 4, dim * 4, n, hipMemcpyDeviceToDevice, ctx->stream));
+  *ap = buf;
+  return LIGHTNE_OK;
+}
+
 extern "C" {
 
 int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
@@ -253,7 +271,22 @@ int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
   lightne_prop_stats st{};
-  LN_CHECK(propagate_forward(ctx, g, a, dim, p, y_out, z_out, &st, gp.P));
+  const float* ap = a;
+  int64_t dp = dim;
+  LN_CHECK(pad_columns(ctx, a, g->n, dim, "prop_pad_a", &ap, &dp));
+  if (dp != dim) {
+    const size_t pb = (size_t)g->n * dp * sizeof(float);
+    float* yp = y_out ? (float*)ctx_buf(ctx, "prop_pad_y", pb) : nullptr;
+    float* zp = z_out ? (float*)ctx_buf(ctx, "prop_pad_z", pb) : nullptr;
+    if ((y_out && !yp) || (z_out && !zp)) return LIGHTNE_ERR_OOM;
+    LN_CHECK(propagate_forward(ctx, g, ap, dp, p, yp, zp, &st, gp.P));
+    hipStream_t s = ctx->stream;
+    if (y_out) LN_HIP(ctx, hipMemcpy2DAsync(y_out, dim * 4, yp, dp * 4, dim * 4, g->n, hipMemcpyDeviceToDevice, s));
+    if (z_out) LN_HIP(ctx, hipMemcpy2DAsync(z_out, dim * 4, zp, dp * 4, dim * 4, g->n, hipMemcpyDeviceToDevice, s));
+    LN_HIP(ctx, hipStreamSynchronize(s));
+  } else {
+    LN_CHECK(propagate_forward(ctx, g, a, dim, p, y_out, z_out, &st, gp.P));
+  }
   auto t1 = std::chrono::steady_clock::now();
   st.t_total = std::chrono::duration<double>(t1 - t0).count();
   if (stats) *stats = st;
@@ -277,16 +310,22 @@ int lightne_chebyshev_expansion(lightne_ctx* ctx, const lightne_graph* g, const 
   auto t0 = std::chrono::steady_clock::now();
   GraphPlan gp;
   LN_CHECK(spmm_plan_build(ctx, g->offsets, g->n, gp.P));
+  // a dim that is not a multiple of 16 runs on a zero-padded copy of width dp
+  const float* ap = a;
+  int64_t dp = dim;
+  LN_CHECK(pad_columns(ctx, a, g->n, dim, "prop_pad_a", &ap, &dp));
   // z after the recurrence's own workspace (y is written over its
   // accumulator)
   float* z = nullptr;
-  WsCarve wc(prop_ws_bytes(g->n, dim, p->precision != LIGHTNE_FP32));
-  wc.add((void**)&z, (size_t)g->n * dim * sizeof(float));
+  WsCarve wc(prop_ws_bytes(g->n, dp, p->precision != LIGHTNE_FP32));
+  wc.add((void**)&z, (size_t)g->n * dp * sizeof(float));
   LN_CHECK(wc.commit(ctx));
   lightne_prop_stats st{};
-  LN_CHECK(propagate_forward(ctx, g, a, dim, p, nullptr, z, &st, gp.P));
+  LN_CHECK(propagate_forward(ctx, g, ap, dp, p, nullptr, z, &st, gp.P));
   auto t1 = std::chrono::steady_clock::now();
-  LN_CHECK(embed_dense_dev(ctx, z, g->n, dim, dim, emb_out, nullptr));
+  // the zero padding columns of z add zero singular values only: the top dim
+  // singular triplets of the padded z are z's
+  LN_CHECK(embed_dense_dev(ctx, z, g->n, dp, dim, emb_out, nullptr));
   auto t2 = std::chrono::steady_clock::now();
   st.t_svd = std::chrono::duration<double>(t2 - t1).count();
   st.t_total = std::chrono::duration<double>(t2 - t0).count();

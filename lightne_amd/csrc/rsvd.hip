@@ -584,10 +584,8 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
   if (!ctx || !Ac || !p || !U_out) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: NULL argument");
   auto* A = const_cast<lightne_csr*>(Ac);
   const int64_t n = A->n, k = p->rank;
-  if (k <= 0 || k % 16 || k > 256)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank must be a multiple of 16 in [16, 256] (got %lld)",
-                    (long long)k);
-  if (k > n) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank exceeds n");
+  if (k <= 0 || k > n)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: rank must be in [1, n] (got %lld)", (long long)k);
   if (p->precision != LIGHTNE_FP32 && p->precision != LIGHTNE_BF16 && p->precision != LIGHTNE_FP64 &&
       p->precision != LIGHTNE_BF16_POWER && p->precision != LIGHTNE_FP64_LEAN)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "lightne_rsvd: unknown precision");
@@ -606,6 +604,15 @@ int lightne_rsvd(lightne_ctx* ctx, const lightne_csr* Ac, const lightne_rsvd_par
       delete F.plan;
     }
     return rc;
+  }
+  if (rsvd_is_wide(k)) {  // k > 256 or not a multiple of 16 (wide.hip)
+    lightne_rsvd_stats st{};
+    std::vector<double> S;
+    LN_CHECK(rsvd_wide(ctx, A, p, k, 1.0, 0, U_out, k, S, &st));
+    if (S_host && !p->random_project_only)
+      for (int64_t j = 0; j < k; ++j) S_host[j] = (float)S[j];
+    if (stats) *stats = st;
+    return LIGHTNE_OK;
   }
   if (p->precision == LIGHTNE_FP64) return rsvd64(ctx, A, p, q, U_out, S_host, stats);
   if (p->precision == LIGHTNE_FP64_LEAN) {
@@ -699,6 +706,24 @@ static int svd_embed(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   const int64_t n = A->n, k = rp->rank;
   S.assign(k, 0.f);
   int rc = LIGHTNE_OK;
+  if (rsvd_is_wide(k) && !rp->upper) {  // k > 256 or not a multiple of 16 (wide.hip)
+    if (k > n) {
+      lightne_csr_free(A);
+      return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "embedding: rank exceeds n");
+    }
+    std::vector<double> Sd;
+    if (rp->random_project_only) {  // Y = A Omega, then U sqrt(S) with S = 1
+      auto* Y = (float*)ctx_buf(ctx, "netsmf_y", (size_t)n * k * sizeof(float));
+      rc = Y ? rsvd_wide(ctx, A, rp, k, 1.0, 0, Y, k, Sd, rst) : LIGHTNE_ERR_OOM;
+      if (rc == LIGHTNE_OK) rc = u_sigma_root_dev(ctx, Y, nullptr, n, k, dim, normalize, emb_out);
+    } else {
+      rc = rsvd_wide(ctx, A, rp, dim, 0.5, normalize, emb_out, dim, Sd, rst);
+    }
+    lightne_csr_free(A);
+    if (rc != LIGHTNE_OK) return rc;
+    for (int64_t j = 0; j < (int64_t)Sd.size(); ++j) S[j] = (float)Sd[j];
+    return LIGHTNE_OK;
+  }
   if (rp->precision == LIGHTNE_FP64) {
     // accuracy reference: explicit U, then U sqrt(S)
     auto* U = (float*)ctx_buf(ctx, "netsmf_u", (size_t)n * k * sizeof(float));
@@ -773,7 +798,8 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   const bool bf = rp->precision == LIGHTNE_BF16 || rp->precision == LIGHTNE_BF16_POWER;
   const size_t blk64 = ((size_t)std::min<int64_t>(g->n, (int64_t)1 << 20) * rp->rank * sizeof(double) + 255) &
                        ~(size_t)255;
-  const size_t front = rp->precision == LIGHTNE_FP64        ? 4 * 2 * pnl
+  const size_t front = rsvd_is_wide(rp->rank)              ? rsvd_wide_front(g->n, rp->rank, rp->precision)
+                       : rp->precision == LIGHTNE_FP64        ? 4 * 2 * pnl
                        : rp->precision == LIGHTNE_FP64_LEAN ? 2 * 2 * pnl + blk64
                                                             : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
   LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front));

@@ -107,7 +107,7 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
     }
     float x[GATHER_U][16];
 #pragma unroll
-    for (int u = 0; u < GATHER_U; ++u) load16<XT>(X + c[u] * a.k, sub, G, x[u]);
+    for (int u = 0; u < GATHER_U; ++u) load16<XT>(X + c[u] * a.ldx, sub, G, x[u]);
 #pragma unroll
     for (int u = 0; u < GATHER_U; ++u)
 #pragma unroll
@@ -123,7 +123,7 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
       w0 = a.val[j];
     }
     float x0[16];
-    load16<XT>(X + c0 * a.k, sub, G, x0);
+    load16<XT>(X + c0 * a.ldx, sub, G, x0);
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] += w0 * x0[i];
   }
@@ -434,6 +434,12 @@ int spmm_run(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a, int xt, int epi, b
   if (a.k % 16 != 0 || a.k < 16 || a.k > 1024)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel width %lld must be a multiple of 16 in [16, 1024]",
                     (long long)a.k);
+  if (a.ldx == 0) a.ldx = a.k;
+  if (a.ldo == 0) a.ldo = a.k;
+  if (a.ldx < a.k || a.ldo < a.k || a.ldx % 16 || a.ldo % 16)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: row strides must be multiples of 16 of at least the width");
+  if ((a.ldx != a.k || a.ldo != a.k) && epi != EPI_STORE && epi != EPI_STORE_F32)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: fused epilogues need contiguous panels");
   if (impl && (epi == EPI_CHEB1 || epi == EPI_CHEB || epi == EPI_CHEB_LAST || epi == EPI_ADJ) &&
       a.impl_mode != IMPL_PROP)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "Chebyshev epilogue needs the propagation operator");
@@ -451,6 +457,8 @@ int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   if (a.k % 16 != 0 || a.k < 16 || a.k > 1024)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "panel width %lld must be a multiple of 16 in [16, 1024]",
                     (long long)a.k);
+  if ((a.ldx && a.ldx != a.k) || (a.ldo && a.ldo != a.k))
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: the FP64 form needs contiguous panels");
   a.short_nnz = P.short_nnz;
   a.nchunks = P.nchunks;
   a.chunk_start = P.chunk_start;

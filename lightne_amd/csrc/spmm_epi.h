@@ -32,6 +32,10 @@ struct SpmmPlan {
 
 struct SpmmArgs {
   int64_t n = 0, k = 0;
+  // row strides of X and of out in elements (0: k).  A stride above k lets a
+  // product run on a column slab of a wider panel; the Chebyshev and adjoint
+  // epilogues need contiguous panels (strides k).
+  int64_t ldx = 0, ldo = 0;
   const int64_t* rp = nullptr;  // row pointers (CSR or graph offsets)
   const int32_t* col = nullptr;
   const float* val = nullptr;
@@ -113,7 +117,7 @@ __device__ __forceinline__ void epilogue(const SpmmArgs& a, int64_t r, int c, fl
   const int64_t o = r * a.k + c;
   if (IMPL) {
     float xs[4];
-    load4<XT>(reinterpret_cast<const XT*>(a.X) + o, xs);
+    load4<XT>(reinterpret_cast<const XT*>(a.X) + r * a.ldx + c, xs);
     if (a.impl_mode == IMPL_PROP) {
       // SpectralPropagation.h:62,66,72: FP degree = d+1; values formed in double
       const double deg = (double)(float)(a.rp[r + 1] - a.rp[r] + 1);
@@ -129,9 +133,9 @@ __device__ __forceinline__ void epilogue(const SpmmArgs& a, int64_t r, int c, fl
 #pragma unroll
   for (int i = 0; i < 4; ++i) v[i] *= a.alpha;
   if (EPI == EPI_STORE) {
-    store4<XT>(reinterpret_cast<XT*>(a.out) + o, v);
+    store4<XT>(reinterpret_cast<XT*>(a.out) + r * a.ldo + c, v);
   } else if (EPI == EPI_STORE_F32) {
-    store4<float>(reinterpret_cast<float*>(a.out) + o, v);
+    store4<float>(reinterpret_cast<float*>(a.out) + r * a.ldo + c, v);
   } else if (EPI == EPI_CHEB1) {
     float av[4], cv[4];
     load4<float>(a.a_in + o, av);
