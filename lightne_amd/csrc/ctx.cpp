@@ -144,7 +144,10 @@ int lightne_ctx_create(int device, void* stream, lightne_ctx** out) {
   ctx->device = device;
   ctx->stream = (hipStream_t)stream;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->num_cus = prop.multiProcessorCount;
+    ctx->lds_per_block = prop.sharedMemPerBlock;
+  }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       lightne_ctx_destroy(ctx);

@@ -104,7 +104,10 @@ constexpr int GR_WAVES = GR_THREADS / 64;
 constexpr int GR_MAX_PAIRS_PER_WAVE = 12;
 constexpr int GR_MAX_V = 9;  // 16-B vectors per thread for one slab: 2 x 32 KB / 16 B / 512 = 8, +1
 
-// rows per LDS slab: 32 FP32 rows or 16 FP64 rows (the same bytes)
+// rows per slab: 32 rows of an FP32 panel, 16 of an FP64 one (the same global
+// bytes per load round).  Both are staged in LDS as FP64, so an FP32 slab takes
+// twice the LDS of an FP64 one: 32 x (k1 + k2) x 8 bytes, 128 KiB for two
+// different 256-wide panels, checked against the device's limit per block.
 template <typename TI>
 struct GramRows {
   static constexpr int R = 128 / sizeof(TI);
@@ -529,6 +532,9 @@ int gram_impl(lightne_ctx* ctx, const TI* X, const TI* Y, int64_t n, int64_t k1,
     LN_HIP(ctx, hipMemcpyAsync(dpairs, padded.data(), padded.size() * sizeof(int2), hipMemcpyHostToDevice,
                                ctx->stream));
   const size_t lds = (size_t)ROWS * (k1 + (X == Y ? 0 : k2)) * sizeof(double);  // staged as FP64
+  if (lds > ctx->lds_per_block)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "gram: a %zu-byte LDS slab exceeds the device's %zu bytes per block",
+                    lds, ctx->lds_per_block);
   if (n > 0) {
     // each launch owns a disjoint set of output tiles
     for (size_t L = 0; L < launch_np.size(); ++L) {

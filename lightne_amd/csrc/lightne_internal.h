@@ -30,6 +30,7 @@ struct lightne_ctx {
   int depth = 0;       // nesting of library calls in progress
   hipEvent_t ev[16] = {};
   int num_cus = 256;
+  size_t lds_per_block = 64 * 1024;  // hipDeviceProp_t::sharedMemPerBlock
   void* scratch = nullptr;  // device address of dense.hip's static scratch array on this device
   void* blas = nullptr;     // rocblas_handle of the wide-rank stage 2 (wide.hip), created on first use
 };

@@ -879,8 +879,20 @@ static size_t ctx_held_bytes(lightne_ctx* ctx) {
 // Sampler, table, trunc-log filter and CSR.  ctx_owned: the CSR arrays are
 // context buffers reused by the next call (the pipeline's own use); else the
 // handle owns them.
+static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                          lightne_csr** out, lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve,
+                          int restarts);
+
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
                   lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve) {
+  return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, 0);
+}
+
+// `restarts`: how often the layout was planned again after a workspace growth
+// (with -sample 1 the second sampler run may keep a few other keys, below)
+static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
+                          lightne_csr** out, lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve,
+                          int restarts) {
   *out = nullptr;
   lightne_sparsify_stats st{};
   TabView tv;
@@ -1193,14 +1205,30 @@ int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsi
         if (p->csr_parts < 2 && need > avail)
           return ctx_fail(ctx, LIGHTNE_ERR_OOM, "stage 1 needs a %.1f GB workspace, %.1f GB are available",
                           need / 1e9, avail / 1e9);
-        if (!ctx_buf(ctx, "ws", need)) return LIGHTNE_ERR_OOM;
+        // -sample 1: a little room for the re-run below to keep a few more keys
+        const size_t slack = p->sample ? need / 1000 + ((size_t)64 << 20) : 0;
+        if (!ctx_buf(ctx, "ws", need + slack)) return LIGHTNE_ERR_OOM;
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
         LN_CHECK(count_pass(true));  // places the cache behind the table iff keep_lw
         if (keep_lw && !(lw && lw_in_ws))
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the value cache did not fit the grown workspace");
-        if (st.nnz != keep.nnz || st.distinct_keys != keep.distinct_keys)
+        // the key -> count multiset is exact (integer counts), so the keys and
+        // their number are the first run's
+        if (st.distinct_keys != keep.distinct_keys || st.samples != keep.samples)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different table on its second run");
+        if (p->sample) {
+          // -sample 1: the FP64 weights are sums in the order of the atomic
+          // adds, so M_exact and a key within an ulp of the trunc-log
+          // threshold (adj > 1) can differ between runs, and with them the
+          // nonzeros per row range; plan the layout again from a fresh run on
+          // the grown workspace
+          if (restarts >= 2)
+            return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the workspace plan did not settle after growing");
+          return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, restarts + 1);
+        }
+        if (st.nnz != keep.nnz)
+          return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different sparsifier on its second run");
         st.csr_parts = keep.csr_parts;
       }
       char* ws = (char*)ctx->bufs["ws"].p;

@@ -43,11 +43,15 @@ def test_graph_without_edges(gpu):
     assert np.isfinite(emb.cpu().numpy()).all() and S[0] > 0
 
 
-@pytest.mark.parametrize("rank,dim", [(20, 16), (16, 32), (272, 128)])
-def test_bad_shapes_are_errors(gpu, rank, dim):
+@pytest.mark.parametrize("rank,dim,prec", [(16, 32, "fp32"), (0, 0, "fp32"), (20, 16, "fp64"),
+                                           (272, 128, "fp64_lean"), (3000, 64, "fp32")])
+def test_bad_shapes_are_errors(gpu, rank, dim, prec):
+    """dim above rank, rank 0, rank above n, and ranks outside the 16-wide
+    tiling in the FP64 forms (any rank runs in FP32 and BF16)."""
     hg = L.chung_lu(2000, 2.1, 12.0, 300.0, 5)
+    p = {"fp32": L.FP32, "fp64": L.FP64, "fp64_lean": L.FP64_LEAN}[prec]
     with pytest.raises(L.LightNEError) as e:
-        L.NetSMF(hg.to_device(), walk_len=5, rank=rank, dim=dim, sample_ratio=0.2, seed=1, ctx=gpu)
+        L.NetSMF(hg.to_device(), walk_len=5, rank=rank, dim=dim, sample_ratio=0.2, seed=1, precision=p, ctx=gpu)
     assert e.value.code == _lib.ERR_BAD_ARG
 
 
@@ -60,7 +64,7 @@ def test_rank_above_n_and_bad_propagation_width(gpu):
     assert e.value.code == _lib.ERR_BAD_ARG
     import torch
 
-    a = torch.zeros(hg.n, 24, device="cuda")
+    a = torch.zeros(hg.n, 1040, device="cuda")  # widths up to 1024
     with pytest.raises(L.LightNEError) as e:
         L.propagate(a, g, ctx=gpu)
     assert e.value.code == _lib.ERR_BAD_ARG

@@ -217,22 +217,31 @@ def test_csr_by_row_ranges_matches_one_sort(gpu, case, parts, upper):
     b.free()
 
 
-def test_netsmf_row_ranges_in_a_fresh_context(gpu):
+@pytest.mark.parametrize("sample", [False, True])
+def test_netsmf_row_ranges_in_a_fresh_context(gpu, sample):
     """The pipeline's context-owned build by row ranges (the workspace grows and
-    the deterministic sampler runs again) matches the single-sort pipeline,
-    on the first and on a repeated call."""
+    the sampler runs again) matches the single-sort pipeline, on the first and
+    on a repeated call: bit for bit with -sample 0; with -sample 1 the FP64
+    weights are sums in atomic order, so the layout is planned again after the
+    growth and the result agrees to rounding."""
     hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
     g = hg.to_device()
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
-    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3)
+    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, sample=sample)
     e1, S1, st1 = L.NetSMF(g, ctx=gpu, **kw)
     ctx2 = L.Context(0)
     try:
         for _ in range(2):
             e2, S2, st2 = L.NetSMF(g, csr_parts=6, ctx=ctx2, **kw)
             assert st2["sparsify"]["csr_parts"] >= 2 and st1["sparsify"]["csr_parts"] == 1
-            np.testing.assert_array_equal(S2, S1)
-            np.testing.assert_array_equal(e2.cpu().numpy(), e1.cpu().numpy())
+            assert st2["sparsify"]["distinct_keys"] == st1["sparsify"]["distinct_keys"]
+            if not sample:
+                np.testing.assert_array_equal(S2, S1)
+                np.testing.assert_array_equal(e2.cpu().numpy(), e1.cpu().numpy())
+            else:
+                assert abs(st2["sparsify"]["nnz"] - st1["sparsify"]["nnz"]) <= 1e-4 * st1["sparsify"]["nnz"]
+                np.testing.assert_allclose(S2, S1, rtol=1e-4)
+                assert H.principal_angles(e2.cpu().numpy()[:, :16], e1.cpu().numpy()[:, :16]) < 1e-2
     finally:
         ctx2.close()
 
