@@ -402,8 +402,9 @@ def main():
     free_b, total_b = torch.cuda.mem_get_info(0)
     log(f"device memory: {total_b / 1e9:.1f} GB total, {free_b / 1e9:.1f} GB free with the graph resident")
     for w in range(args.warmup):
+        t1 = time.perf_counter()
         run_pipeline(L, g, cfg, ratio, L.FP32, ctx)
-        log(f"warmup {w} done")
+        log(f"warmup {w}: {time.perf_counter() - t1:.3f}s")
     torch.cuda.synchronize()
     times = []
     t_start = time.perf_counter()

@@ -46,10 +46,12 @@ int csr_plan(lightne_ctx* ctx, lightne_csr* A) {
   return LIGHTNE_OK;
 }
 
-// Y(f32) = A X, X in FP32 or BF16
-int csr_mm(lightne_ctx* ctx, lightne_csr* A, const void* X, int xt, int64_t k, float* Y) {
+// Y(f32) = A X, X in FP32 or BF16; `partial` (nparts x k floats, or NULL for
+// a context buffer) holds the partial sums of rows split over several chunks
+int csr_mm(lightne_ctx* ctx, lightne_csr* A, const void* X, int xt, int64_t k, float* Y, float* partial = nullptr) {
   LN_CHECK(csr_plan(ctx, A));
   SpmmArgs a;
+  a.partial = partial;
   a.n = A->n;
   a.k = k;
   a.rp = A->row_ptr;
@@ -487,11 +489,23 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   const size_t pf = (size_t)n * k * sizeof(float);
   float *P1 = nullptr, *P2 = nullptr;  // Y / Q; Omega (FP32), then B
   __hip_bfloat16* PB = nullptr;
+  float* part = nullptr;  // SpMM partial sums
+  LN_CHECK(csr_plan(ctx, A));
   {
     WsCarve wc;
     wc.add((void**)&P1, pf);
     wc.add((void**)&P2, pf);
     if (bf) wc.add((void**)&PB, (size_t)n * k * 2);
+    // the partial sums behind the panels when they fit before a CSR that
+    // lives at the workspace's back (the pipeline's), else a buffer of their
+    // own: at n = 2^26 they are 15 GB that would otherwise sit beside the
+    // workspace
+    const size_t part_b = (size_t)A->plan->nparts * k * sizeof(float);
+    const auto& ws = ctx->bufs["ws"];
+    const char* lo = (const char*)std::min({(const void*)A->row_ptr, (const void*)A->col, (const void*)A->val});
+    const bool csr_in_ws = ws.p && lo >= (const char*)ws.p && lo < (const char*)ws.p + ws.bytes;
+    const size_t limit = csr_in_ws ? (size_t)(lo - (const char*)ws.p) : (ws.p ? ws.bytes : 0);
+    if (part_b > 0 && wc.end() + ((part_b + 255) & ~(size_t)255) <= limit) wc.add((void**)&part, part_b);
     LN_CHECK(wc.commit(ctx));
   }
   hipEvent_t* e = ctx->ev;
@@ -509,7 +523,7 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   if (p->sparse_project)
     LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, false));
   else
-    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1));
+    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1, part));
   LN_HIP(ctx, hipEventRecord(e[3], s));
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
@@ -533,7 +547,8 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
     // B = A Q gathers the FP32 Q (the refinement pass)
     const bool gather_bf = bf_power && (it + 1 < q);
     if (gather_bf) LN_CHECK(f32_to_bf16(ctx, P1, PB, n * k));
-    LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2));
+    LN_CHECK(csr_mm(ctx, A, gather_bf ? (const void*)PB : (const void*)P1, gather_bf ? PANEL_BF16 : PANEL_F32, k, P2,
+                    part));
     st.spmm_calls++;
     LN_HIP(ctx, hipEventRecord(e[6], s));
     LN_HIP(ctx, hipEventSynchronize(e[6]));

@@ -1128,8 +1128,13 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // in_ws: the ranges must fit the workspace as it is
     auto plan = [&](size_t lwb, bool in_ws) -> int {
       size_t part_room;
+      // the pipeline's later stages allocate outside the workspace: per-row
+      // SpMM plan scratch (about 40 bytes per row), chunk lists, the graph
+      // plan's partial sums; a repeated call must find them free again, or
+      // it evicts cached buffers and waits for freed memory to be wiped
+      const size_t later = ctx_owned ? 80 * ((size_t)g->n + 1) + ((size_t)4 << 30) : 0;
       if (ctx_owned)
-        part_room = avail > tab_bytes + lwb + csr_bytes ? avail - tab_bytes - lwb - csr_bytes : 0;
+        part_room = avail > tab_bytes + lwb + csr_bytes + later ? avail - tab_bytes - lwb - csr_bytes - later : 0;
       else
         part_room = std::max(ws_now - tab_bytes, avail > ws_now + csr_bytes ? avail - ws_now - csr_bytes : 0);
       if (in_ws) {
@@ -1152,7 +1157,9 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
                         (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
       rng.clear();
       rcnt.clear();
-      const int max_buckets = std::max(1, (1 << 22) >> shift);
+      // ranges of at most 2^24 rows: (row - r0) << colbits | col then has at
+      // most 24 + 26 = 50 bits, six radix passes (two of 9 bits)
+      const int max_buckets = std::max(1, (1 << 24) >> shift);
       int lo = 0;
       uint64_t acc = 0;
       for (int b = 0; b < nbuck; ++b) {

@@ -217,7 +217,7 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   a.multi_first = P.multi_first;
   a.G = (int)(a.k / 16);
   const int NG = 64 / a.G;
-  if (P.nparts > 0) {
+  if (P.nparts > 0 && !a.partial) {  // the caller may bring them (nparts x k floats)
     a.partial = (float*)ctx_buf(ctx, "spmm_partial", (size_t)P.nparts * a.k * sizeof(float));
     if (!a.partial) return LIGHTNE_ERR_OOM;
   }

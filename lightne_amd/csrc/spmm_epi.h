@@ -60,7 +60,7 @@ struct SpmmArgs {
   int64_t nmulti = 0;
   const int32_t* multi_row = nullptr;
   const int64_t* multi_first = nullptr;
-  float* partial = nullptr;
+  float* partial = nullptr;  // nparts x k partial sums; NULL: a context buffer
 };
 
 // chunk_nnz <= 0: the default chunk length

@@ -1,7 +1,7 @@
 """Size-independent properties at the full R-MAT scale 26 workload (n = 67.1M,
 2m = 1.84B directed edges, T = 5, M = 10 m = 9.2e9 samples, k = 144, d = 128,
 q = 1), where no host oracle can follow.  This is the only size whose
-sparsifier (1.5e10 nonzeros) must be built in row ranges and whose
+sparsifier (1.5e10 nonzeros) must be built in row ranges by itself and whose
 propagation writes over its input: the embedding does not depend on the
 number of row ranges, the first call (which sizes the workspace and runs the
 deterministic sampler a second time after growing it) equals later calls on
@@ -40,7 +40,7 @@ def test_c5_stage1_and_2_properties(gpu, rmat26):
     sp = st1["sparsify"]
     print(f"samples={sp['samples']} distinct={sp['distinct_keys']} nnz={sp['nnz']} ranges={sp['csr_parts']}",
           flush=True)
-    assert sp["csr_parts"] >= 16 and sp["table_layout"] == 2
+    assert sp["csr_parts"] >= 4 and sp["table_layout"] == 2  # one sort would need 529 GB
     assert sp["samples"] == sp["attempts"] and sp["M_exact"] == sp["samples"]
     assert 2 * sp["kept_upper"] - hg.n <= sp["nnz"] <= 2 * sp["kept_upper"]
     assert sp["kept_upper"] <= sp["distinct_keys"] <= sp["samples"]
