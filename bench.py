@@ -380,6 +380,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device 3 times (a kernel of known traffic "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--gather-calibration", action="store_true",
+                    help="after the timed steps, gather 2^25 random 64-byte rows of a 4 GiB table 3 times "
+                         "(index_select: 2 GiB of rows and 256 MiB of indices read, 2 GiB written; random access "
+                         "of known traffic for scaling FETCH_SIZE on gather kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU (the whole pipeline on one MI355X); --gpus {args.gpus} "
@@ -426,6 +430,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.gather_calibration:
+        gen = torch.Generator(device="cuda:0").manual_seed(9)
+        src = torch.empty((1 << 26, 16), dtype=torch.float32, device="cuda:0").fill_(1.0)
+        idx = torch.randint(0, 1 << 26, (1 << 25,), generator=gen, device="cuda:0")
+        for _ in range(3):
+            out = src.index_select(0, idx)
+        torch.cuda.synchronize()
+        del src, idx, out
     ms_per_step = 1000.0 * wall / args.steps
     sp, rs = st["sparsify"], st["rsvd"]
     n, k, d = hg.n, cfg["k"], cfg["d"]
@@ -606,7 +618,7 @@ def main():
         "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
         "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
         "table_capacity": sp["table_capacity"], "table_slot_bytes": slot_b, "csr_row_ranges": sp["csr_parts"],
-        "csr_value_cache": bool(sp["value_cache"]), "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
+        "csr_value_cache": bool(sp["value_cache"]), "sampler_runs": sp["sampler_runs"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "fp64_reference": fp64, "adjoint": adjoint, "save": save, "roofline": roofline,
         "cpu_baseline": cpu,
     }

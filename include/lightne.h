@@ -158,7 +158,8 @@ typedef struct {
   uint32_t csr_parts;      /* row ranges the CSR was built in (1 = one sort) */
   uint32_t value_cache;    /* 1: the CSR values came from the count pass's
                               per-slot cache, not a second read of degrees  */
-  uint32_t reserved;
+  uint32_t sampler_runs;   /* 1, or 2 when the workspace grew after the count
+                              pass and the sampler ran again                */
 } lightne_sparsify_stats;
 
 /* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */

@@ -897,6 +897,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   lightne_sparsify_stats st{};
   TabView tv;
   LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
+  st.sampler_runs = 1;
   hipStream_t s = ctx->stream;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
@@ -1198,6 +1199,25 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       lw_allowed = false;  // a regrowth's count pass must not place it either
       ctx->err.clear();
       LN_CHECK(plan(0, false));
+      // A repeated call finds the workspace an earlier call sized: ranges that
+      // fit it as it is avoid a regrowth (a second sampler run and the wait
+      // for the freed memory's wipe) that a plan made from a slightly
+      // different free-memory reading would force.  Taken unless it needs
+      // half as many ranges again.
+      if (ctx_owned && p->csr_parts < 2 && ws_now > 0) {
+        const auto rng_free = rng;
+        const auto rcnt_free = rcnt;
+        const size_t pe_free = pe, part_free = part_bytes;
+        const bool fits = plan(0, true) == LIGHTNE_OK;
+        ctx->err.clear();
+        if (!(fits && rng.size() <= rng_free.size() + rng_free.size() / 2 + 1 &&
+              std::max(tab_bytes + part_bytes, front_reserve) + csr_bytes <= ws_now)) {
+          rng = rng_free;
+          rcnt = rcnt_free;
+          pe = pe_free;
+          part_bytes = part_free;
+        }
+      }
     }
     const size_t lwb = keep_lw ? lw_bytes : 0;
     st.csr_parts = (uint32_t)rng.size();
@@ -1217,6 +1237,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         if (!ctx_buf(ctx, "ws", need + slack)) return LIGHTNE_ERR_OOM;
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
+        st.sampler_runs = keep.sampler_runs + 1;
         LN_CHECK(count_pass(true));  // places the cache behind the table iff keep_lw
         if (keep_lw && !(lw && lw_in_ws))
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the value cache did not fit the grown workspace");
@@ -1386,6 +1407,7 @@ int lightne_sparsify_counts(lightne_ctx* ctx, const lightne_graph* g, const ligh
   lightne_sparsify_stats st{};
   TabView tv;
   LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
+  st.sampler_runs = 1;
   const uint64_t cap = tv.cap;
   // the compacted table addresses slots with 32-bit indices
   if (cap > (1ull << 32))
