@@ -194,9 +194,12 @@ int lightne_csr_free(lightne_csr* c);
 
 /* ---- stage 2: randomized SVD (mklredsvd.cc:93-403) -------------------------- */
 typedef struct {
-  int64_t rank;             /* -rank: sketch width k (d + oversampling)       */
+  int64_t rank;             /* -rank: sketch width k (d + oversampling), any k
+                               in [1, n] below 32768 in the FP32 and BF16
+                               forms; the FP64 forms take multiples of 16 up
+                               to 256, sparse_project any k up to 256        */
   int32_t power_iters;      /* q >= 1; 1 = the reference schedule             */
-  int32_t precision;        /* LIGHTNE_FP32 / LIGHTNE_BF16 / LIGHTNE_FP64 panels */
+  int32_t precision;        /* LIGHTNE_FP32 / BF16 / BF16_POWER / FP64 / FP64_LEAN */
   int32_t random_project_only; /* -random_project_only (mklredsvd.cc:254-258) */
   int32_t upper;            /* A holds only the upper triangle (symmetric)    */
   uint64_t seed;            /* sketch seed (reference: VSL seed 777)          */
@@ -236,12 +239,14 @@ typedef struct {
 } lightne_prop_stats;
 
 /* y = a - sum_{i<K} c_i T_i a and z = (A + I) y (SpectralPropagation.h:94-221).
- * a: n x dim (device, not modified).  y_out / z_out may be NULL. */
+ * a: n x dim (device, not modified), dim in [1, 1024].  y_out / z_out may be
+ * NULL. */
 int lightne_propagate(lightne_ctx* ctx, const lightne_graph* g, const float* a, int64_t dim,
                       const lightne_prop_params* p, float* y_out, float* z_out, lightne_prop_stats* stats);
 
 /* get_embedding_dense: thin SVD of mat (n x rank), emb = rownorm(U[:, :dim] sqrt(S)).
- * S_host (rank values, descending) may be NULL. */
+ * Any 1 <= dim <= rank <= n with rank < 32768.  S_host (rank values,
+ * descending) may be NULL. */
 int lightne_embed_dense(lightne_ctx* ctx, const float* mat, int64_t n, int64_t rank, int64_t dim,
                         float* emb_out, float* S_host);
 

@@ -5,7 +5,9 @@
 // The Chebyshev recurrence of order K (:94-208) runs as 2 gathers per step with
 // the AXPYs fused into the second gather's epilogue; y = a - conv is written by
 // the last step, z = (A + I) y by one more gather (:210-221), and the dense
-// tall SVD (:16-43) is CholeskyQR2 + a host FP64 SVD of the R factor.
+// tall SVD (:16-43) is the FP64 Gram z^T z, its host FP64 eigendecomposition
+// (sigma^2, V) and one panel product z V S^-1/2 with a row-normalising
+// epilogue (z's column space is its left singular space: no tall QR).
 #include "lightne_internal.h"
 #include "spmm_epi.h"
 

@@ -1120,9 +1120,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // [table | cache | ranges | CSR]; otherwise the range buffers use the
     // workspace's room behind the table or memory of their own.
     const size_t ws_now = ctx->bufs["ws"].bytes;
-    // ranges of whole buckets, at most 2^22 rows each (6 radix passes at
-    // n = 2^26), sized for the room left beside the table, the value cache
-    // (lwb bytes) and the CSR
+    // ranges of whole buckets, at most 2^24 rows each, sized for the room
+    // left beside the table, the value cache (lwb bytes) and the CSR
     std::vector<std::pair<int, int>> rng;  // bucket ranges
     std::vector<uint64_t> rcnt;
     size_t pe = 1, part_bytes = 0;
