@@ -442,10 +442,16 @@ def main():
     sp, rs = st["sparsify"], st["rsvd"]
     n, k, d = hg.n, cfg["k"], cfg["d"]
     nnz = sp["nnz"]
-    # SpMM gather model (FP32 panels): row ptrs + (col, val) + gathered rows + written rows
+    # SpMM gather model: row ptrs + (col, val) + gathered rows + written rows.
+    # The sketch product Y = A Omega gathers the BF16-stored sketch (2 bytes
+    # per element); the later products B = A Q gather FP32 panels.
     spmm_bytes = 8 * (n + 1) + nnz * 8 + nnz * k * 4 + n * k * 4
-    t_spmm_call = rs["t_spmm"] / max(rs["spmm_calls"], 1)
-    spmm_gbs = spmm_bytes / t_spmm_call / 1e9
+    sketch_bytes = 8 * (n + 1) + nnz * 8 + nnz * k * 2 + n * k * 4
+    fp32_calls = max(rs["spmm_calls"] - 1, 0)
+    t_fp32 = rs["t_spmm"] - rs["t_spmm_sketch"]
+    t_spmm_call = t_fp32 / fp32_calls if fp32_calls else float("nan")
+    spmm_gbs = spmm_bytes / t_spmm_call / 1e9 if fp32_calls else None
+    sketch_gbs = sketch_bytes / rs["t_spmm_sketch"] / 1e9 if rs["t_spmm_sketch"] > 0 else None
     sampled_eps = sp["samples"] / sp["t_sample"] if sp["t_sample"] > 0 else None
     # sampler byte model: 20 B / walk step + one slot read and written per
     # sample (8-byte or 16-byte slots) + 12 B / directed edge
@@ -458,10 +464,13 @@ def main():
               "prop_cheb": pst["t_cheb"], "prop_aggregate": pst["t_aggregate"], "prop_svd": pst["t_svd"],
               "prop_total": pst["t_total"]}
     # dominant kernel by measured device time (HIP events on the context
-    # stream): the stage-2 SpMM k_spmm<float,1,false> (+ k_spmm_fixup for rows
-    # split over several chunks) or the sampler k_sample
-    cands = {"spmm_stage2": (rs["t_spmm"], spmm_bytes * rs["spmm_calls"], rs["spmm_calls"]),
-             "sampler": (sp["t_sample"], sampler_bytes, 1)}
+    # stream): the stage-2 FP32 SpMM B = A Q, k_spmm<float,1,false> (+
+    # k_spmm_fixup for rows split over several chunks), the sketch product
+    # k_spmm<bf16,1,false>, or the sampler k_sample
+    cands = {"sampler": (sp["t_sample"], sampler_bytes, 1),
+             "spmm_sketch": (rs["t_spmm_sketch"], sketch_bytes, 1)}
+    if fp32_calls:
+        cands["spmm_stage2"] = (t_fp32, spmm_bytes * fp32_calls, fp32_calls)
     dom = max(cands, key=lambda c: cands[c][0])
     t_dom, bytes_dom, launches = cands[dom]
     achieved = bytes_dom / t_dom / 1e9
@@ -615,10 +624,14 @@ def main():
                                f" n={hg.n} m_undirected={hg.m // 2} T={cfg['T']} M={sp['samples']} k={k} d={d} "
                                f"q={cfg['q']} K={cfg['K']} mu=0.2 theta=0.5 FP32",
                    "parallelism": "1 GPU"},
-        "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1), "spmm_gbs": round(spmm_gbs, 1),
+        "sampled_edges_per_s": sampled_eps, "sampler_gbs": round(sampler_gbs, 1),
+        "spmm_gbs": round(spmm_gbs, 1) if spmm_gbs else None,
+        "spmm_sketch": {"gbs": round(sketch_gbs, 1) if sketch_gbs else None,
+                        "ms": round(1e3 * rs["t_spmm_sketch"], 3), "bytes": int(sketch_bytes),
+                        "frac": round(sketch_gbs / HBM_PEAK_GBS, 4) if sketch_gbs else None},
         "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
         "table_capacity": sp["table_capacity"], "table_slot_bytes": slot_b, "csr_row_ranges": sp["csr_parts"],
-        "csr_value_cache": bool(sp["value_cache"]), "sampler_runs": sp["sampler_runs"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
+        "csr_value_cache": int(sp["value_cache"]), "sampler_runs": sp["sampler_runs"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "fp64_reference": fp64, "adjoint": adjoint, "save": save, "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -135,7 +135,10 @@ typedef struct {
                                each slot's CSR value where memory allows), 1
                                always (the workspace grows for it), -1 never;
                                the CSR is the same either way               */
-  int32_t reserved;
+  int32_t wide_offsets;     /* 0: the walks read 32-bit row pointers when the
+                               graph has fewer than 2^32 directed edges and
+                               the graph's 64-bit offsets otherwise; 1: the
+                               64-bit offsets always (the same samples)     */
 } lightne_sparsify_params;
 
 typedef struct {
@@ -157,7 +160,10 @@ typedef struct {
   uint32_t table_layout;   /* 1 = 16-byte slots, 2 = 8-byte slots          */
   uint32_t csr_parts;      /* row ranges the CSR was built in (1 = one sort) */
   uint32_t value_cache;    /* 1: the CSR values came from the count pass's
-                              per-slot cache, not a second read of degrees  */
+                              per-slot cache, not a second read of degrees;
+                              2: the cache covered the first slots only
+                              (lightne_netsmf keeps it in its output buffer,
+                              which stage 2 writes later); 0: no cache     */
   uint32_t sampler_runs;   /* 1, or 2 when the workspace grew after the count
                               pass and the sampler ran again                */
 } lightne_sparsify_stats;
@@ -211,6 +217,7 @@ typedef struct {
 typedef struct {
   double t_sketch, t_spmm, t_qr, t_gemm, t_small_svd, t_total; /* seconds */
   int32_t spmm_calls, qr_calls;
+  double t_spmm_sketch;     /* the first product Y = A Omega (part of t_spmm) */
 } lightne_rsvd_stats;
 
 /* U_out: n x rank row-major (device).  S_host: rank singular values,

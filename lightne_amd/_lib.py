@@ -54,7 +54,7 @@ class SparsifyParams(C.Structure):
         ("walk_len", C.c_uint64), ("negative", C.c_uint64), ("sample", C.c_int32), ("upper", C.c_int32),
         ("sample_ratio", C.c_float), ("mem_ratio", C.c_float), ("tablesz", C.c_uint64), ("seed", C.c_uint64),
         ("max_table_bytes", C.c_uint64), ("table_layout", C.c_int32), ("csr_parts", C.c_int32),
-        ("value_cache", C.c_int32), ("reserved", C.c_int32),
+        ("value_cache", C.c_int32), ("wide_offsets", C.c_int32),
     ]
 
 
@@ -83,6 +83,7 @@ class RsvdStats(C.Structure):
     _fields_ = [
         ("t_sketch", C.c_double), ("t_spmm", C.c_double), ("t_qr", C.c_double), ("t_gemm", C.c_double),
         ("t_small_svd", C.c_double), ("t_total", C.c_double), ("spmm_calls", C.c_int32), ("qr_calls", C.c_int32),
+        ("t_spmm_sketch", C.c_double),
     ]
 
 

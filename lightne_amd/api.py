@@ -315,15 +315,19 @@ def sample_scalars(n: int, m: int, walk_len: int = 10, sample_ratio: float = 1.0
 
 
 def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                     max_table_bytes, table_layout=0, csr_parts=0, value_cache=0) -> SparsifyParams:
+                     max_table_bytes, table_layout=0, csr_parts=0, value_cache=0, wide_offsets=0) -> SparsifyParams:
     """``table_layout``: 0 auto, 1 16-byte slots, 2 8-byte slots (n <= 2^26).
     ``csr_parts``: 0 auto, k >= 2 builds the CSR in at least k row ranges.
     ``value_cache`` (pipeline): 0 auto, 1 always, -1 never cache the count
-    pass's per-slot CSR values for the emission (same CSR either way)."""
+    pass's per-slot CSR values for the emission (same CSR either way).
+    ``wide_offsets``: 0 auto (32-bit row pointers for the walks when the graph
+    has fewer than 2^32 directed edges), 1 the graph's 64-bit offsets always
+    (same samples)."""
     return SparsifyParams(walk_len=walk_len, negative=negative, sample=int(bool(sample)), upper=int(bool(upper)),
                           sample_ratio=sample_ratio, mem_ratio=mem_ratio, tablesz=table_size,
                           seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes,
-                          table_layout=table_layout, csr_parts=csr_parts, value_cache=value_cache)
+                          table_layout=table_layout, csr_parts=csr_parts, value_cache=value_cache,
+                          wide_offsets=int(wide_offsets))
 
 
 @dataclasses.dataclass
@@ -336,11 +340,12 @@ class SparseTable:
 
 def sparsify_counts(graph: DeviceGraph, seed: int, walk_len: int = 10, sample: bool = False,
                     negative: int = 1, sample_ratio: float = 1.0, mem_ratio: float = 1.0, table_size: int = 0,
-                    max_table_bytes: int = 0, table_layout: int = 0, ctx: Optional[Context] = None) -> SparseTable:
+                    max_table_bytes: int = 0, table_layout: int = 0, wide_offsets: int = 0,
+                    ctx: Optional[Context] = None) -> SparseTable:
     """Key -> (count, weight) multiset of the sampler (PathEmbed.h:128-188)."""
     ctx = ctx or context()
     p = _sparsify_params(seed, walk_len, False, sample, table_size, negative, sample_ratio, mem_ratio,
-                         max_table_bytes, table_layout)
+                         max_table_bytes, table_layout, wide_offsets=wide_offsets)
     st = SparsifyStats()
     tab = C.c_void_p()
     g = graph.desc()
@@ -430,13 +435,14 @@ class CSR:
 def generate_trunc_log_matrix(graph: DeviceGraph, seed: int, walks_per_edge: int = 100, walk_len: int = 10,
                               upper: bool = False, sample: bool = False, table_size: int = 0, negative: int = 1,
                               sample_ratio: float = 1.0, mem_ratio: float = 1.0, max_table_bytes: int = 0,
-                              table_layout: int = 0, csr_parts: int = 0, ctx: Optional[Context] = None):
+                              table_layout: int = 0, csr_parts: int = 0, wide_offsets: int = 0,
+                              ctx: Optional[Context] = None):
     """Stage 1 (PathEmbed.h:81-316): returns (CSR, stats).  ``walks_per_edge`` is
     accepted and unused, as in the reference (PathEmbed.h:107-108)."""
     del walks_per_edge
     ctx = ctx or context()
     p = _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample_ratio, mem_ratio,
-                         max_table_bytes, table_layout, csr_parts)
+                         max_table_bytes, table_layout, csr_parts, wide_offsets=wide_offsets)
     st = SparsifyStats()
     h = C.c_void_p()
     g = graph.desc()

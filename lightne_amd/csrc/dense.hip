@@ -72,6 +72,26 @@ __global__ void k_gaussian(uint64_t seed, uint32_t stream, int64_t count, float*
   }
 }
 
+// the same values stored as BF16: every value is BF16-representable, so the
+// 16-bit panel holds the sketch exactly at half the bytes
+__global__ void k_gaussian_bf16(uint64_t seed, uint32_t stream, int64_t count, __hip_bfloat16* __restrict__ out) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * p < count;
+       p += (int64_t)gridDim.x * blockDim.x) {
+  uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;
+  const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * log(u1));
+  const double t = 6.283185307179586476925286766559 * u2;
+  const unsigned short b0 = (unsigned short)(__float_as_uint(round_bf16((float)(r * cos(t)))) >> 16);
+  out[2 * p] = *reinterpret_cast<const __hip_bfloat16*>(&b0);
+  if (2 * p + 1 < count) {
+    const unsigned short b1 = (unsigned short)(__float_as_uint(round_bf16((float)(r * sin(t)))) >> 16);
+    out[2 * p + 1] = *reinterpret_cast<const __hip_bfloat16*>(&b1);
+  }
+  }
+}
+
 // the same n x k sketch written into a panel of row stride ld: element e of
 // the stream goes to row e / k, column e % k
 __global__ void k_gaussian_ld(uint64_t seed, uint32_t stream, int64_t n, int64_t k, int64_t ld,
@@ -432,6 +452,15 @@ int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t coun
   if (count <= 0) return LIGHTNE_OK;
   const int64_t pairs = (count + 1) / 2;
   hipLaunchKernelGGL(k_gaussian, dim3(grid_blocks(pairs, 256)), dim3(256), 0, ctx->stream, seed, stream,
+                     count, out);
+  LN_LAUNCH(ctx);
+  return LIGHTNE_OK;
+}
+
+int gaussian_fill_bf16(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, __hip_bfloat16* out) {
+  if (count <= 0) return LIGHTNE_OK;
+  const int64_t pairs = (count + 1) / 2;
+  hipLaunchKernelGGL(k_gaussian_bf16, dim3(grid_blocks(pairs, 256)), dim3(256), 0, ctx->stream, seed, stream,
                      count, out);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;

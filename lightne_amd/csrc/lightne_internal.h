@@ -128,9 +128,11 @@ int dev_alloc(lightne_ctx* ctx, DevMem& m, size_t bytes, const char* what);
 // stage 1 (sparsify.hip): sampler, table, trunc-log filter, CSR.  ctx_owned:
 // the CSR arrays are context memory (reused by the next call) and the handle
 // does not free them; they then stay clear of the first front_reserve bytes
-// of the workspace, which the caller's next stage may carve.
+// of the workspace, which the caller's next stage may carve.  xs (xs_bytes,
+// may be NULL): device memory free for stage 1 to use as scratch, e.g. an
+// output buffer that is written only after stage 1.
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
-                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve);
+                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve, float* xs, size_t xs_bytes);
 // Y = A Omega for the very sparse random projection Omega (sparseproj.hip);
 // Y is n x k FP32, or FP64 when f64
 int sparse_project_sketch(lightne_ctx* ctx, const int64_t* rp, const int32_t* col, const float* val, int64_t n,
@@ -178,6 +180,9 @@ enum PanelType { PANEL_F32 = 0, PANEL_BF16 = 1 };
 // dense panels (dense.hip)
 // ---------------------------------------------------------------------------
 int gaussian_fill(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, float* out);
+// the same values as gaussian_fill, stored as BF16 (exactly: they are
+// BF16-representable)
+int gaussian_fill_bf16(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t count, __hip_bfloat16* out);
 // the same values as gaussian_fill(count = n k) in an n x k panel of row stride ld
 int gaussian_fill_ld(lightne_ctx* ctx, uint64_t seed, uint32_t stream, int64_t n, int64_t k, int64_t ld,
                      float* out);

@@ -15,14 +15,15 @@
 // The FP32 and BF16 forms replace the last line by the thin SVD of B from its
 // FP64 Gram, B^T B = V S^2 V^T, U = B V S^-1 (range(Z) = range(B) for any
 // invertible P, so U and S are the same); the FP64 form keeps it as written.
-// In the BF16 form the sketch Omega, whose values are BF16-representable, is
-// gathered as BF16 with FP32 accumulation: the product is the FP32 form's,
-// bit for bit, at half the gathered bytes.  The power iterations gather the
-// FP32 Q: a BF16 rounding of Q (2^-9) is amplified by 1/(sigma_d/sigma_{d+1}
-// - 1) in the top-d subspace, ~5000 on an R-MAT scale-24 sparsifier.  The
-// BF16_POWER form also gathers the power-iteration panels as BF16 (all but the
-// last product B = A Q, the FP32 refinement pass).  Grams are FP64 in every
-// form.
+// The sketch Omega is drawn BF16-representable and stored as BF16 in every
+// non-FP64 form: Y = A Omega gathered from it with FP32 accumulation is the
+// product with the FP32 values, bit for bit, at half the gathered bytes.  The
+// power iterations gather the FP32 Q in the FP32 and BF16 forms (which are
+// therefore the same in stage 2): a BF16 rounding of Q (2^-9) is amplified by
+// 1/(sigma_d/sigma_{d+1} - 1) in the top-d subspace, ~5000 on an R-MAT
+// scale-24 sparsifier.  The BF16_POWER form also gathers the power-iteration
+// panels as BF16 (all but the last product B = A Q, the FP32 refinement pass).
+// Grams are FP64 in every form.
 #include "lightne_internal.h"
 #include "spmm_epi.h"
 
@@ -266,6 +267,7 @@ static int rsvd64(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
   t_spmm += lap(e[2], e[3]);
+  st.t_spmm_sketch = lap(e[2], e[3]);
   st.spmm_calls = 1;
   if (p->random_project_only) {
     LN_CHECK(f64_to_f32(ctx, P1, U_out, n * k));
@@ -386,6 +388,7 @@ static int rsvd64_lean(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_para
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
   t_spmm += lap(e[2], e[3]);
+  st.t_spmm_sketch = lap(e[2], e[3]);
   st.spmm_calls = 1;
   float* stage = reinterpret_cast<float*>(P0);  // FP32 result staging (P0's bytes hold 2 n k floats)
   if (p->random_project_only) {
@@ -484,10 +487,13 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   lightne_rsvd_stats& st = *stp;
   st = lightne_rsvd_stats{};
   hipStream_t s = ctx->stream;
-  const bool bf = p->precision == LIGHTNE_BF16 || p->precision == LIGHTNE_BF16_POWER;
   const bool bf_power = p->precision == LIGHTNE_BF16_POWER;
   const size_t pf = (size_t)n * k * sizeof(float);
-  float *P1 = nullptr, *P2 = nullptr;  // Y / Q; Omega (FP32), then B
+  // P1: Y, then Q.  P2: the sketch Omega stored as BF16 in its first half
+  // (every value is BF16-representable, so the 16-bit panel is the FP32
+  // sketch exactly and Y = A Omega is the FP32 product bit for bit at half the
+  // gathered bytes), then B.  PB: the BF16_POWER form's rounded Q.
+  float *P1 = nullptr, *P2 = nullptr;
   __hip_bfloat16* PB = nullptr;
   float* part = nullptr;  // SpMM partial sums
   LN_CHECK(csr_plan(ctx, A));
@@ -495,7 +501,7 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
     WsCarve wc;
     wc.add((void**)&P1, pf);
     wc.add((void**)&P2, pf);
-    if (bf) wc.add((void**)&PB, (size_t)n * k * 2);
+    if (bf_power) wc.add((void**)&PB, (size_t)n * k * 2);
     // the partial sums behind the panels when they fit before a CSR that
     // lives at the workspace's back (the pipeline's), else a buffer of their
     // own: at n = 2^26 they are 15 GB that would otherwise sit beside the
@@ -513,21 +519,20 @@ static int rsvd_core(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params
   auto lap = [&](hipEvent_t a, hipEvent_t b) { return ev_seconds(a, b); };
 
   LN_HIP(ctx, hipEventRecord(e[0], s));
-  if (!p->sparse_project) {
-    LN_CHECK(gaussian_fill(ctx, p->seed, 0u, n * k, P2));
-    if (bf) LN_CHECK(f32_to_bf16(ctx, P2, PB, n * k));
-  }
+  auto* omega = reinterpret_cast<__hip_bfloat16*>(P2);
+  if (!p->sparse_project) LN_CHECK(gaussian_fill_bf16(ctx, p->seed, 0u, n * k, omega));
   LN_HIP(ctx, hipEventRecord(e[1], s));
   LN_CHECK(csr_plan(ctx, A));
   LN_HIP(ctx, hipEventRecord(e[2], s));
   if (p->sparse_project)
     LN_CHECK(sparse_project_sketch(ctx, A->row_ptr, A->col, A->val, n, k, p->seed, p->sparse_project_s, P1, false));
   else
-    LN_CHECK(csr_mm(ctx, A, bf ? (const void*)PB : (const void*)P2, bf ? PANEL_BF16 : PANEL_F32, k, P1, part));
+    LN_CHECK(csr_mm(ctx, A, (const void*)omega, PANEL_BF16, k, P1, part));
   LN_HIP(ctx, hipEventRecord(e[3], s));
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
   t_spmm += lap(e[2], e[3]);
+  st.t_spmm_sketch = lap(e[2], e[3]);
   st.spmm_calls = 1;
   if (p->random_project_only) {
     *B_out = P1;
@@ -810,14 +815,35 @@ int lightne_netsmf(lightne_ctx* ctx, const lightne_graph* g, const lightne_spars
   rp = &rpf;
   // stage 2 carves its panels from the workspace front (rsvd_core)
   const size_t pnl = ((size_t)g->n * rp->rank * sizeof(float) + 255) & ~(size_t)255;
-  const bool bf = rp->precision == LIGHTNE_BF16 || rp->precision == LIGHTNE_BF16_POWER;
+  const bool bf_power = rp->precision == LIGHTNE_BF16_POWER;
   const size_t blk64 = ((size_t)std::min<int64_t>(g->n, (int64_t)1 << 20) * rp->rank * sizeof(double) + 255) &
                        ~(size_t)255;
   const size_t front = rsvd_is_wide(rp->rank)              ? rsvd_wide_front(g->n, rp->rank, rp->precision)
                        : rp->precision == LIGHTNE_FP64        ? 4 * 2 * pnl
                        : rp->precision == LIGHTNE_FP64_LEAN ? 2 * 2 * pnl + blk64
-                                                            : 2 * pnl + (bf ? pnl / 2 + 256 : 0);
-  LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front));
+                                                            : 2 * pnl + (bf_power ? pnl / 2 + 256 : 0);
+  // emb_out is written only by stage 2: stage 1 may use it as scratch when it
+  // is device memory of this context's device, apart from the graph
+  float* xs = nullptr;
+  size_t xs_bytes = 0;
+  {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, emb_out) == hipSuccess && at.type == hipMemoryTypeDevice &&
+        at.device == ctx->device) {
+      const size_t b = (size_t)g->n * (size_t)dim * sizeof(float);
+      auto apart = [&](const void* q, size_t qb) {
+        const char *a0 = (const char*)emb_out, *q0 = (const char*)q;
+        return !q || qb == 0 || a0 + b <= q0 || q0 + qb <= a0;
+      };
+      if (apart(g->offsets, (size_t)(g->n + 1) * 8) && apart(g->edges, (size_t)g->m * 4)) {
+        xs = emb_out;
+        xs_bytes = b;
+      }
+    } else {
+      (void)hipGetLastError();  // a host pointer: the query's error is not this call's
+    }
+  }
+  LN_CHECK(sparsify_impl(ctx, g, sp, &A, &st.sparsify, true, front, xs, xs_bytes));
   if (upper) st.sparsify.nnz = st.sparsify.kept_upper;
   auto t1 = std::chrono::steady_clock::now();
   std::vector<float> S;

@@ -169,18 +169,18 @@ __global__ void k_off32(const int64_t* __restrict__ off, int64_t count, uint32_t
 }
 
 __global__ void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m, uint32_t* __restrict__ src) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
-  // largest u with off[u] <= j
-  int64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int64_t mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= j)
-      lo = mid;
-    else
-      hi = mid - 1;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    // largest u with off[u] <= j
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+      int64_t mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= j)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    src[j] = (uint32_t)lo;
   }
-  src[j] = (uint32_t)lo;
 }
 
 // block-wide sum of three counters, one atomic per counter per block
@@ -207,11 +207,15 @@ __device__ __forceinline__ void block_add3(unsigned long long* dst, uint64_t a, 
 }
 
 struct SampleArgs {
-  const uint32_t* off;  // row pointers as 32 bits (2m < 2^32): half the bytes of the walk's hot array
+  // row pointers: 32 bits when 2m < 2^32 (half the bytes of the walk's hot
+  // array), else the graph's own 64-bit offsets
+  const uint32_t* off32;
+  const int64_t* off64;
   const uint32_t* adj;
   uint64_t ith;  // 1: compressed-graph neighbour lookup
   const uint32_t* src;
   int64_t m;
+  int64_t j0;  // first directed edge of this launch
   uint64_t seed;
   uint64_t walk_len;
   uint64_t exp_rv;
@@ -231,10 +235,11 @@ struct SampleArgs {
 // neighbour i - 1 unless i is a multiple of 64: the result of
 // get_ith_neighbor on such graphs (ligra/encodings/byte_pd_amortized.h:653-688
 // decodes i - start - 1 differences after the block's first edge).
-__device__ __forceinline__ uint32_t walk(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
-                                         uint32_t x, uint64_t len, uint64_t& s, bool& bad, uint64_t ith) {
+template <typename OffT>
+__device__ __forceinline__ uint32_t walk(const OffT* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t x,
+                                         uint64_t len, uint64_t& s, bool& bad, uint64_t ith) {
   for (uint64_t t = 0; t < len; ++t) {
-    const uint32_t o = off[x];
+    const OffT o = off[x];
     const uint64_t d = (uint64_t)(off[x + 1] - o);
     if (d == 0) {
       bad = true;
@@ -296,9 +301,15 @@ __device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint3
   return 0;
 }
 
-template <int MODE, bool PACKED>
+template <int MODE, bool PACKED, bool OFF64>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = a.j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const auto* off = [&] {
+    if constexpr (OFF64)
+      return a.off64;
+    else
+      return a.off32;
+  }();
   uint64_t n_samples = 0, n_steps = 0, n_attempts = 0;
   bool overflow = false, bad = false, side_full = false;
   if (j < a.m) {
@@ -315,8 +326,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     n_attempts = e;
     double p = 1.0, ipw = 1.0;
     if (MODE == 1) {
-      const double du = (double)(a.off[u + 1] - a.off[u]);
-      const double dv = (double)(a.off[v + 1] - a.off[v]);
+      const double du = (double)(off[u + 1] - off[u]);
+      const double dv = (double)(off[v + 1] - off[v]);
       p = (1.0 / du + 1.0 / dv) * a.logn;
       p = p < 1.0 ? p : 1.0;
       ipw = 1.0 / p;
@@ -333,8 +344,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       rr = hash64_2(s);
       const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
       s = hash64_2(rr);
-      const uint32_t x = walk(a.off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith);
-      const uint32_t z = walk(a.off, a.adj, v, r - kk, s, bad, a.ith);
+      const uint32_t x = walk(off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith);
+      const uint32_t z = walk(off, a.adj, v, r - kk, s, bad, a.ith);
       n_steps += r - 1;
       const int rc = table_add<MODE, PACKED>(a, min(x, z), max(x, z), ipw);
       if (rc == 0) overflow = true;
@@ -427,11 +438,12 @@ struct TruncArgs {
   int colbits;
   uint64_t* skey;
   float* sval;
-  // per-slot value cache: the count pass writes every slot's CSR value
-  // ((float)log(adj) when kept, else 0) to lw_out; an emission pass given
-  // lw_in reads it instead of the two degrees
+  // per-slot value cache: the count pass writes the CSR value of every slot
+  // below lw_n ((float)log(adj) when kept, else 0) to lw_out; an emission
+  // pass given lw_in reads it for those slots instead of the two degrees
   float* lw_out;
   const float* lw_in;
+  uint64_t lw_n;
 };
 
 // PathEmbed.h:203-210: adjusted weight of a key, evaluated in the reference's
@@ -483,7 +495,7 @@ __device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const R
   const bool iu = u >= r0 && u < r1;
   const bool iv = !a.upper && u != v && v >= r0 && v < r1;
   if (!iu && !iv) return 0;
-  if (a.lw_in) {
+  if (a.lw_in && i < a.lw_n) {
     const float l = a.lw_in[i];
     if (!(l > 0.f)) return 0;
     lwv = l;
@@ -537,12 +549,13 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
     // the value cache: one 16-byte store per thread (a wave writes 1 KB
     // contiguously; ib is a multiple of TR_ITEMS), scalar stores at the tail
     if (a.lw_out) {
-      if (ib + TR_ITEMS <= b1) {
+      const uint64_t lim = min(b1, a.lw_n);
+      if (ib + TR_ITEMS <= lim) {
         *reinterpret_cast<float4*>(a.lw_out + ib) = make_float4(lwo[0], lwo[1], lwo[2], lwo[3]);
       } else {
 #pragma unroll
         for (int j = 0; j < TR_ITEMS; ++j)
-          if (ib + j < b1) a.lw_out[ib + j] = lwo[j];
+          if (ib + j < lim) a.lw_out[ib + j] = lwo[j];
       }
     }
   }
@@ -675,9 +688,6 @@ int check_graph(lightne_ctx* ctx, const lightne_graph* g) {
   if (g->n >= (int64_t)1 << 32) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 vertices");
   if (g->ith_rule != LIGHTNE_ITH_PLAIN && g->ith_rule != LIGHTNE_ITH_BYTEPD)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph ith_rule must be LIGHTNE_ITH_PLAIN or LIGHTNE_ITH_BYTEPD");
-  // the sampler runs one lane per directed edge in one launch
-  if (g->m >= ((int64_t)1 << 32) - 256)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "graph has >= 2^32 directed edges");
   return LIGHTNE_OK;
 }
 
@@ -690,6 +700,8 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "walk_len and negative must be >= 1");
   if (p->table_layout < 0 || p->table_layout > 2 || p->csr_parts < 0)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "table_layout must be 0, 1 or 2 and csr_parts >= 0");
+  if (p->wide_offsets != 0 && p->wide_offsets != 1)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "wide_offsets must be 0 or 1");
   const bool fits_packed = (uint64_t)g->n <= (1ull << PK_VB);
   if (p->table_layout == 2 && !fits_packed)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "the packed table layout needs n <= 2^26");
@@ -734,8 +746,10 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   LN_CHECK(wc.commit(ctx));
   auto* stats = (unsigned long long*)ctx_buf(ctx, "sample_stats", 8 * sizeof(unsigned long long));
   auto* ovf = packed ? (unsigned long long*)ctx_buf(ctx, "sample_ovf", OVF_CAP * 16) : nullptr;
-  auto* off32 = (uint32_t*)ctx_buf(ctx, "sample_off32", (size_t)(g->n + 1) * 4);
-  if (!stats || (packed && !ovf) || !off32) return LIGHTNE_ERR_OOM;
+  // 32-bit row pointers for the walks while every offset fits (2m < 2^32)
+  const bool off64 = p->wide_offsets == 1 || (uint64_t)g->m >= ((uint64_t)1 << 32);
+  auto* off32 = off64 ? nullptr : (uint32_t*)ctx_buf(ctx, "sample_off32", (size_t)(g->n + 1) * 4);
+  if (!stats || (packed && !ovf) || (!off64 && !off32)) return LIGHTNE_ERR_OOM;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
@@ -751,13 +765,16 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   if (wtab) LN_HIP(ctx, hipMemsetAsync(wtab, 0, cap * sizeof(double), s));
   LN_HIP(ctx, hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
   if (g->m > 0) {
-    const unsigned nb = (unsigned)((g->m + 255) / 256);
-    hipLaunchKernelGGL(k_edge_src, dim3(nb), dim3(256), 0, s, g->offsets, g->n, g->m, src);
+    hipLaunchKernelGGL(k_edge_src, dim3(grid_blocks(g->m, 256, (int64_t)1 << 22)), dim3(256), 0, s, g->offsets,
+                       g->n, g->m, src);
     LN_LAUNCH(ctx);
-    hipLaunchKernelGGL(k_off32, dim3(grid_blocks(g->n + 1, 256)), dim3(256), 0, s, g->offsets, g->n + 1, off32);
-    LN_LAUNCH(ctx);
+    if (!off64) {
+      hipLaunchKernelGGL(k_off32, dim3(grid_blocks(g->n + 1, 256)), dim3(256), 0, s, g->offsets, g->n + 1, off32);
+      LN_LAUNCH(ctx);
+    }
     SampleArgs a;
-    a.off = off32;
+    a.off32 = off32;
+    a.off64 = g->offsets;
     a.adj = g->edges;
     a.ith = g->ith_rule == LIGHTNE_ITH_BYTEPD ? 1u : 0u;
     a.src = src;
@@ -774,18 +791,26 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     a.cap = cap;
     a.stats = stats;
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
-    if (p->sample) {
-      if (packed)
-        hipLaunchKernelGGL((k_sample<1, true>), dim3(nb), dim3(256), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_sample<1, false>), dim3(nb), dim3(256), 0, s, a);
-    } else {
-      if (packed)
-        hipLaunchKernelGGL((k_sample<0, true>), dim3(nb), dim3(256), 0, s, a);
-      else
-        hipLaunchKernelGGL((k_sample<0, false>), dim3(nb), dim3(256), 0, s, a);
+    // one lane per directed edge; a launch holds at most 2^31 lanes (one grid
+    // dimension holds fewer than 2^32 work-items)
+    constexpr int64_t LANES = (int64_t)1 << 31;
+    for (int64_t j0 = 0; j0 < g->m; j0 += LANES) {
+      a.j0 = j0;
+      const unsigned nb = (unsigned)((std::min(LANES, g->m - j0) + 255) / 256);
+#define LN_SAMPLE(M, P, O) hipLaunchKernelGGL((k_sample<M, P, O>), dim3(nb), dim3(256), 0, s, a)
+      switch ((p->sample ? 4 : 0) + (packed ? 2 : 0) + (off64 ? 1 : 0)) {
+        case 0: LN_SAMPLE(0, false, false); break;
+        case 1: LN_SAMPLE(0, false, true); break;
+        case 2: LN_SAMPLE(0, true, false); break;
+        case 3: LN_SAMPLE(0, true, true); break;
+        case 4: LN_SAMPLE(1, false, false); break;
+        case 5: LN_SAMPLE(1, false, true); break;
+        case 6: LN_SAMPLE(1, true, false); break;
+        default: LN_SAMPLE(1, true, true); break;
+      }
+#undef LN_SAMPLE
+      LN_LAUNCH(ctx);
     }
-    LN_LAUNCH(ctx);
   } else {
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
   }
@@ -881,18 +906,20 @@ static size_t ctx_held_bytes(lightne_ctx* ctx) {
 // handle owns them.
 static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
                           lightne_csr** out, lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve,
-                          int restarts);
+                          float* xs, size_t xs_bytes, int restarts);
 
 int sparsify_impl(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p, lightne_csr** out,
-                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve) {
-  return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, 0);
+                  lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve, float* xs, size_t xs_bytes) {
+  return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, xs, xs_bytes, 0);
 }
 
 // `restarts`: how often the layout was planned again after a workspace growth
-// (with -sample 1 the second sampler run may keep a few other keys, below)
+// (with -sample 1 the second sampler run may keep a few other keys, below).
+// xs: device memory of xs_bytes the caller does not need during this call
+// (the pipeline's output buffer), or NULL.
 static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
                           lightne_csr** out, lightne_sparsify_stats* stats, bool ctx_owned, size_t front_reserve,
-                          int restarts) {
+                          float* xs, size_t xs_bytes, int restarts) {
   *out = nullptr;
   lightne_sparsify_stats st{};
   TabView tv;
@@ -923,12 +950,33 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   // a buffer of its own for a single-sort build that surely fits beside it;
   // the layout chosen after the count keeps it only where it needs no extra
   // memory (value_cache = 1: always, growing the workspace; -1: never).
+  // Where the workspace cannot hold it, the caller's scratch xs holds the
+  // values of the first slots it has room for (the rest read the degrees).
   const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
   const size_t lw_bytes = al(tv.cap * sizeof(float));
   const bool lw_force = p->value_cache > 0;
   bool lw_allowed = ctx_owned && p->value_cache >= 0;
   float* lw = nullptr;
   bool lw_in_ws = false;  // else a buffer of its own (single-sort builds only)
+  bool lw_ext = false;    // in the caller's scratch, slots [0, lw_n)
+  uint64_t lw_n = 0;
+  const bool ext_allowed = xs && p->value_cache >= 0;
+  const uint64_t ext_n = std::min<uint64_t>(tv.cap, xs_bytes / sizeof(float) / TR_ITEMS * TR_ITEMS);
+  // an in-workspace cache the layout cannot keep: the values of its first
+  // ext_n slots move to the caller's scratch before anything overwrites them
+  auto demote = [&]() -> int {
+    if (lw_ext) return LIGHTNE_OK;
+    if (lw && lw_in_ws && ext_allowed && ext_n > 0) {
+      LN_HIP(ctx, hipMemcpyAsync(xs, lw, ext_n * sizeof(float), hipMemcpyDeviceToDevice, s));
+      lw = xs;
+      lw_ext = true;
+      lw_n = ext_n;
+    } else {
+      lw = nullptr;
+    }
+    lw_in_ws = false;
+    return LIGHTNE_OK;
+  };
   // totals (PathEmbed.h:175-200), then per-block counts of the kept entries
   // over contiguous slot ranges and their scan; repeated after a workspace
   // regrowth re-ran the (deterministic) sampler
@@ -977,7 +1025,15 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         lw_in_ws = true;
       }
     }
+    lw_ext = false;
+    lw_n = tv.cap;
+    if (!lw && ext_allowed && ext_n > 0) {
+      lw = xs;
+      lw_ext = true;
+      lw_n = ext_n;
+    }
     ta.lw_out = lw;
+    ta.lw_n = lw_n;
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     if (hist) {
       bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
@@ -1033,7 +1089,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // fit the workspace behind it
     if (lw && lw_in_ws && !lw_force &&
         std::max(tab_bytes + lw_bytes + al(nz * 8), front_reserve) + csr_bytes > ws_bytes)
-      lw = nullptr;
+      LN_CHECK(demote());
     const size_t lwb = lw && lw_in_ws ? lw_bytes : 0;
     const size_t front = std::max(tab_bytes + lwb + al(nz * 8), front_reserve);
     const bool in_ws = ctx_owned && front + csr_bytes <= ws_bytes;
@@ -1080,7 +1136,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.sval = v_emit;
     ta.lw_out = nullptr;
     ta.lw_in = lw;
-    st.value_cache = lw != nullptr;
+    ta.lw_n = lw_n;
+    st.value_cache = lw ? (lw_ext ? 2u : 1u) : 0u;
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
                        (const uint64_t*)blk_off);
@@ -1182,8 +1239,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // keep the value cache only inside the workspace and where the plan with
     // it fits the workspace as it is (no regrowth, which would re-run the
     // sampler)
-    if (lw && !lw_in_ws) lw = nullptr;
-    bool keep_lw = ctx_owned && ((lw != nullptr) || (lw_force && lw_allowed));
+    if (lw && !lw_in_ws && !lw_ext) lw = nullptr;
+    bool keep_lw = ctx_owned && ((lw && lw_in_ws) || (lw_force && lw_allowed));
     if (keep_lw) {
       const int rc = plan(lw_bytes, !lw_force);
       if (rc != LIGHTNE_OK) {
@@ -1194,7 +1251,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       }
     }
     if (!keep_lw) {
-      lw = nullptr;
+      LN_CHECK(demote());
       lw_allowed = false;  // a regrowth's count pass must not place it either
       ctx->err.clear();
       LN_CHECK(plan(0, false));
@@ -1252,7 +1309,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
           // the grown workspace
           if (restarts >= 2)
             return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the workspace plan did not settle after growing");
-          return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, restarts + 1);
+          return sparsify_sized(ctx, g, p, out, stats, ctx_owned, front_reserve, xs, xs_bytes, restarts + 1);
         }
         if (st.nnz != keep.nnz)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "the sampler gave a different sparsifier on its second run");
@@ -1285,7 +1342,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.sval = pval;
     ta.lw_out = nullptr;
     ta.lw_in = lw;
-    st.value_cache = lw != nullptr;
+    ta.lw_n = lw_n;
+    st.value_cache = lw ? (lw_ext ? 2u : 1u) : 0u;
     int64_t base = 0;
     for (size_t r = 0; r < rng.size(); ++r) {
       const uint64_t r0 = (uint64_t)rng[r].first << shift;
@@ -1475,7 +1533,7 @@ int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_spa
                      lightne_sparsify_stats* stats) {
   if (!ctx || !out) return global_fail(LIGHTNE_ERR_BAD_ARG, "lightne_sparsify: NULL argument");
   CallScope scope(ctx);
-  return sparsify_impl(ctx, g, p, out, stats, false, 0);
+  return sparsify_impl(ctx, g, p, out, stats, false, 0, nullptr, 0);
 }
 
 }  // extern "C"

@@ -27,6 +27,9 @@ namespace {
 
 constexpr int SHORT_NNZ = 48;
 constexpr int CHUNK_NNZ = 512;
+// waves per launch: one grid dimension holds fewer than 2^32 work-items, so
+// larger products run as several launches over consecutive wave ranges
+constexpr int64_t MAX_WAVES = ((int64_t)1 << 26) - 64;
 
 // Lane column layout.  A group of G = k/16 lanes covers one k-wide row; lane
 // `sub` owns 16 columns as chunks of CW consecutive columns (CW = 4 floats or
@@ -136,11 +139,11 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
 // the result is deterministic), which applies the epilogue or writes the
 // chunk's partial sum.
 template <typename XT, int EPI, bool IMPL>
-__global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int64_t w_short) {
+__global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int64_t w_short, int64_t wave0) {
   const int lane = threadIdx.x & 63;
   const int G = a.G, NG = 64 / G;
   const int g = lane / G, sub = lane - g * G;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t wave = wave0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   float acc[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
@@ -185,9 +188,9 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int64_t w_short) {
 
 // rows with several chunks: add partials in chunk order, then the epilogue
 template <typename XT, int EPI, bool IMPL>
-__global__ __launch_bounds__(256) void k_spmm_fixup(SpmmArgs a) {
+__global__ __launch_bounds__(256) void k_spmm_fixup(SpmmArgs a, int64_t wave0) {
   const int lane = threadIdx.x & 63;
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t w = wave0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (w >= a.nmulti) return;
   const int ncol4 = (int)(a.k >> 2);
   const int64_t row = a.multi_row[w];
@@ -223,14 +226,14 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   }
   const int64_t waves_short = (a.n + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
-  if (waves >= ((int64_t)1 << 26))  // one grid dimension holds < 2^32 work-items
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: %lld waves exceed one launch", (long long)waves);
-  hipLaunchKernelGGL((k_spmm<XT, EPI, IMPL>), dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a,
-                     waves_short);
-  LN_LAUNCH(ctx);
-  if (P.nmulti > 0) {
-    hipLaunchKernelGGL((k_spmm_fixup<XT, EPI, IMPL>), dim3((unsigned)((P.nmulti + 3) / 4)), dim3(256), 0,
-                       ctx->stream, a);
+  for (int64_t w0 = 0; w0 < waves; w0 += MAX_WAVES) {
+    hipLaunchKernelGGL((k_spmm<XT, EPI, IMPL>), dim3((unsigned)((std::min(MAX_WAVES, waves - w0) + 3) / 4)),
+                       dim3(256), 0, ctx->stream, a, waves_short, w0);
+    LN_LAUNCH(ctx);
+  }
+  for (int64_t w0 = 0; w0 < P.nmulti; w0 += MAX_WAVES) {
+    hipLaunchKernelGGL((k_spmm_fixup<XT, EPI, IMPL>), dim3((unsigned)((std::min(MAX_WAVES, P.nmulti - w0) + 3) / 4)),
+                       dim3(256), 0, ctx->stream, a, w0);
     LN_LAUNCH(ctx);
   }
   return LIGHTNE_OK;
@@ -263,11 +266,11 @@ __device__ __forceinline__ void store_row64(double* __restrict__ out, int64_t k,
     *reinterpret_cast<double2*>(out + 2 * sub + 2 * G * q) = make_double2(acc[2 * q], acc[2 * q + 1]);
 }
 
-__global__ __launch_bounds__(256) void k_spmm64(SpmmArgs a, int64_t w_short) {
+__global__ __launch_bounds__(256) void k_spmm64(SpmmArgs a, int64_t w_short, int64_t wave0) {
   const int lane = threadIdx.x & 63;
   const int G = a.G, NG = 64 / G;
   const int g = lane / G, sub = lane - g * G;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t wave = wave0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   double acc[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) acc[i] = 0.0;
@@ -302,9 +305,9 @@ __global__ __launch_bounds__(256) void k_spmm64(SpmmArgs a, int64_t w_short) {
     store_row64(part + (int64_t)slot * a.k, a.k, G, sub, acc);
 }
 
-__global__ __launch_bounds__(256) void k_spmm64_fixup(SpmmArgs a) {
+__global__ __launch_bounds__(256) void k_spmm64_fixup(SpmmArgs a, int64_t wave0) {
   const int lane = threadIdx.x & 63;
-  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t w = wave0 + (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (w >= a.nmulti) return;
   const int64_t row = a.multi_row[w];
   const int64_t s0 = a.multi_first[w], s1 = a.multi_first[w + 1];
@@ -475,12 +478,14 @@ int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   }
   const int64_t waves_short = (a.n + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
-  if (waves >= ((int64_t)1 << 26))
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: %lld waves exceed one launch", (long long)waves);
-  hipLaunchKernelGGL(k_spmm64, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, a, waves_short);
-  LN_LAUNCH(ctx);
-  if (P.nmulti > 0) {
-    hipLaunchKernelGGL(k_spmm64_fixup, dim3((unsigned)((P.nmulti + 3) / 4)), dim3(256), 0, ctx->stream, a);
+  for (int64_t w0 = 0; w0 < waves; w0 += MAX_WAVES) {
+    hipLaunchKernelGGL(k_spmm64, dim3((unsigned)((std::min(MAX_WAVES, waves - w0) + 3) / 4)), dim3(256), 0,
+                       ctx->stream, a, waves_short, w0);
+    LN_LAUNCH(ctx);
+  }
+  for (int64_t w0 = 0; w0 < P.nmulti; w0 += MAX_WAVES) {
+    hipLaunchKernelGGL(k_spmm64_fixup, dim3((unsigned)((std::min(MAX_WAVES, P.nmulti - w0) + 3) / 4)), dim3(256),
+                       0, ctx->stream, a, w0);
     LN_LAUNCH(ctx);
   }
   return LIGHTNE_OK;

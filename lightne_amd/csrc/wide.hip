@@ -368,6 +368,7 @@ int rsvd_wide(lightne_ctx* ctx, lightne_csr* A, const lightne_rsvd_params* p, in
   LN_HIP(ctx, hipEventSynchronize(e[3]));
   st.t_sketch = lap(e[0], e[1]);
   t_spmm += lap(e[2], e[3]);
+  st.t_spmm_sketch = lap(e[2], e[3]);
   st.spmm_calls = 1;
   if (p->random_project_only) {
     LN_HIP(ctx, hipMemcpy2DAsync(out, ldo * 4, P1, kp * 4, k * 4, n, hipMemcpyDeviceToDevice, s));

@@ -269,6 +269,33 @@ def test_value_cache_gives_the_same_embedding(gpu, parts, sample):
     assert torch.equal(e2, e0)
 
 
+def test_value_cache_in_the_output_buffer(gpu):
+    """A fresh context's first pipeline call by row ranges has no workspace
+    room for the value cache: the pipeline's output buffer (written only by
+    stage 2) holds the values of the first slots it has room for, the rest read
+    the degrees (value_cache = 2), and the embedding is the one built without
+    a cache, bit for bit."""
+    import torch
+
+    hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
+    g = hg.to_device()
+    ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
+    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, csr_parts=6)
+    e0, S0, st0 = L.NetSMF(g, value_cache=-1, ctx=gpu, **kw)
+    c2 = L.Context(0)
+    try:
+        e1, S1, st1 = L.NetSMF(g, ctx=c2, **kw)
+        print(f"value_cache {st1['sparsify']['value_cache']}, table slots {st1['sparsify']['table_capacity']}, "
+              f"output floats {hg.n * 32}")
+        assert st1["sparsify"]["value_cache"] == 2
+        assert st1["sparsify"]["table_capacity"] > hg.n * 32  # the cache covers some slots only
+        assert st1["sparsify"]["nnz"] == st0["sparsify"]["nnz"]
+        np.testing.assert_array_equal(S1, S0)
+        assert torch.equal(e1, e0)
+    finally:
+        c2.close()
+
+
 def test_csr_prefix_copy(gpu):
     """The entries of the first rows holding a fraction of the nonzeros come to
     the host alone (row_ptr whole); out-of-range counts are errors."""

@@ -275,3 +275,34 @@ def test_netsmf_is_deterministic(gpu):
     e2, S2, _ = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=4, ctx=gpu)
     np.testing.assert_array_equal(S1, S2)
     np.testing.assert_array_equal(e1.cpu().numpy(), e2.cpu().numpy())
+
+
+def test_sketch_product_is_the_fp32_product_bit_for_bit(gpu):
+    """Y = A Omega gathers the sketch from its BF16 storage: the values are
+    BF16-representable, so the result is the FP32 product itself.  Rows of at
+    most 40 nonzeros are summed one nonzero at a time in index order, which a
+    host restatement reproduces exactly (with and without fused multiply-add);
+    the device must equal one of the two in every element."""
+    n, k, seed = 4000, 144, 99
+    rng = np.random.default_rng(3)
+    deg = rng.integers(0, 41, n)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(deg)
+    col = np.concatenate([np.sort(rng.choice(n, d, replace=False)) for d in deg]).astype(np.int32)
+    val = rng.standard_normal(rp[-1]).astype(np.float32)
+    csr = L.CSR.from_arrays(rp, col, val, ctx=gpu)
+    Y = L.MKLRedSVD(csr, k, random_project_only=True, seed=seed, ctx=gpu).run().matU.cpu().numpy()
+    Om = O.gaussian(seed, 0, n * k).reshape(n, k)
+    fma = np.zeros((n, k), np.float32)
+    plain = np.zeros((n, k), np.float32)
+    for j in range(int(deg.max())):
+        r = np.nonzero(deg > j)[0]
+        w = val[rp[r] + j][:, None]
+        x = Om[col[rp[r] + j]]
+        # w * x is exact in FP64; the sum is rounded once to FP64, then to FP32
+        fma[r] = (w.astype(np.float64) * x.astype(np.float64) + fma[r].astype(np.float64)).astype(np.float32)
+        plain[r] = plain[r] + w * x
+    same = (Y.view(np.int32) == fma.view(np.int32)) | (Y.view(np.int32) == plain.view(np.int32))
+    print(f"equal to the fused order: {np.mean(Y.view(np.int32) == fma.view(np.int32)):.6f}, "
+          f"to the unfused: {np.mean(Y.view(np.int32) == plain.view(np.int32)):.6f}")
+    assert same.all()
