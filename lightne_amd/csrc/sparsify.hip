@@ -129,9 +129,12 @@ __device__ __forceinline__ RawSlot raw_load(const TabView& t, uint64_t i) {
 
 __device__ __forceinline__ RawSlot raw_empty(const TabView& t) { return RawSlot{t.pk ? 0ull : EMPTY_KEY, 0u}; }
 
-// endpoints u <= v and the exact count; false when empty
-__device__ __forceinline__ bool raw_get(const TabView& t, const RawSlot& r, uint32_t& u, uint32_t& v, uint64_t& cnt) {
-  if (t.pk) {
+// endpoints u <= v and the exact count; false when empty.  PK: the layout
+// (8-byte slots) known at compile time
+template <bool PK>
+__device__ __forceinline__ bool raw_get_t(const TabView& t, const RawSlot& r, uint32_t& u, uint32_t& v,
+                                          uint64_t& cnt) {
+  if constexpr (PK) {
     const uint64_t w = r.a;
     if (!(w & PK_OCC)) return false;
     u = (uint32_t)((w >> PK_VB) & PK_VMASK);
@@ -139,12 +142,17 @@ __device__ __forceinline__ bool raw_get(const TabView& t, const RawSlot& r, uint
     cnt = w >> 54;
     if (w & PK_OVF) cnt += ovf_get(t.ovf, w & PK_WORD);
     return true;
+  } else {
+    if (r.a == EMPTY_KEY) return false;
+    u = (uint32_t)(r.a >> 32);
+    v = (uint32_t)r.a;
+    cnt = r.b;
+    return true;
   }
-  if (r.a == EMPTY_KEY) return false;
-  u = (uint32_t)(r.a >> 32);
-  v = (uint32_t)r.a;
-  cnt = r.b;
-  return true;
+}
+
+__device__ __forceinline__ bool raw_get(const TabView& t, const RawSlot& r, uint32_t& u, uint32_t& v, uint64_t& cnt) {
+  return t.pk ? raw_get_t<true>(t, r, u, v, cnt) : raw_get_t<false>(t, r, u, v, cnt);
 }
 
 __device__ __forceinline__ bool tab_get(const TabView& t, uint64_t i, uint32_t& u, uint32_t& v, uint64_t& cnt) {
@@ -475,10 +483,41 @@ constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in 
 static_assert(TR_ITEMS == 4, "the value cache is written as one float4 per thread");
 
 // slots per block of the trunc-log passes: a multiple of TR_ITEMS, so every
-// thread's TR_ITEMS slots start 16-byte aligned in the value cache
+// thread's TR_ITEMS slots start 16-byte aligned in the value cache (and
+// 32-byte aligned in a table of 8-byte slots)
 inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
   const uint64_t pb = (cap + nblk - 1) / nblk;
   return (pb + TR_ITEMS - 1) / TR_ITEMS * TR_ITEMS;
+}
+
+// a thread's TR_ITEMS consecutive slots [ib, ib + TR_ITEMS) of a scan, empty
+// past b1: a whole group of 8-byte slots as two 16-byte loads (ib is a
+// multiple of TR_ITEMS and the table starts 256-byte aligned)
+template <bool PK>
+__device__ __forceinline__ void load_group(const TabView& t, uint64_t ib, uint64_t b1, RawSlot* rs) {
+  if constexpr (PK) {
+    if (ib + TR_ITEMS <= b1) {
+      const auto* p = reinterpret_cast<const ulonglong2*>(t.pk + ib);
+      const ulonglong2 x0 = p[0], x1 = p[1];
+      rs[0] = RawSlot{x0.x, 0u};
+      rs[1] = RawSlot{x0.y, 0u};
+      rs[2] = RawSlot{x1.x, 0u};
+      rs[3] = RawSlot{x1.y, 0u};
+      return;
+    }
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? RawSlot{t.pk[ib + j], 0u} : RawSlot{0ull, 0u};
+  } else {
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      if (ib + j < b1) {
+        const Slot sl = t.wide[ib + j];
+        rs[j] = RawSlot{sl.key, sl.cnt};
+      } else {
+        rs[j] = RawSlot{EMPTY_KEY, 0u};
+      }
+    }
+  }
 }
 
 // CSR entries of a loaded slot whose row lies in [r0, r1): 0, 1 or 2 (eu:
@@ -487,11 +526,12 @@ inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
 // computed when want_value).  Slots with neither endpoint in the range skip
 // the degree reads; with the value cache there are none (a kept key's cached
 // value is > 0: log(adj) >= log(1 + 2^-52) > 0 stays positive as a float).
+template <bool PK>
 __device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const RawSlot& r, uint32_t r0, uint32_t r1,
                                          uint32_t& u, uint32_t& v, float& lwv, bool want_value, bool& eu, bool& ev) {
   uint64_t cnt;
   eu = ev = false;
-  if (!raw_get(a.t, r, u, v, cnt)) return 0;
+  if (!raw_get_t<PK>(a.t, r, u, v, cnt)) return 0;
   const bool iu = u >= r0 && u < r1;
   const bool iv = !a.upper && u != v && v >= r0 && v < r1;
   if (!iu && !iv) return 0;
@@ -513,7 +553,7 @@ __device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const R
 // pass 1: per-block (contiguous slot range) emitted entries and kept keys;
 // with HIST also the block's row histogram (HIST_BUCKETS ranges of 2^shift
 // rows), stored bucket-major: bh[bucket * gridDim.x + block]
-template <bool HIST>
+template <bool HIST, bool PK>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_t per_block,
                                                             uint32_t* __restrict__ blk_emit,
                                                             uint32_t* __restrict__ blk_kept, int shift, int nbuck,
@@ -529,15 +569,14 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
   for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
     const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
     RawSlot rs[TR_ITEMS];
-#pragma unroll
-    for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
+    load_group<PK>(a.t, ib, b1, rs);
     float lwo[TR_ITEMS];
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
       uint32_t u, v;
       float lwv = 0.f;
       bool eu, ev;
-      const int c = emit_slot(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, lwv, a.lw_out != nullptr, eu, ev);
+      const int c = emit_slot<PK>(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, lwv, a.lw_out != nullptr, eu, ev);
       e += c;
       kk += c > 0;
       lwo[j] = c > 0 ? lwv : 0.f;
@@ -598,6 +637,7 @@ __global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* 
 // slot order at the block's offset (the scan of pass 1's per-block counts,
 // or of a row range's), as sort keys ((row - r0) << colbits | col) and float
 // values.  No contended atomics; the emission order is deterministic.
+template <bool PK>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
                                                            const uint64_t* __restrict__ blk_off) {
   __shared__ uint32_t s_w[TR_THREADS / 64];
@@ -607,15 +647,14 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t
   for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
     const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
     RawSlot rs[TR_ITEMS];
-#pragma unroll
-    for (int j = 0; j < TR_ITEMS; ++j) rs[j] = ib + j < b1 ? raw_load(a.t, ib + j) : raw_empty(a.t);
+    load_group<PK>(a.t, ib, b1, rs);
     uint32_t uu[TR_ITEMS], vv[TR_ITEMS];
     float lw[TR_ITEMS];
     bool eu[TR_ITEMS], ev[TR_ITEMS];
     int c = 0;
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j)
-      c += emit_slot(a, ib + j, rs[j], r0, r1, uu[j], vv[j], lw[j], true, eu[j], ev[j]);
+      c += emit_slot<PK>(a, ib + j, rs[j], r0, r1, uu[j], vv[j], lw[j], true, eu[j], ev[j]);
     // block-wide exclusive scan of the threads' counts
     int incl = c;
 #pragma unroll
@@ -1020,8 +1059,11 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
           lw = (float*)ctx_buf(ctx, "s1_lw", lw_bytes);
           if (!lw) ctx->err.clear();  // no room after all: the emission reads the degrees
         }
-      } else if (tab_bytes + lw_bytes <= ws_b) {
-        lw = (float*)((char*)ctx->bufs["ws"].p + tab_bytes);  // row ranges: behind the table
+      } else if (tab_bytes + lw_bytes + (ext_allowed && ext_n > 0 ? csr_b : 0) <= ws_b) {
+        // row ranges: behind the table (with the caller's scratch at hand,
+        // only where the CSR still fits behind it: a cache the layout drops
+        // would be copied there after the count)
+        lw = (float*)((char*)ctx->bufs["ws"].p + tab_bytes);
         lw_in_ws = true;
       }
     }
@@ -1038,11 +1080,19 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     if (hist) {
       bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
       if (!bh) return LIGHTNE_ERR_OOM;
-      hipLaunchKernelGGL(k_trunc_count<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept,
-                         shift, nbuck, bh);
+      if (tv.pk)
+        hipLaunchKernelGGL((k_trunc_count<true, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+                           blk_kept, shift, nbuck, bh);
+      else
+        hipLaunchKernelGGL((k_trunc_count<true, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+                           blk_kept, shift, nbuck, bh);
     } else {
-      hipLaunchKernelGGL(k_trunc_count<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit, blk_kept,
-                         0, 0, (uint32_t*)nullptr);
+      if (tv.pk)
+        hipLaunchKernelGGL((k_trunc_count<false, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+                           blk_kept, 0, 0, (uint32_t*)nullptr);
+      else
+        hipLaunchKernelGGL((k_trunc_count<false, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+                           blk_kept, 0, 0, (uint32_t*)nullptr);
     }
     LN_LAUNCH(ctx);
     LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
@@ -1139,8 +1189,12 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.lw_n = lw_n;
     st.value_cache = lw ? (lw_ext ? 2u : 1u) : 0u;
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
-    hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
-                       (const uint64_t*)blk_off);
+    if (tv.pk)
+      hipLaunchKernelGGL(k_trunc_emit<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
+                         (const uint64_t*)blk_off);
+    else
+      hipLaunchKernelGGL(k_trunc_emit<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
+                         (const uint64_t*)blk_off);
     LN_LAUNCH(ctx);
     // rows sorted by column: a key's table slot depends on the order concurrent
     // inserts claimed slots, so the column order is what makes the matrix (and
@@ -1363,8 +1417,12 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         if ((int64_t)got != cnt)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu holds %llu entries, the histogram %lld",
                           (unsigned long long)r0, (unsigned long long)r1, (unsigned long long)got, (long long)cnt);
-        hipLaunchKernelGGL(k_trunc_emit, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
-                           (uint32_t)r1, (const uint64_t*)blk_off);
+        if (tv.pk)
+          hipLaunchKernelGGL(k_trunc_emit<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
+                             (uint32_t)r1, (const uint64_t*)blk_off);
+        else
+          hipLaunchKernelGGL(k_trunc_emit<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
+                             (uint32_t)r1, (const uint64_t*)blk_off);
         LN_LAUNCH(ctx);
         LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + colbits, ktmp, vtmp, 0, &kres, &vres));
       }
