@@ -608,14 +608,20 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
     for (int b = threadIdx.x; b < nbuck; b += TR_THREADS) bh[(uint64_t)b * gridDim.x + blockIdx.x] = s_h[b];
 }
 
-// row histogram totals: bucket b sums its nblk block counts
-__global__ void k_bh_totals(const uint32_t* __restrict__ bh, int nbuck, int nblk,
-                            unsigned long long* __restrict__ hist) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// row histogram totals: bucket b (one block per bucket) sums its nblk block
+// counts, read along the bucket's contiguous row of bh
+__global__ __launch_bounds__(256) void k_bh_totals(const uint32_t* __restrict__ bh, int nbuck, int nblk,
+                                                   unsigned long long* __restrict__ hist) {
+  __shared__ unsigned long long s_t[256 / 64];
+  const int b = blockIdx.x;
   if (b >= nbuck) return;
   unsigned long long t = 0;
-  for (int k = 0; k < nblk; ++k) t += bh[(uint64_t)b * nblk + k];
-  hist[b] = t;
+  for (int k = threadIdx.x; k < nblk; k += 256) t += bh[(uint64_t)b * nblk + k];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
+  if ((threadIdx.x & 63) == 0) s_t[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) hist[b] = s_t[0] + s_t[1] + s_t[2] + s_t[3];
 }
 
 // per-block entries of the buckets [lo, hi) (a row range)
@@ -967,7 +973,10 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   hipStream_t s = ctx->stream;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
-  const int nblk = 4096;
+  // blocks of the trunc-log passes (contiguous slot ranges): enough that the
+  // last round of resident blocks leaves little of the device idle on large
+  // tables (about seven 256-thread blocks fit a CU)
+  const int nblk = tv.cap >= (1ull << 30) ? 16384 : 4096;
   auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
   auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
   auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
@@ -1211,7 +1220,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     if (!bh) LN_CHECK(count_pass(true));
     auto* hist = (unsigned long long*)ctx_buf(ctx, "tr_hist", HIST_BUCKETS * 8);
     if (!hist) return LIGHTNE_ERR_OOM;
-    hipLaunchKernelGGL(k_bh_totals, dim3((nbuck + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh, nbuck, nblk,
+    hipLaunchKernelGGL(k_bh_totals, dim3(nbuck), dim3(256), 0, s, (const uint32_t*)bh, nbuck, nblk,
                        hist);
     LN_LAUNCH(ctx);
     std::vector<unsigned long long> hh(HIST_BUCKETS);
