@@ -127,8 +127,6 @@ __device__ __forceinline__ RawSlot raw_load(const TabView& t, uint64_t i) {
   return RawSlot{sl.key, sl.cnt};
 }
 
-__device__ __forceinline__ RawSlot raw_empty(const TabView& t) { return RawSlot{t.pk ? 0ull : EMPTY_KEY, 0u}; }
-
 // endpoints u <= v and the exact count; false when empty.  PK: the layout
 // (8-byte slots) known at compile time
 template <bool PK>
@@ -446,23 +444,27 @@ struct TruncArgs {
   int colbits;
   uint64_t* skey;
   float* sval;
-  // per-slot value cache: the count pass writes the CSR value of every slot
-  // below lw_n ((float)log(adj) when kept, else 0) to lw_out; an emission
-  // pass given lw_in reads it for those slots instead of the two degrees
+  // value cache, indexed by the rank of an occupied slot among all occupied
+  // slots in slot order: the count pass writes the CSR value of every
+  // occupied slot of rank below lw_n ((float)log(adj) when kept, else 0) to
+  // lw_out; an emission pass given lw_in reads it for those slots instead of
+  // the two degrees.  occ_off[block * TR_WAVES + wave]: the rank of the first
+  // occupied slot of that wave's quarter of the block.
   float* lw_out;
   const float* lw_in;
   uint64_t lw_n;
+  const uint64_t* occ_off;
 };
 
-// PathEmbed.h:203-210: adjusted weight of a key, evaluated in the reference's
+// PathEmbed.h:203-210: adjusted weight of a key of value val (count or
+// -sample 1 weight) and endpoint degrees du, dv, evaluated in the reference's
 // order (w * factor / d_u / d_v, diagonal weight doubled)
-__device__ __forceinline__ double trunc_adj(const TruncArgs& a, uint64_t i, uint32_t u, uint32_t v, uint64_t cnt) {
-  const double du = (double)a.deg[u];
-  const double dv = (double)a.deg[v];
-  const double val = a.t.wtab ? a.t.wtab[i] : (double)cnt;
+__device__ __forceinline__ double trunc_adj_w(const TruncArgs& a, uint32_t u, uint32_t v, double val, uint32_t du,
+                                              uint32_t dv) {
   const double wgh = (u != v) ? val : val * 2;
-  return wgh * a.factor / du / dv;
+  return wgh * a.factor / (double)du / (double)dv;
 }
+
 
 constexpr int TR_THREADS = 256;
 
@@ -480,14 +482,42 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
 
 constexpr int HIST_BUCKETS = 4096;
 constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in flight
-static_assert(TR_ITEMS == 4, "the value cache is written as one float4 per thread");
+constexpr int TR_WAVES = TR_THREADS / 64;
 
-// slots per block of the trunc-log passes: a multiple of TR_ITEMS, so every
-// thread's TR_ITEMS slots start 16-byte aligned in the value cache (and
-// 32-byte aligned in a table of 8-byte slots)
+// slots per block of the trunc-log passes: a multiple of TR_WAVES * TR_ITEMS.
+// Wave w of a block scans its own contiguous quarter of the block's slots
+// (wave_span), so every thread's TR_ITEMS slots start 32-byte aligned in a
+// table of 8-byte slots, and a wave numbers the occupied slots of its
+// quarter in slot order by itself (the value cache's index).
 inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
+  constexpr uint64_t U = (uint64_t)TR_WAVES * TR_ITEMS;
   const uint64_t pb = (cap + nblk - 1) / nblk;
-  return (pb + TR_ITEMS - 1) / TR_ITEMS * TR_ITEMS;
+  return (pb + U - 1) / U * U;
+}
+
+// the calling wave's quarter [w0, w1) of its block's slots; q (a multiple of
+// TR_ITEMS) is the same for every wave, so loops over [0, q) take the same
+// number of steps across the block (w1 < w0 + q only at the table's end)
+struct WaveSpan {
+  uint64_t w0, w1, q;
+};
+__device__ __forceinline__ WaveSpan wave_span(uint64_t cap, uint64_t per_block) {
+  const uint64_t q = per_block / TR_WAVES;
+  const uint64_t w0 = (uint64_t)blockIdx.x * per_block + (uint64_t)(threadIdx.x >> 6) * q;
+  return WaveSpan{w0, min(cap, w0 + q), q};
+}
+
+// exclusive prefix of c over the wave's lanes; *tot: the wave's total
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t c, uint32_t* tot) {
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += t;
+  }
+  *tot = __shfl(incl, 63, 64);
+  return incl - c;
 }
 
 // a thread's TR_ITEMS consecutive slots [ib, ib + TR_ITEMS) of a scan, empty
@@ -520,34 +550,32 @@ __device__ __forceinline__ void load_group(const TabView& t, uint64_t ib, uint64
   }
 }
 
-// CSR entries of a loaded slot whose row lies in [r0, r1): 0, 1 or 2 (eu:
-// row u, ev: row v); trunc-log keeps a key iff log(adj) > 0
-// (PathEmbed.h:203-217), and the entry's value is (float)log(adj) (lwv,
-// computed when want_value).  Slots with neither endpoint in the range skip
-// the degree reads; with the value cache there are none (a kept key's cached
-// value is > 0: log(adj) >= log(1 + 2^-52) > 0 stays positive as a float).
 template <bool PK>
-__device__ __forceinline__ int emit_slot(const TruncArgs& a, uint64_t i, const RawSlot& r, uint32_t r0, uint32_t r1,
-                                         uint32_t& u, uint32_t& v, float& lwv, bool want_value, bool& eu, bool& ev) {
-  uint64_t cnt;
-  eu = ev = false;
-  if (!raw_get_t<PK>(a.t, r, u, v, cnt)) return 0;
-  const bool iu = u >= r0 && u < r1;
-  const bool iv = !a.upper && u != v && v >= r0 && v < r1;
-  if (!iu && !iv) return 0;
-  if (a.lw_in && i < a.lw_n) {
-    const float l = a.lw_in[i];
-    if (!(l > 0.f)) return 0;
-    lwv = l;
-  } else {
-    const double adj = trunc_adj(a, i, u, v, cnt);
-    // log(adj) > 0  <=>  adj > 1 for any faithful log
-    if (!(adj > 1.0)) return 0;
-    if (want_value) lwv = (float)log(adj);
+__device__ __forceinline__ bool raw_occ(const RawSlot& r) {
+  if constexpr (PK)
+    return (r.a & PK_OCC) != 0;
+  else
+    return r.a != EMPTY_KEY;
+}
+
+// occupied slots of every wave's quarter of the trunc-log partition:
+// wocc[block * TR_WAVES + wave] (their scan numbers the occupied slots in
+// slot order for the value cache)
+template <bool PK>
+__global__ __launch_bounds__(TR_THREADS) void k_occ_counts(TabView t, uint64_t per_block,
+                                                           uint32_t* __restrict__ wocc) {
+  const WaveSpan ws = wave_span(t.cap, per_block);
+  const int lane = threadIdx.x & 63;
+  uint32_t c = 0;
+  for (uint64_t t0 = 0; t0 < ws.q; t0 += 64 * TR_ITEMS) {
+    RawSlot rs[TR_ITEMS];
+    load_group<PK>(t, ws.w0 + t0 + (uint64_t)lane * TR_ITEMS, ws.w1, rs);
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) c += raw_occ<PK>(rs[j]) ? 1u : 0u;
   }
-  eu = iu;
-  ev = iv;
-  return (int)iu + (int)iv;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+  if (lane == 0) wocc[(uint64_t)blockIdx.x * TR_WAVES + (threadIdx.x >> 6)] = c;
 }
 
 // pass 1: per-block (contiguous slot range) emitted entries and kept keys;
@@ -558,44 +586,72 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
                                                             uint32_t* __restrict__ blk_emit,
                                                             uint32_t* __restrict__ blk_kept, int shift, int nbuck,
                                                             uint32_t* __restrict__ bh) {
-  __shared__ uint32_t s_w[TR_THREADS / 64];
+  __shared__ uint32_t s_w[TR_WAVES];
   __shared__ uint32_t s_h[HIST ? HIST_BUCKETS : 1];
   if (HIST) {
     for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
     __syncthreads();
   }
-  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
+  const WaveSpan ws = wave_span(a.t.cap, per_block);
+  const int lane = threadIdx.x & 63;
+  // occupied rank of the wave's next occupied slot (value cache index)
+  uint64_t orun = a.lw_out ? a.occ_off[(uint64_t)blockIdx.x * TR_WAVES + (threadIdx.x >> 6)] : 0;
   uint32_t e = 0, kk = 0;
-  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
-    const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
+  for (uint64_t t0 = 0; t0 < ws.q; t0 += 64 * TR_ITEMS) {
+    const uint64_t ib = ws.w0 + t0 + (uint64_t)lane * TR_ITEMS;
     RawSlot rs[TR_ITEMS];
-    load_group<PK>(a.t, ib, b1, rs);
+    load_group<PK>(a.t, ib, ws.w1, rs);
+    // decode the group, then issue all its degree reads (and -sample 1
+    // weights) before using any: eight random reads in flight per thread
+    uint32_t uu[TR_ITEMS], vv[TR_ITEMS], du[TR_ITEMS], dv[TR_ITEMS];
+    uint64_t cc[TR_ITEMS];
+    double wv[TR_ITEMS];
+    uint32_t occ = 0;
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      if (raw_get_t<PK>(a.t, rs[j], uu[j], vv[j], cc[j])) {
+        occ |= 1u << j;
+      } else {
+        uu[j] = vv[j] = 0;  // an empty slot reads vertex 0's degree, unused
+        cc[j] = 0;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      du[j] = a.deg[uu[j]];
+      dv[j] = a.deg[vv[j]];
+      wv[j] = (a.t.wtab && ((occ >> j) & 1u)) ? a.t.wtab[ib + j] : (double)cc[j];
+    }
     float lwo[TR_ITEMS];
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
-      uint32_t u, v;
-      float lwv = 0.f;
-      bool eu, ev;
-      const int c = emit_slot<PK>(a, ib + j, rs[j], 0u, 0xffffffffu, u, v, lwv, a.lw_out != nullptr, eu, ev);
-      e += c;
-      kk += c > 0;
-      lwo[j] = c > 0 ? lwv : 0.f;
+      lwo[j] = 0.f;
+      if (!((occ >> j) & 1u)) continue;
+      const uint32_t u = uu[j], v = vv[j];
+      const double adj = trunc_adj_w(a, u, v, wv[j], du[j], dv[j]);
+      // log(adj) > 0  <=>  adj > 1 for any faithful log
+      if (!(adj > 1.0)) continue;
+      if (a.lw_out) lwo[j] = (float)log(adj);
+      const bool ev = !a.upper && u != v;
+      e += 1 + (ev ? 1 : 0);
+      ++kk;
       if (HIST) {
-        if (eu) atomicAdd(&s_h[u >> shift], 1u);
+        atomicAdd(&s_h[u >> shift], 1u);
         if (ev) atomicAdd(&s_h[v >> shift], 1u);
       }
     }
-    // the value cache: one 16-byte store per thread (a wave writes 1 KB
-    // contiguously; ib is a multiple of TR_ITEMS), scalar stores at the tail
+    // the value cache: the wave's occupied slots in slot order take
+    // consecutive ranks (a wave writes one contiguous run of floats)
     if (a.lw_out) {
-      const uint64_t lim = min(b1, a.lw_n);
-      if (ib + TR_ITEMS <= lim) {
-        *reinterpret_cast<float4*>(a.lw_out + ib) = make_float4(lwo[0], lwo[1], lwo[2], lwo[3]);
-      } else {
+      uint32_t wtot;
+      uint64_t ci = orun + wave_excl_scan((uint32_t)__popc(occ), &wtot);
 #pragma unroll
-        for (int j = 0; j < TR_ITEMS; ++j)
-          if (ib + j < lim) a.lw_out[ib + j] = lwo[j];
+      for (int j = 0; j < TR_ITEMS; ++j) {
+        if (!((occ >> j) & 1u)) continue;
+        if (ci < a.lw_n) a.lw_out[ci] = lwo[j];
+        ++ci;
       }
+      orun += wtot;
     }
   }
   const uint32_t te = block_sum_u32(e, s_w);
@@ -639,28 +695,91 @@ __global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* 
     deg[i] = (uint32_t)(off[i + 1] - off[i]);
 }
 
-// pass 2: same partition; the entries of rows [r0, r1) (all rows: 0, ~0) in
-// slot order at the block's offset (the scan of pass 1's per-block counts,
-// or of a row range's), as sort keys ((row - r0) << colbits | col) and float
-// values.  No contended atomics; the emission order is deterministic.
+// pass 2: same partition; the entries of rows [r0, r1) (all rows: 0, ~0) at
+// the block's offset (the scan of pass 1's per-block counts, or of a row
+// range's), as sort keys ((row - r0) << colbits | col) and float values, in
+// an order fixed by the partition (the sort that follows orders them
+// completely: no two entries share a key).  No contended atomics.
 template <bool PK>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
                                                            const uint64_t* __restrict__ blk_off) {
-  __shared__ uint32_t s_w[TR_THREADS / 64];
-  const uint64_t b0 = (uint64_t)blockIdx.x * per_block, b1 = min(a.t.cap, b0 + per_block);
+  __shared__ uint32_t s_w[TR_WAVES];
+  const WaveSpan ws = wave_span(a.t.cap, per_block);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint64_t run = blk_off[blockIdx.x];
-  for (uint64_t t0 = b0; t0 < b1; t0 += TR_THREADS * TR_ITEMS) {
-    const uint64_t ib = t0 + (uint64_t)threadIdx.x * TR_ITEMS;
+  uint64_t orun = a.lw_in ? a.occ_off[(uint64_t)blockIdx.x * TR_WAVES + wid] : 0;
+  for (uint64_t t0 = 0; t0 < ws.q; t0 += 64 * TR_ITEMS) {
+    const uint64_t ib = ws.w0 + t0 + (uint64_t)lane * TR_ITEMS;
     RawSlot rs[TR_ITEMS];
-    load_group<PK>(a.t, ib, b1, rs);
+    load_group<PK>(a.t, ib, ws.w1, rs);
+    // decode the group: occupied slots (occ), those with an endpoint in the
+    // range (inr) and which rows they emit (eu: row u, ev: row v)
     uint32_t uu[TR_ITEMS], vv[TR_ITEMS];
-    float lw[TR_ITEMS];
+    uint64_t cc[TR_ITEMS];
     bool eu[TR_ITEMS], ev[TR_ITEMS];
+    uint32_t occ = 0, inr = 0;
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      eu[j] = ev[j] = false;
+      if (!raw_get_t<PK>(a.t, rs[j], uu[j], vv[j], cc[j])) {
+        uu[j] = vv[j] = 0;
+        continue;
+      }
+      occ |= 1u << j;
+      eu[j] = uu[j] >= r0 && uu[j] < r1;
+      ev[j] = !a.upper && uu[j] != vv[j] && vv[j] >= r0 && vv[j] < r1;
+      if (eu[j] || ev[j]) inr |= 1u << j;
+    }
+    // value cache index of the thread's first occupied slot
+    uint64_t ci = 0;
+    if (a.lw_in) {
+      uint32_t wtot;
+      ci = orun + wave_excl_scan((uint32_t)__popc(occ), &wtot);
+      orun += wtot;
+    }
+    // every read of the group before any use: cached values of in-range
+    // slots, degrees (and -sample 1 weights) of in-range slots past the cache
+    float lw[TR_ITEMS];
+    uint32_t du[TR_ITEMS], dv[TR_ITEMS];
+    double wv[TR_ITEMS];
+    uint32_t cached = 0;
+#pragma unroll
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      const uint64_t cj = ci + (uint64_t)__popc(occ & ((1u << j) - 1u));
+      lw[j] = 0.f;
+      du[j] = dv[j] = 1;
+      wv[j] = 0.0;
+      if (!((inr >> j) & 1u)) continue;
+      if (a.lw_in && cj < a.lw_n) {
+        cached |= 1u << j;
+        lw[j] = a.lw_in[cj];
+      } else {
+        du[j] = a.deg[uu[j]];
+        dv[j] = a.deg[vv[j]];
+        wv[j] = a.t.wtab ? a.t.wtab[ib + j] : (double)cc[j];
+      }
+    }
+    // trunc-log keeps a key iff log(adj) > 0 (PathEmbed.h:203-217); a kept
+    // key's cached value is > 0 (log(adj) >= log(1 + 2^-52) stays positive
+    // as a float), a dropped one's is 0
     int c = 0;
 #pragma unroll
-    for (int j = 0; j < TR_ITEMS; ++j)
-      c += emit_slot<PK>(a, ib + j, rs[j], r0, r1, uu[j], vv[j], lw[j], true, eu[j], ev[j]);
+    for (int j = 0; j < TR_ITEMS; ++j) {
+      if (!((inr >> j) & 1u)) continue;
+      bool keep;
+      if ((cached >> j) & 1u) {
+        keep = lw[j] > 0.f;
+      } else {
+        const double adj = trunc_adj_w(a, uu[j], vv[j], wv[j], du[j], dv[j]);
+        keep = adj > 1.0;  // log(adj) > 0 <=> adj > 1 for any faithful log
+        if (keep) lw[j] = (float)log(adj);
+      }
+      if (!keep) {
+        eu[j] = ev[j] = false;
+        continue;
+      }
+      c += (eu[j] ? 1 : 0) + (ev[j] ? 1 : 0);
+    }
     // block-wide exclusive scan of the threads' counts
     int incl = c;
 #pragma unroll
@@ -672,7 +791,7 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t
     if (lane == 63) s_w[wid] = (uint32_t)incl;
     __syncthreads();
     uint32_t wbase = 0, tot = 0;
-    for (int w = 0; w < TR_THREADS / 64; ++w) {
+    for (int w = 0; w < TR_WAVES; ++w) {
       if (w < wid) wbase += s_w[w];
       tot += s_w[w];
     }
@@ -980,7 +1099,10 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
   auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
   auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
-  if (!blk_emit || !blk_kept || !blk_off) return LIGHTNE_ERR_OOM;
+  // occupied slots per wave quarter and their scan (the value cache's index)
+  auto* wocc = (uint32_t*)ctx_buf(ctx, "tr_wocc", (size_t)nblk * TR_WAVES * sizeof(uint32_t));
+  auto* occ_off = (uint64_t*)ctx_buf(ctx, "tr_occ", ((size_t)nblk * TR_WAVES + 1) * sizeof(uint64_t));
+  if (!blk_emit || !blk_kept || !blk_off || !wocc || !occ_off) return LIGHTNE_ERR_OOM;
   TruncArgs ta;
   int64_t nnz = 0;
   auto* deg = (uint32_t*)ctx_buf(ctx, "s1_deg", (size_t)g->n * 4);
@@ -991,27 +1113,29 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
   const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
   uint32_t* bh = nullptr;  // per-block row histograms, when taken
-  // Value cache: the count pass leaves every slot's CSR value in a float
-  // array right behind the table in the workspace (context-owned builds
-  // only), so the emission passes read it instead of two degrees at random
-  // places.  Written whenever the workspace holds it beside the table, or as
-  // a buffer of its own for a single-sort build that surely fits beside it;
-  // the layout chosen after the count keeps it only where it needs no extra
-  // memory (value_cache = 1: always, growing the workspace; -1: never).
-  // Where the workspace cannot hold it, the caller's scratch xs holds the
-  // values of the first slots it has room for (the rest read the degrees).
+  // Value cache: the count pass leaves the CSR value of every occupied slot
+  // (one float per distinct key, in slot order) in an array right behind the
+  // table in the workspace (context-owned builds only), so the emission
+  // passes read it instead of two degrees at random places.  Written
+  // whenever the workspace holds it beside the table, or as a buffer of its
+  // own for a single-sort build that surely fits beside it; the layout chosen
+  // after the count keeps it only where it needs no extra memory
+  // (value_cache = 1: always, growing the workspace; -1: never).  Where the
+  // workspace cannot hold it, the caller's scratch xs holds the values of
+  // the first occupied slots it has room for (the rest read the degrees).
   const size_t tab_bytes = al(tv.cap * (tv.pk ? 8 : sizeof(Slot))) + (tv.wtab ? al(tv.cap * sizeof(double)) : 0);
-  const size_t lw_bytes = al(tv.cap * sizeof(float));
+  size_t lw_bytes = 0;  // set by the count pass: one float per distinct key
   const bool lw_force = p->value_cache > 0;
   bool lw_allowed = ctx_owned && p->value_cache >= 0;
   float* lw = nullptr;
   bool lw_in_ws = false;  // else a buffer of its own (single-sort builds only)
-  bool lw_ext = false;    // in the caller's scratch, slots [0, lw_n)
+  bool lw_ext = false;    // in the caller's scratch, occupied ranks [0, lw_n)
   uint64_t lw_n = 0;
   const bool ext_allowed = xs && p->value_cache >= 0;
-  const uint64_t ext_n = std::min<uint64_t>(tv.cap, xs_bytes / sizeof(float) / TR_ITEMS * TR_ITEMS);
+  uint64_t ext_n = 0;  // set by the count pass
   // an in-workspace cache the layout cannot keep: the values of its first
-  // ext_n slots move to the caller's scratch before anything overwrites them
+  // ext_n occupied slots move to the caller's scratch before anything
+  // overwrites them
   auto demote = [&]() -> int {
     if (lw_ext) return LIGHTNE_OK;
     if (lw && lw_in_ws && ext_allowed && ext_n > 0) {
@@ -1039,6 +1163,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     const double vol = (double)(uint64_t)g->m;
     st.factor = vol * (double)(uint64_t)g->m / st.M_exact / 2 / (double)p->negative;
     LN_HIP(ctx, hipEventRecord(ctx->ev[4], s));
+    lw_bytes = al(std::max<uint64_t>(distinct, 1) * sizeof(float));
+    ext_n = std::min<uint64_t>(distinct, xs_bytes / sizeof(float));
     ta = TruncArgs{};
     ta.t = tv;
     ta.deg = deg;
@@ -1047,7 +1173,10 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.colbits = colbits;
     lw = nullptr;
     lw_in_ws = false;
-    if (lw_allowed) {
+    // the caller's scratch, where it holds the whole cache, costs the
+    // workspace nothing (an in-workspace cache shrinks the row ranges)
+    const bool ext_whole = ext_allowed && distinct > 0 && ext_n >= distinct;
+    if (lw_allowed && (lw_force || !ext_whole)) {
       const size_t nb = 2 * (size_t)distinct + 1;  // nnz <= 2 x kept <= 2 x distinct
       const size_t csr_b = al((g->n + 1) * 8) + 2 * al(nb * 4);
       const size_t ws_b = ctx->bufs["ws"].bytes;
@@ -1077,7 +1206,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       }
     }
     lw_ext = false;
-    lw_n = tv.cap;
+    lw_n = distinct;
     if (!lw && ext_allowed && ext_n > 0) {
       lw = xs;
       lw_ext = true;
@@ -1086,6 +1215,16 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.lw_out = lw;
     ta.lw_n = lw_n;
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
+    if (lw) {
+      // rank of each wave quarter's first occupied slot
+      if (tv.pk)
+        hipLaunchKernelGGL(k_occ_counts<true>, dim3(nblk), dim3(TR_THREADS), 0, s, tv, per_block, wocc);
+      else
+        hipLaunchKernelGGL(k_occ_counts<false>, dim3(nblk), dim3(TR_THREADS), 0, s, tv, per_block, wocc);
+      LN_LAUNCH(ctx);
+      LN_CHECK(scan_u32_to_u64(ctx, wocc, occ_off, (int64_t)nblk * TR_WAVES));
+      ta.occ_off = occ_off;
+    }
     if (hist) {
       bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
       if (!bh) return LIGHTNE_ERR_OOM;
@@ -1106,10 +1245,16 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     LN_LAUNCH(ctx);
     LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
     std::vector<uint32_t> hk(nblk);
-    uint64_t htot = 0;
+    uint64_t htot = 0, otot = distinct;
     LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    if (lw)
+      LN_HIP(ctx, hipMemcpyAsync(&otot, occ_off + (size_t)nblk * TR_WAVES, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 s));
     LN_HIP(ctx, hipStreamSynchronize(s));
+    if (otot != distinct)
+      return ctx_fail(ctx, LIGHTNE_ERR_HIP, "value cache numbered %llu occupied slots, the table holds %llu keys",
+                      (unsigned long long)otot, (unsigned long long)distinct);
     nnz = (int64_t)htot;
     st.kept_upper = 0;
     for (uint32_t v : hk) st.kept_upper += v;
@@ -1194,7 +1339,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.skey = skey;
     ta.sval = v_emit;
     ta.lw_out = nullptr;
-    ta.lw_in = lw;
+    ta.lw_in = ta.occ_off ? lw : nullptr;  // numbered by the count pass
     ta.lw_n = lw_n;
     st.value_cache = lw ? (lw_ext ? 2u : 1u) : 0u;
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
@@ -1404,7 +1549,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     ta.skey = pkey;
     ta.sval = pval;
     ta.lw_out = nullptr;
-    ta.lw_in = lw;
+    ta.lw_in = ta.occ_off ? lw : nullptr;  // numbered by the count pass
     ta.lw_n = lw_n;
     st.value_cache = lw ? (lw_ext ? 2u : 1u) : 0u;
     int64_t base = 0;

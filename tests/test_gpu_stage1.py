@@ -269,26 +269,30 @@ def test_value_cache_gives_the_same_embedding(gpu, parts, sample):
     assert torch.equal(e2, e0)
 
 
-def test_value_cache_in_the_output_buffer(gpu):
+@pytest.mark.parametrize("rank,dim", [(64, 32), (256, 256)])
+def test_value_cache_in_the_output_buffer(gpu, rank, dim):
     """A fresh context's first pipeline call by row ranges has no workspace
     room for the value cache: the pipeline's output buffer (written only by
-    stage 2) holds the values of the first slots it has room for, the rest read
-    the degrees (value_cache = 2), and the embedding is the one built without
-    a cache, bit for bit."""
+    stage 2) holds the values of the first occupied slots it has room for (all
+    of them at dim 256; at dim 32 the rest read the degrees), value_cache = 2,
+    and the embedding is the one built without a cache, bit for bit."""
     import torch
 
     hg = L.chung_lu(20000, 2.1, 10.0, 500.0, 7)
     g = hg.to_device()
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 20.0)
-    kw = dict(walk_len=10, rank=64, dim=32, sample_ratio=ratio, seed=3, csr_parts=6)
+    kw = dict(walk_len=10, rank=rank, dim=dim, sample_ratio=ratio, seed=3, csr_parts=6)
     e0, S0, st0 = L.NetSMF(g, value_cache=-1, ctx=gpu, **kw)
     c2 = L.Context(0)
     try:
         e1, S1, st1 = L.NetSMF(g, ctx=c2, **kw)
-        print(f"value_cache {st1['sparsify']['value_cache']}, table slots {st1['sparsify']['table_capacity']}, "
-              f"output floats {hg.n * 32}")
+        keys = st1["sparsify"]["distinct_keys"]
+        print(f"value_cache {st1['sparsify']['value_cache']}, distinct keys {keys}, output floats {hg.n * dim}")
         assert st1["sparsify"]["value_cache"] == 2
-        assert st1["sparsify"]["table_capacity"] > hg.n * 32  # the cache covers some slots only
+        if dim == 32:
+            assert keys > hg.n * dim  # the cache covers the first occupied slots only
+        else:
+            assert keys <= hg.n * dim  # the whole cache
         assert st1["sparsify"]["nnz"] == st0["sparsify"]["nnz"]
         np.testing.assert_array_equal(S1, S0)
         assert torch.equal(e1, e0)
