@@ -132,9 +132,9 @@ typedef struct {
                                when one global sort does not fit; k >= 2:
                                at least k ranges                           */
   int32_t value_cache;      /* pipeline only: 0 auto (the count pass caches
-                               each slot's CSR value where memory allows), 1
-                               always (the workspace grows for it), -1 never;
-                               the CSR is the same either way               */
+                               each distinct key's CSR value where memory
+                               allows), 1 always (the workspace grows for
+                               it), -1 never; the CSR is the same either way */
   int32_t wide_offsets;     /* 0: the walks read 32-bit row pointers when the
                                graph has fewer than 2^32 directed edges and
                                the graph's 64-bit offsets otherwise; 1: the
@@ -160,10 +160,11 @@ typedef struct {
   uint32_t table_layout;   /* 1 = 16-byte slots, 2 = 8-byte slots          */
   uint32_t csr_parts;      /* row ranges the CSR was built in (1 = one sort) */
   uint32_t value_cache;    /* 1: the CSR values came from the count pass's
-                              per-slot cache, not a second read of degrees;
-                              2: the cache covered the first slots only
-                              (lightne_netsmf keeps it in its output buffer,
-                              which stage 2 writes later); 0: no cache     */
+                              cache in the workspace, not a second read of
+                              degrees; 2: from the cache in lightne_netsmf's
+                              output buffer (which stage 2 writes later),
+                              covering all keys or the first occupied slots
+                              (the rest read the degrees); 0: no cache     */
   uint32_t sampler_runs;   /* 1, or 2 when the workspace grew after the count
                               pass and the sampler ran again                */
 } lightne_sparsify_stats;

@@ -319,7 +319,8 @@ def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample
     """``table_layout``: 0 auto, 1 16-byte slots, 2 8-byte slots (n <= 2^26).
     ``csr_parts``: 0 auto, k >= 2 builds the CSR in at least k row ranges.
     ``value_cache`` (pipeline): 0 auto, 1 always, -1 never cache the count
-    pass's per-slot CSR values for the emission (same CSR either way).
+    pass's CSR value of every distinct key for the emission (same CSR either
+    way).
     ``wide_offsets``: 0 auto (32-bit row pointers for the walks when the graph
     has fewer than 2^32 directed edges), 1 the graph's 64-bit offsets always
     (same samples)."""

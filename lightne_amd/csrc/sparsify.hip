@@ -1371,11 +1371,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     std::vector<unsigned long long> hh(HIST_BUCKETS);
     LN_HIP(ctx, hipMemcpyAsync(hh.data(), hist, nbuck * 8, hipMemcpyDeviceToHost, s));
     LN_HIP(ctx, hipStreamSynchronize(s));
-    uint64_t hsum = 0, hmax = 0;
-    for (int b = 0; b < nbuck; ++b) {
-      hsum += hh[b];
-      hmax = std::max<uint64_t>(hmax, hh[b]);
-    }
+    uint64_t hsum = 0;
+    for (int b = 0; b < nbuck; ++b) hsum += hh[b];
     if (hsum != (uint64_t)nnz)
       return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row histogram holds %llu entries, the count pass %lld",
                       (unsigned long long)hsum, (long long)nnz);
@@ -1389,6 +1386,18 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // left beside the table, the value cache (lwb bytes) and the CSR
     std::vector<std::pair<int, int>> rng;  // bucket ranges
     std::vector<uint64_t> rcnt;
+    // Where a range's sort buffers (keys, key temporary: 8 bytes per entry;
+    // values, value temporary: 4) live.  The CSR slices of the ranges after
+    // it are still unwritten, so its col and val arrays have a free tail of
+    // 4 bytes per later entry each:
+    //   RP_OWN   all four in the range buffer region (24 B per entry there);
+    //   RP_KEYS  keys in the col tail, key temporary in the val tail, values
+    //            in the range buffer region (8 B per entry there);
+    //   RP_TAILS keys and values in the col tail, temporaries in the val tail.
+    // Early ranges, with many entries after them, can thus be larger than
+    // the range buffer region alone allows: fewer ranges, fewer table scans.
+    enum { RP_OWN = 0, RP_KEYS = 1, RP_TAILS = 2 };
+    std::vector<int> rplace;
     size_t pe = 1, part_bytes = 0;
     // in_ws: the ranges must fit the workspace as it is
     auto plan = [&](size_t lwb, bool in_ws) -> int {
@@ -1406,42 +1415,86 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         const size_t fixed = std::max(tab_bytes + lwb, front_reserve) + csr_bytes;
         part_room = std::min(part_room, ws_now > fixed ? ws_now - fixed : 0);
       }
-      uint64_t cap_e;
-      if (p->csr_parts >= 2) {
-        cap_e = ((uint64_t)nnz + p->csr_parts - 1) / p->csr_parts;
-      } else {
-        cap_e = part_room > 4096 ? (part_room - 4096) / 24 : 0;
-        cap_e = std::min<uint64_t>(cap_e, ((uint64_t)nnz + 3) / 4);  // at least 4 ranges: memory for later stages
-      }
-      cap_e = std::max<uint64_t>(cap_e, hmax);
-      cap_e = std::max<uint64_t>(cap_e, 1);
-      if (p->csr_parts < 2 && 24 * cap_e + 4096 > part_room)
-        return ctx_fail(ctx, LIGHTNE_ERR_OOM,
-                        "sparsifier of %lld entries: %.1f GB of table and %.1f GB of CSR leave %.1f GB for sort "
-                        "buffers, the densest row range needs %.1f GB",
-                        (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, 24.0 * hmax / 1e9);
+      const bool forced = p->csr_parts >= 2;
+      // at least 4 ranges: memory for later stages
+      const uint64_t quarter = ((uint64_t)nnz + 3) / 4;
+      // entries a range may hold in the range buffer region alone (RP_OWN),
+      // with its keys in the CSR tails (RP_KEYS), or in the tails alone
+      // (RP_TAILS), when `rem` entries (this range's and the later ones')
+      // are still to be built; 256-byte alignment of every array
+      const uint64_t own_cap = part_room > 1024 ? (part_room - 1024) / 24 : 0;
+      auto cap_of = [&](uint64_t rem, int* place) -> uint64_t {
+        uint64_t best = own_cap;
+        *place = RP_OWN;
+        if (!forced) {
+          const uint64_t keys_cap = std::min<uint64_t>(4 * rem > 768 ? (4 * rem - 768) / 12 : 0,
+                                                       part_room > 512 ? (part_room - 512) / 8 : 0);
+          const uint64_t tails_cap = 4 * rem > 1024 ? (4 * rem - 1024) / 16 : 0;
+          if (keys_cap > best) {
+            best = keys_cap;
+            *place = RP_KEYS;
+          }
+          if (tails_cap > best) {
+            best = tails_cap;
+            *place = RP_TAILS;
+          }
+        }
+        return std::min<uint64_t>(best, quarter);
+      };
       rng.clear();
       rcnt.clear();
+      rplace.clear();
       // ranges of at most 2^24 rows: (row - r0) << colbits | col then has at
       // most 24 + 26 = 50 bits, six radix passes (two of 9 bits)
       const int max_buckets = std::max(1, (1 << 24) >> shift);
-      int lo = 0;
+      const uint64_t forced_cap = forced ? ((uint64_t)nnz + p->csr_parts - 1) / p->csr_parts : 0;
+      uint64_t rem = (uint64_t)nnz;
+      int lo = 0, place = RP_OWN;
       uint64_t acc = 0;
+      uint64_t cap_e = forced ? forced_cap : cap_of(rem, &place);
       for (int b = 0; b < nbuck; ++b) {
-        if (b > lo && (acc + hh[b] > cap_e || b - lo + 1 > max_buckets)) {
+        if (b > lo && (acc + hh[b] > std::max<uint64_t>(cap_e, 1) || b - lo + 1 > max_buckets)) {
           rng.push_back({lo, b});
           rcnt.push_back(acc);
+          rplace.push_back(place);
+          rem -= acc;
           lo = b;
           acc = 0;
+          cap_e = forced ? forced_cap : cap_of(rem, &place);
         }
         acc += hh[b];
       }
       rng.push_back({lo, nbuck});
       rcnt.push_back(acc);
-      uint64_t pmax = 0;
-      for (uint64_t c : rcnt) pmax = std::max(pmax, c);
-      pe = (size_t)std::max<uint64_t>(pmax, 1);
-      part_bytes = 2 * al(pe * 8) + 2 * al(pe * 4);
+      rplace.push_back(place);
+      // each range takes the placement that needs the least of the range
+      // buffer region among those its CSR tails allow (a range the plan made
+      // larger than its tails hold, a single dense bucket, falls back to its
+      // own region); the region is sized for the largest need
+      part_bytes = 0;
+      uint64_t pmax = 1, done = 0;
+      for (size_t r = 0; r < rng.size(); ++r) {
+        const uint64_t e = std::max<uint64_t>(rcnt[r], 1);
+        done += rcnt[r];
+        const uint64_t tail = 4 * ((uint64_t)nnz - done);  // bytes free in each of col and val after it
+        if (al(e * 8) + al(e * 4) + 256 <= tail)
+          rplace[r] = RP_TAILS;
+        else if (al(e * 8) + 256 <= tail)
+          rplace[r] = RP_KEYS;
+        else
+          rplace[r] = RP_OWN;
+        const size_t need = rplace[r] == RP_OWN ? 2 * al(e * 8) + 2 * al(e * 4)
+                            : rplace[r] == RP_KEYS ? 2 * al(e * 4)
+                                                   : 0;
+        part_bytes = std::max(part_bytes, need);
+        pmax = std::max(pmax, e);
+      }
+      pe = (size_t)pmax;
+      if (!forced && part_bytes > part_room)
+        return ctx_fail(ctx, LIGHTNE_ERR_OOM,
+                        "sparsifier of %lld entries: %.1f GB of table and %.1f GB of CSR leave %.1f GB for sort "
+                        "buffers, the row ranges need %.1f GB",
+                        (long long)nnz, tab_bytes / 1e9, csr_bytes / 1e9, part_room / 1e9, part_bytes / 1e9);
       return LIGHTNE_OK;
     };
     // keep the value cache only inside the workspace and where the plan with
@@ -1471,6 +1524,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       if (ctx_owned && p->csr_parts < 2 && ws_now > 0) {
         const auto rng_free = rng;
         const auto rcnt_free = rcnt;
+        const auto rplace_free = rplace;
         const size_t pe_free = pe, part_free = part_bytes;
         const bool fits = plan(0, true) == LIGHTNE_OK;
         ctx->err.clear();
@@ -1478,6 +1532,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
               std::max(tab_bytes + part_bytes, front_reserve) + csr_bytes <= ws_now)) {
           rng = rng_free;
           rcnt = rcnt_free;
+          rplace = rplace_free;
           pe = pe_free;
           part_bytes = part_free;
         }
@@ -1542,12 +1597,6 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       col = (int32_t*)mcol.p;
       val = (float*)mval.p;
     }
-    auto* pkey = (uint64_t*)pbuf;
-    auto* ktmp = (uint64_t*)(pbuf + al(pe * 8));
-    auto* pval = (float*)(pbuf + 2 * al(pe * 8));
-    auto* vtmp = (float*)(pbuf + 2 * al(pe * 8) + al(pe * 4));
-    ta.skey = pkey;
-    ta.sval = pval;
     ta.lw_out = nullptr;
     ta.lw_in = ta.occ_off ? lw : nullptr;  // numbered by the count pass
     ta.lw_n = lw_n;
@@ -1557,6 +1606,31 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       const uint64_t r0 = (uint64_t)rng[r].first << shift;
       const uint64_t r1 = std::min<uint64_t>((uint64_t)rng[r].second << shift, (uint64_t)g->n);
       const int64_t cnt = (int64_t)rcnt[r];
+      // this range's sort buffers (the plan checked that they fit)
+      const size_t e = (size_t)std::max<int64_t>(cnt, 1);
+      auto up = [&](const void* q) { return (char*)(((uintptr_t)q + 255) & ~(uintptr_t)255); };
+      uint64_t *pkey, *ktmp;
+      float *pval, *vtmp;
+      if (rplace[r] == RP_OWN) {
+        pkey = (uint64_t*)pbuf;
+        ktmp = (uint64_t*)(pbuf + al(e * 8));
+        pval = (float*)(pbuf + 2 * al(e * 8));
+        vtmp = (float*)(pbuf + 2 * al(e * 8) + al(e * 4));
+      } else {
+        char* ctail = up(col + base + cnt);
+        char* vtail = up(val + base + cnt);
+        pkey = (uint64_t*)ctail;
+        ktmp = (uint64_t*)vtail;
+        if (rplace[r] == RP_KEYS) {
+          pval = (float*)pbuf;
+          vtmp = (float*)(pbuf + al(e * 4));
+        } else {
+          pval = (float*)(ctail + al(e * 8));
+          vtmp = (float*)(vtail + al(e * 8));
+        }
+      }
+      ta.skey = pkey;
+      ta.sval = pval;
       uint64_t* kres = pkey;
       float* vres = pval;
       if (cnt > 0) {
