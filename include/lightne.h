@@ -183,7 +183,8 @@ int lightne_table_info(const lightne_table* t, uint64_t* nkeys);
 int lightne_table_copy(lightne_ctx* ctx, const lightne_table* t, uint64_t* keys, uint32_t* counts, double* weights);
 int lightne_table_free(lightne_table* t);
 
-/* Full stage 1: sampler, table, trunc-log, CSR with rows sorted by column.
+/* Full stage 1: sampler, table, trunc-log, CSR (each row's entries in an
+ * order the key set alone fixes: the same on every run).
  * (generate_trunc_log_matrix_v2 with rows_start != NULL, PathEmbed.h:224-316.) */
 int lightne_sparsify(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify_params* p,
                      lightne_csr** out, lightne_sparsify_stats* stats);

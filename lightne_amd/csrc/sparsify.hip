@@ -365,6 +365,143 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   if (side_full) atomicOr(&a.stats[5], 1ull);
 }
 
+// ---------------------------------------------------------------------------
+// Canonical slot order.  Which slots of a linear-probing table are occupied
+// does not depend on the order of the inserts, and neither does which keys a
+// cluster (a maximal run of occupied slots) holds: exactly those whose home
+// slots lie in it.  Which key sits in which slot of its cluster does depend on
+// that order, and concurrent inserts claim slots in a different order on
+// every run.  This pass sorts every cluster by key.  The table is then no
+// longer a probe layout (nothing looks a key up after it: the passes that
+// follow scan the slots), but its slot order depends on the key set alone, so
+// every pass that follows slot order (the trunc-log emission, hence the order
+// of the entries within a CSR row and of every sum over a row) is the same on
+// every run.
+//
+// A block step owns the clusters headed in CN_WIN slots and stages those
+// slots, the one before and CN_LA after them in LDS, with an occupancy bitmap
+// (one ballot per 64 slots).  Every staged occupied slot finds its cluster's
+// head and end from the bitmap (the nearest empty slots before and after it)
+// and, when the head is owned, moves to the head's slot plus its rank (the
+// number of keys of its cluster below its own).  A cluster running past the
+// staged slots raises *err and keeps its order (the caller then orders the
+// CSR rows by column instead).
+// ---------------------------------------------------------------------------
+constexpr int CN_THREADS = 256;
+constexpr int CN_WIN = 1024;                      // owned slots per block step
+constexpr int CN_LA = 512;                        // staged slots past the owned ones
+constexpr int CN_STAGE = 1 + CN_WIN + CN_LA;      // the slot before, the owned ones, the lookahead
+constexpr int CN_BM = (CN_STAGE + 63) / 64;       // occupancy bitmap words
+static_assert(CN_THREADS % 64 == 0, "one bitmap word per wave and 64 staged slots");
+
+struct CanonArgs {
+  Slot* wide;
+  unsigned long long* pk;
+  double* wtab;
+  uint64_t cap;
+  uint32_t* err;
+};
+
+// a staged slot holds the word (packed) or the key (wide)
+template <bool PK>
+__device__ __forceinline__ bool cn_occ(uint64_t w) {
+  return PK ? (w & PK_OCC) != 0 : w != EMPTY_KEY;
+}
+template <bool PK>
+__device__ __forceinline__ uint64_t cn_key(uint64_t w) {
+  return PK ? (w & (PK_OCC - 1)) : w;
+}
+
+__device__ __forceinline__ uint64_t cn_pos(uint64_t i, uint64_t j, uint64_t cap) {
+  const uint64_t p = i + j;
+  return p >= cap ? p - cap : p;
+}
+
+template <bool PK, bool WT>
+__global__ __launch_bounds__(CN_THREADS) void k_table_canon(CanonArgs a) {
+  __shared__ uint64_t s_w[CN_STAGE];
+  __shared__ uint32_t s_c[PK ? 1 : CN_STAGE];
+  __shared__ double s_x[WT ? CN_STAGE : 1];
+  __shared__ uint64_t s_bm[CN_BM];  // bit k % 64 of word k / 64: staged slot k occupied
+  const uint64_t cap = a.cap;
+  const int tid = threadIdx.x;
+  for (uint64_t w0 = (uint64_t)blockIdx.x * CN_WIN; w0 < cap; w0 += (uint64_t)gridDim.x * CN_WIN) {
+    // staged index k is slot st0 + k (modulo cap): the slot before w0, the
+    // owned ones, the lookahead
+    const uint64_t st0 = w0 ? w0 - 1 : cap - 1;
+    for (int k0 = 0; k0 < CN_STAGE; k0 += CN_THREADS) {  // block-uniform: every lane ballots
+      const int k = k0 + tid;
+      bool occ = false;
+      if (k < CN_STAGE) {
+        const uint64_t p = cn_pos(st0, (uint64_t)k % cap, cap);
+        uint64_t w;
+        if (PK) {
+          w = a.pk[p];
+        } else {
+          const Slot sl = a.wide[p];
+          w = sl.key;
+          s_c[k] = sl.cnt;
+        }
+        s_w[k] = w;
+        occ = cn_occ<PK>(w);
+        if (WT) s_x[k] = a.wtab[p];
+      }
+      const uint64_t b = __ballot(occ);
+      if ((tid & 63) == 0 && k < CN_STAGE) s_bm[k >> 6] = b;
+    }
+    __syncthreads();
+    for (int k = 1 + tid; k < CN_STAGE; k += CN_THREADS) {
+      if (!((s_bm[k >> 6] >> (k & 63)) & 1ull)) continue;
+      const uint64_t w = s_w[k];
+      // the cluster's head: after the nearest empty slot before k (h = 0:
+      // every staged slot before k is occupied)
+      int h = 0;
+      for (int wi = k >> 6; wi >= 0; --wi) {
+        uint64_t z = ~s_bm[wi];
+        if (wi == (k >> 6)) z &= (1ull << (k & 63)) - 1;  // the slots before k
+        if (z) {
+          h = 64 * wi + 63 - __clzll((long long)z) + 1;
+          break;
+        }
+      }
+      // owned: headed in [w0, w0 + CN_WIN) (h = 0: headed before w0; past the
+      // table's end: the wrapped start, owned by the first step)
+      if (h == 0 || h > CN_WIN || w0 + (uint64_t)(h - 1) >= cap) continue;
+      // the cluster's end: its first empty slot after k (staged bits past
+      // CN_STAGE read as empty)
+      int e = CN_STAGE;
+      for (int wi = k >> 6; wi < CN_BM; ++wi) {
+        uint64_t z = ~s_bm[wi];
+        if (wi == (k >> 6)) z &= ~((2ull << (k & 63)) - 1);  // the slots after k
+        if (z) {
+          e = 64 * wi + __ffsll((long long)z) - 1;
+          break;
+        }
+      }
+      if (e >= CN_STAGE) {  // runs past the staged slots: left as it is
+        atomicOr(a.err, 1u);
+        continue;
+      }
+      const uint64_t key = cn_key<PK>(w);
+      int rank = 0;
+      for (int r = h; r < e; ++r) rank += cn_key<PK>(s_w[r]) < key ? 1 : 0;
+      if (h + rank == k) continue;
+      const uint64_t q = cn_pos(st0, (uint64_t)(h + rank), cap);
+      if (PK) {
+        a.pk[q] = w;
+      } else {
+        Slot sl;
+        sl.key = w;
+        sl.cnt = s_c[k];
+        sl.pad = 0;
+        a.wide[q] = sl;
+      }
+      if (WT) a.wtab[q] = s_x[k];
+    }
+    __syncthreads();
+  }
+}
+
 // per-block partial (distinct, count sum, weight sum); fixed reduction order
 __global__ __launch_bounds__(256) void k_table_reduce(TabView t, unsigned long long* __restrict__ part_u,
                                                       double* __restrict__ part_w) {
@@ -999,6 +1136,41 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   return LIGHTNE_OK;
 }
 
+// puts every cluster of the table in canonical order (k_table_canon);
+// *canonical = false when a cluster was too long to reorder (the table is a
+// valid one either way)
+int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical) {
+  *canonical = false;
+  auto* err = (uint32_t*)ctx_buf(ctx, "canon_err", sizeof(uint32_t));
+  if (!err) return LIGHTNE_ERR_OOM;
+  LN_HIP(ctx, hipMemsetAsync(err, 0, sizeof(uint32_t), ctx->stream));
+  CanonArgs a;
+  a.wide = const_cast<Slot*>(tv.wide);
+  a.pk = const_cast<unsigned long long*>(tv.pk);
+  a.wtab = const_cast<double*>(tv.wtab);
+  a.cap = tv.cap;
+  a.err = err;
+  const unsigned nb = (unsigned)std::min<uint64_t>((tv.cap + CN_WIN - 1) / CN_WIN, 16384);
+  const hipStream_t s = ctx->stream;
+  if (tv.pk) {
+    if (tv.wtab)
+      hipLaunchKernelGGL((k_table_canon<true, true>), dim3(nb), dim3(CN_THREADS), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_table_canon<true, false>), dim3(nb), dim3(CN_THREADS), 0, s, a);
+  } else {
+    if (tv.wtab)
+      hipLaunchKernelGGL((k_table_canon<false, true>), dim3(nb), dim3(CN_THREADS), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_table_canon<false, false>), dim3(nb), dim3(CN_THREADS), 0, s, a);
+  }
+  LN_LAUNCH(ctx);
+  uint32_t h = 0;
+  LN_HIP(ctx, hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  *canonical = h == 0;
+  return LIGHTNE_OK;
+}
+
 int table_totals(lightne_ctx* ctx, const TabView& tv, uint64_t* distinct, uint64_t* cnt_sum, double* w_sum) {
   const int nb = 2048;
   auto* pu = (unsigned long long*)ctx_buf(ctx, "tred_u", 2 * nb * sizeof(unsigned long long));
@@ -1089,6 +1261,11 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   TabView tv;
   LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
   st.sampler_runs = 1;
+  // canonical slot order: the emission order is then the same on every run,
+  // so the CSR rows need sorting on the row bits only (their entries keep the
+  // emission order); otherwise rows are sorted by column too
+  bool canon = false;
+  LN_CHECK(table_canon(ctx, tv, &canon));
   hipStream_t s = ctx->stream;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
@@ -1330,7 +1507,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     if (kbytes + nz * 4 <= tab_bytes) {
       k_tmp = (uint64_t*)ws;
       v_tmp = (float*)(ws + kbytes);
-    } else if (radix_sort_passes(2 * colbits) % 2 == 1) {
+    } else if (radix_sort_passes(canon ? colbits : 2 * colbits) % 2 == 1) {
       k_tmp = (uint64_t*)ctx_buf(ctx, "rs_keys", nz * 8);
       v_emit = (float*)ctx_buf(ctx, "rs_vals", nz * 4);
       if (!k_tmp || !v_emit) return LIGHTNE_ERR_OOM;
@@ -1355,7 +1532,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     // every summation over a row) deterministic
     uint64_t* kres = nullptr;
     float* vres = nullptr;
-    LN_CHECK(radix_sort_pairs(ctx, skey, v_emit, nnz, 2 * colbits, k_tmp, v_tmp, 0, &kres, &vres));
+    LN_CHECK(radix_sort_pairs(ctx, skey, v_emit, nnz, canon ? colbits : 2 * colbits, k_tmp, v_tmp,
+                              canon ? colbits : 0, &kres, &vres));
     hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(nnz + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, nnz,
                        g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val, (int64_t)0);
     LN_LAUNCH(ctx);
@@ -1557,6 +1735,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
         st.sampler_runs = keep.sampler_runs + 1;
+        LN_CHECK(table_canon(ctx, tv, &canon));
         LN_CHECK(count_pass(true));  // places the cache behind the table iff keep_lw
         if (keep_lw && !(lw && lw_in_ws))
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the value cache did not fit the grown workspace");
@@ -1652,7 +1831,8 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
           hipLaunchKernelGGL(k_trunc_emit<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
                              (uint32_t)r1, (const uint64_t*)blk_off);
         LN_LAUNCH(ctx);
-        LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + colbits, ktmp, vtmp, 0, &kres, &vres));
+        LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + (canon ? 0 : colbits), ktmp, vtmp,
+                                  canon ? colbits : 0, &kres, &vres));
       }
       hipLaunchKernelGGL(k_csr_finish, dim3(grid_blocks(cnt + 1, 256)), dim3(256), 0, s, (const uint64_t*)kres, cnt,
                          (int64_t)(r1 - r0), colbits, rp + r0, col + base, (const float*)vres, val + base, base);

@@ -31,6 +31,14 @@ def c1_sample_ratio(n, m_directed, T=10, per_edge=50.0):
     return float(np.float32(per_edge * m_directed / (T * n * np.log(n))))
 
 
+def sort_rows(rp, col, val):
+    """The CSR with every row's entries in column order (the device keeps
+    them in its table's canonical slot order; the oracle sorts by column)."""
+    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int64), np.diff(rp))
+    o = np.lexsort((col, rows))
+    return rp, col[o], val[o]
+
+
 def principal_angles(A, B):
     """Largest principal angle (rad) between the column spaces of A and B."""
     qa = np.linalg.qr(np.asarray(A, np.float64))[0]

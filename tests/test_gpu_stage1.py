@@ -1,5 +1,6 @@
 """Stage 1 on the device vs the oracle: per-key sample counts bit-exact, CSR
-exact (values within one float ulp)."""
+rows equal as sets (values within one float ulp), and each row's entries in
+the order the canonical table layout fixes."""
 import numpy as np
 import pytest
 
@@ -50,8 +51,10 @@ def test_counts_match_golden_and_oracle(gpu, rec):
 
 
 def _compare_csr(csr, ref, ulps=1):
-    rp, col, val = csr.to_numpy()
-    erp, ecol, evl, _ = ref
+    """The device CSR against a reference, row by row as sets of (column,
+    value): the same columns, values within `ulps`."""
+    rp, col, val = H.sort_rows(*csr.to_numpy())
+    erp, ecol, evl = H.sort_rows(*ref[:3])
     np.testing.assert_array_equal(rp, erp)
     np.testing.assert_array_equal(col, ecol)
     d = np.abs(val.view(np.int32).astype(np.int64) - evl.view(np.int32).astype(np.int64))
@@ -76,6 +79,87 @@ def test_csr_matches_oracle(gpu, rec):
     assert st["kept_upper"] == ref[3]["kept_upper"]
     assert st["nnz"] == csr.nnz == ref[3]["nnz"]
     _compare_csr(csr, ref)
+    csr.free()
+
+
+def _canonical_row_order(keys, kept, cap, nblk=4096, per_thread=4, threads=256):
+    """Host model of the CSR's within-row order: linear probing of `keys` in a
+    table of `cap` slots, each cluster (maximal run of occupied slots) sorted
+    by key, then the
+    trunc-log emission's order over the slots (nblk blocks of per_block
+    slots, each block's waves scanning their own quarter in steps of
+    64 x per_thread slots), the kept keys' entries (u, v) then (v, u), and a
+    stable sort by row.  Returns the entries' (row, col) in CSR order."""
+    h = np.array([(O.hash64_2(int(k)) * cap) >> 64 for k in keys], np.int64)
+    slot = np.full(cap, -1, np.int64)
+    for i, hi in enumerate(h):
+        p = int(hi)
+        while slot[p] >= 0:
+            p = p + 1 if p + 1 < cap else 0
+        slot[p] = i
+    # clusters: maximal runs of occupied slots (the table has an empty slot)
+    occ = slot >= 0
+    start = int(np.flatnonzero(~occ)[0]) + 1
+    order = [(start + j) % cap for j in range(cap)]
+    run = []
+    for p in order + [None]:
+        if p is not None and occ[p]:
+            run.append(p)
+            continue
+        if len(run) > 1:
+            ids = slot[run]
+            slot[run] = ids[np.argsort(keys[ids])]
+        run = []
+    pb = -(-cap // nblk)
+    unit = 4 * per_thread
+    pb = -(-pb // unit) * unit
+    q = pb // 4
+    pos = np.flatnonzero(occ)
+    b, o = pos // pb, pos % pb
+    w, oq = o // q, o % q
+    t, lane, j = oq // (64 * per_thread), (oq % (64 * per_thread)) // per_thread, oq % per_thread
+    pos = pos[np.lexsort((j, lane, w, t, b))]
+    rows, cols = [], []
+    for p in pos:
+        i = slot[p]
+        if not kept[i]:
+            continue
+        u, v = int(keys[i] >> 32), int(keys[i] & 0xFFFFFFFF)
+        rows.append(u)
+        cols.append(v)
+        if u != v:
+            rows.append(v)
+            cols.append(u)
+    rows, cols = np.array(rows, np.int64), np.array(cols, np.int32)
+    o = np.argsort(rows, kind="stable")
+    return rows[o], cols[o]
+
+
+@pytest.mark.parametrize("layout", [1, 2])
+@pytest.mark.parametrize("case", ["cl2000_t10_s0", "cl200_t5_s1"])
+def test_csr_row_order_is_the_canonical_emission_order(gpu, case, layout):
+    """The entries of every CSR row come in the order a host model of the
+    canonical table layout and of the emission predicts, for both slot
+    layouts, so that order (and every FP32 sum over a row) does not depend on
+    the order in which concurrent inserts claimed slots."""
+    rec = _rec(case)
+    off, edg = H.golden_graph(rec["graph"])
+    csr, st = L.generate_trunc_log_matrix(dgraph(off, edg), int(rec["seed"]), walk_len=rec["walk_len"],
+                                          sample=bool(rec["sample"]), sample_ratio=rec["sample_ratio"],
+                                          table_layout=layout, ctx=gpu)
+    assert st["table_layout"] == layout
+    rp, col, _ = csr.to_numpy()
+    keys, _, wgh, _ = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
+                               threads=1)
+    ref = O.trunc_log_csr(off, keys, wgh, M_exact=st["M_exact"]) if rec["sample"] else \
+        O.trunc_log_csr(off, keys, wgh)
+    erp, ecol = ref[0], ref[1]
+    erows = np.repeat(np.arange(len(erp) - 1, dtype=np.int64), np.diff(erp))
+    kept_pairs = set(zip(erows[erows <= ecol].tolist(), ecol[erows <= ecol].tolist()))
+    kept = np.array([(int(k >> 32), int(k & 0xFFFFFFFF)) in kept_pairs for k in keys])
+    rows, cols = _canonical_row_order(keys.astype(np.uint64), kept, int(st["table_capacity"]))
+    np.testing.assert_array_equal(rp, erp)
+    np.testing.assert_array_equal(col, cols)
     csr.free()
 
 

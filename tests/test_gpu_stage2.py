@@ -239,8 +239,10 @@ def test_rsvd_sparse_projection_c1(gpu):
 def test_upper_storage_is_the_same_matrix(gpu):
     """-upper 1 (PathEmbed.h:284-309, mklredsvd.cc:95-98): upper-triangle storage
     denotes U + U^T - diag(U).  The randomized SVD of the upper CSR equals the
-    one of the full CSR bit for bit, and NetSMF gives the same embedding with the
-    upper storage's nnz reported."""
+    one of the full CSR up to the order of the FP32 sums over a row (the
+    expansion orders each row by column, the full build keeps the canonical
+    table order), and NetSMF gives the same embedding with the upper storage's
+    nnz reported."""
     hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
     g = hg.to_device()
@@ -250,11 +252,12 @@ def test_upper_storage_is_the_same_matrix(gpu):
     assert sf["nnz"] == 2 * su["nnz"] - _diag_count(up)
     a = L.MKLRedSVD(up, 144, upper=True, seed=3, ctx=gpu).run()
     b = L.MKLRedSVD(full, 144, seed=3, ctx=gpu).run()
-    np.testing.assert_array_equal(a.S, b.S)
-    np.testing.assert_array_equal(a.matU.cpu().numpy(), b.matU.cpu().numpy())
+    np.testing.assert_allclose(a.S, b.S, rtol=1e-5)
+    assert H.principal_angles(a.matU.cpu().numpy()[:, :128], b.matU.cpu().numpy()[:, :128]) < 1e-4
     e1, S1, st1 = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, upper=True, ctx=gpu)
     e0, S0, st0 = L.NetSMF(g, walk_len=10, rank=144, dim=128, sample_ratio=ratio, seed=1, ctx=gpu)
-    np.testing.assert_array_equal(e1.cpu().numpy(), e0.cpu().numpy())
+    np.testing.assert_allclose(S1, S0, rtol=1e-5)
+    assert H.principal_angles(e1.cpu().numpy(), e0.cpu().numpy()) < 1e-4
     assert st1["sparsify"]["nnz"] == su["nnz"]
 
 
@@ -266,8 +269,9 @@ def _diag_count(csr):
 
 def test_netsmf_is_deterministic(gpu):
     """Two runs with the same seeds give the same embedding bit for bit (the
-    table's slot order varies with the order of concurrent inserts; rows sorted
-    by column and fixed-order reductions keep every sum in one order)."""
+    table's slot order varies with the order of concurrent inserts; the pass
+    that puts every cluster of the table in canonical order, and fixed-order
+    reductions, keep every sum in one order)."""
     hg = L.chung_lu(10312, 2.1, 65.0, 3900.0, 1)
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 50.0)
     g = hg.to_device()
