@@ -241,12 +241,14 @@ struct SampleArgs {
 // neighbour i - 1 unless i is a multiple of 64: the result of
 // get_ith_neighbor on such graphs (ligra/encodings/byte_pd_amortized.h:653-688
 // decodes i - start - 1 differences after the block's first edge).
+// The start vertex's row (o0, d0) comes from the caller, which reads it once
+// for all of an edge's samples (every walk from u or v starts there).
 template <typename OffT>
 __device__ __forceinline__ uint32_t walk(const OffT* __restrict__ off, const uint32_t* __restrict__ adj, uint32_t x,
-                                         uint64_t len, uint64_t& s, bool& bad, uint64_t ith) {
+                                         uint64_t len, uint64_t& s, bool& bad, uint64_t ith, OffT o0, uint64_t d0) {
   for (uint64_t t = 0; t < len; ++t) {
-    const OffT o = off[x];
-    const uint64_t d = (uint64_t)(off[x + 1] - o);
+    const OffT o = t ? off[x] : o0;
+    const uint64_t d = t ? (uint64_t)(off[x + 1] - o) : d0;
     if (d == 0) {
       bad = true;
       break;
@@ -331,9 +333,12 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     }
     n_attempts = e;
     double p = 1.0, ipw = 1.0;
+    // the rows of u and v, where every walk of this edge's samples starts
+    const auto ou = off[u], ov = off[v];
+    const uint64_t du_ = (uint64_t)(off[u + 1] - ou), dv_ = (uint64_t)(off[v + 1] - ov);
     if (MODE == 1) {
-      const double du = (double)(off[u + 1] - off[u]);
-      const double dv = (double)(off[v + 1] - off[v]);
+      const double du = (double)du_;
+      const double dv = (double)dv_;
       p = (1.0 / du + 1.0 / dv) * a.logn;
       p = p < 1.0 ? p : 1.0;
       ipw = 1.0 / p;
@@ -350,8 +355,8 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       rr = hash64_2(s);
       const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
       s = hash64_2(rr);
-      const uint32_t x = walk(off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith);
-      const uint32_t z = walk(off, a.adj, v, r - kk, s, bad, a.ith);
+      const uint32_t x = walk(off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith, ou, du_);
+      const uint32_t z = walk(off, a.adj, v, r - kk, s, bad, a.ith, ov, dv_);
       n_steps += r - 1;
       const int rc = table_add<MODE, PACKED>(a, min(x, z), max(x, z), ipw);
       if (rc == 0) overflow = true;
@@ -388,7 +393,7 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
 // CSR rows by column instead).
 // ---------------------------------------------------------------------------
 constexpr int CN_THREADS = 256;
-constexpr int CN_WIN = 1024;                      // owned slots per block step
+constexpr int CN_WIN = 2048;                      // owned slots per block step
 constexpr int CN_LA = 512;                        // staged slots past the owned ones
 constexpr int CN_STAGE = 1 + CN_WIN + CN_LA;      // the slot before, the owned ones, the lookahead
 constexpr int CN_BM = (CN_STAGE + 63) / 64;       // occupancy bitmap words
