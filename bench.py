@@ -263,6 +263,12 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
         del Y, Q, Z, Cm
     s2n, s2r = nnz / max(nnz2, 1), n / max(rr, 1)
     t_svd = t_sp * s2n + t_rows * s2r + t_small
+    # what ran, unscaled: seconds of each timed piece and the work it covered
+    measured = {"sampler": {"s": round(t_s1, 3), "directed_edges": e_done, "samples": int(st["samples"])},
+                "trunc_log_csr": {"s": round(t_csr, 3), "directed_edges": e_done},
+                "rsvd_sparse_products": {"s": round(t_sp, 3), "nnz": nnz2, "products": 1 + q},
+                "rsvd_dense": {"s": round(t_rows, 3), "rows": rr},
+                "rsvd_small_svd": {"s": round(t_small, 3)}}
     if not full:
         note.append(f"rSVD sparse products on {nnz2}/{nnz} nnz ({t_sp:.2f} s, x{s2n:.2f}), QR/GEMM on {rr}/{n} rows "
                     f"({t_rows:.2f} s, x{s2r:.2f}), k x k SVD once")
@@ -308,6 +314,9 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
             K.gesvd_left(ar.copy())
             t_tall = time.perf_counter() - t0
             s3n, s3r = nnz3_all / max(nnz3, 1), n / max(rr, 1)
+            measured["propagation_one_product"] = {"s": round(t_p, 3), "operator_nnz": nnz3}
+            measured["propagation_updates"] = {"s": round(t_u, 3), "rows": rr}
+            measured["tall_svd"] = {"s": round(t_tall, 3), "rows": rr}
             t_prop = t_p * s3n * (2 * cfg["K"] - 1) + t_u * s3r
             t_tall *= s3r
             note.append(f"propagation: 1 of {2 * cfg['K'] - 1} products on {nnz3}/{nnz3_all} operator nnz "
@@ -346,6 +355,8 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
         np.linalg.svd(z.astype(np.float32), full_matrices=False)
         t_tall = time.perf_counter() - t0
         note.append("propagation with the oracle's C kernels (MKL absent), full graph")
+        measured["propagation"] = {"s": round(t_prop, 3)}
+        measured["tall_svd"] = {"s": round(t_tall, 3), "rows": n}
     del Om
     log(f"cpu: propagation {t_prop:.2f}s tall svd {t_tall:.2f}s")
     t_sample, t_csr_full = t_s1 * s1, t_csr * s1
@@ -354,9 +365,12 @@ def cpu_baseline(hg, cfg, ratio, csr_parts):
             f"GNU threads)" if use_mkl else "stage 1: oracle C/OpenMP; stages 2-3: oracle C kernels + numpy LAPACK")
     sample = (f"whole workload, unscaled, {threads} threads; {libs}" if full else
               f"ESTIMATE from bounded samples of the same workload, {threads} threads; {libs}; " + "; ".join(note))
+    if full and use_mkl:
+        measured["propagation"] = {"s": round(t_prop, 3)}
+        measured["tall_svd"] = {"s": round(t_tall, 3), "rows": n}
     return {
         "value": round(total, 3), "unit": "s", "cores": threads, "kind": "port", "estimate": not full,
-        "sample": sample, "mkl": use_mkl,
+        "sample": sample, "mkl": use_mkl, "measured_unscaled": measured,
         "sampled_edges_per_s": st["samples"] / t_s1,
         "stages_s": {"sample": round(t_sample, 3), "csr": round(t_csr_full, 3), "svd": round(t_svd, 3),
                      "propagation": round(t_prop + t_tall, 3)},
