@@ -472,7 +472,8 @@ def main():
     slot_b = 8 if sp["table_layout"] == 2 else 16
     sampler_bytes = 20 * sp["walk_steps"] + 2 * slot_b * sp["samples"] + 12 * hg.m
     sampler_gbs = sampler_bytes / sp["t_sample"] / 1e9
-    stages = {"sample": sp["t_sample"], "table": sp["t_table"], "csr": sp["t_csr"], "sparsify_wall": st["t_sparsify"],
+    stages = {"sample": sp["t_sample"], "table": sp["t_table"], "canon": sp["t_canon"], "csr": sp["t_csr"],
+              "sparsify_wall": st["t_sparsify"],
               "rsvd_spmm": rs["t_spmm"], "rsvd_qr": rs["t_qr"], "rsvd_gemm": rs["t_gemm"],
               "rsvd_small_svd": rs["t_small_svd"], "rsvd_total": rs["t_total"], "netsmf_total": st["t_total"],
               "prop_cheb": pst["t_cheb"], "prop_aggregate": pst["t_aggregate"], "prop_svd": pst["t_svd"],
@@ -644,7 +645,7 @@ def main():
                         "ms": round(1e3 * rs["t_spmm_sketch"], 3), "bytes": int(sketch_bytes),
                         "frac": round(sketch_gbs / HBM_PEAK_GBS, 4) if sketch_gbs else None},
         "sparsifier_nnz": nnz, "samples": sp["samples"], "distinct_keys": sp["distinct_keys"],
-        "table_capacity": sp["table_capacity"], "table_slot_bytes": slot_b, "csr_row_ranges": sp["csr_parts"],
+        "table_capacity": sp["table_capacity"], "table_slot_bytes": slot_b, "csr_row_ranges": sp["csr_parts"], "csr_row_order": int(sp["row_order"]),
         "csr_value_cache": int(sp["value_cache"]), "sampler_runs": sp["sampler_runs"], "stages_s": {k_: round(v, 5) for k_, v in stages.items()},
         "bf16": bf16, "fp64_reference": fp64, "adjoint": adjoint, "save": save, "roofline": roofline,
         "cpu_baseline": cpu,

@@ -167,6 +167,13 @@ typedef struct {
                               (the rest read the degrees); 0: no cache     */
   uint32_t sampler_runs;   /* 1, or 2 when the workspace grew after the count
                               pass and the sampler ran again                */
+  double t_canon;          /* device seconds of the pass that sorts every
+                              table cluster by key                         */
+  uint32_t row_order;      /* order of each CSR row's entries: 1 fixed by
+                              the key set (canonical table order), 2 by
+                              column (a table cluster was too long to
+                              reorder); the same on every run either way   */
+  uint32_t reserved_;
 } lightne_sparsify_stats;
 
 /* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */

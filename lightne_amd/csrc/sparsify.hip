@@ -1144,11 +1144,12 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
 // puts every cluster of the table in canonical order (k_table_canon);
 // *canonical = false when a cluster was too long to reorder (the table is a
 // valid one either way)
-int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical) {
+int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical, double* seconds) {
   *canonical = false;
   auto* err = (uint32_t*)ctx_buf(ctx, "canon_err", sizeof(uint32_t));
   if (!err) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemsetAsync(err, 0, sizeof(uint32_t), ctx->stream));
+  LN_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
   CanonArgs a;
   a.wide = const_cast<Slot*>(tv.wide);
   a.pk = const_cast<unsigned long long*>(tv.pk);
@@ -1169,10 +1170,12 @@ int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical) {
       hipLaunchKernelGGL((k_table_canon<false, false>), dim3(nb), dim3(CN_THREADS), 0, s, a);
   }
   LN_LAUNCH(ctx);
+  LN_HIP(ctx, hipEventRecord(ctx->ev[7], s));
   uint32_t h = 0;
   LN_HIP(ctx, hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, s));
   LN_HIP(ctx, hipStreamSynchronize(s));
   *canonical = h == 0;
+  *seconds = ev_seconds(ctx->ev[6], ctx->ev[7]);
   return LIGHTNE_OK;
 }
 
@@ -1270,7 +1273,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   // so the CSR rows need sorting on the row bits only (their entries keep the
   // emission order); otherwise rows are sorted by column too
   bool canon = false;
-  LN_CHECK(table_canon(ctx, tv, &canon));
+  LN_CHECK(table_canon(ctx, tv, &canon, &st.t_canon));
   hipStream_t s = ctx->stream;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
@@ -1740,7 +1743,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
         const lightne_sparsify_stats keep = st;
         LN_CHECK(run_sampler(ctx, g, p, &st, &tv));
         st.sampler_runs = keep.sampler_runs + 1;
-        LN_CHECK(table_canon(ctx, tv, &canon));
+        LN_CHECK(table_canon(ctx, tv, &canon, &st.t_canon));
         LN_CHECK(count_pass(true));  // places the cache behind the table iff keep_lw
         if (keep_lw && !(lw && lw_in_ws))
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "stage 1: the value cache did not fit the grown workspace");
@@ -1857,6 +1860,7 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   st.t_table = ev_seconds(ctx->ev[3], ctx->ev[4]);
   st.t_csr = ev_seconds(ctx->ev[4], ctx->ev[5]);
   *out = csr;
+  st.row_order = canon ? 1u : 2u;
   if (stats) *stats = st;
   return LIGHTNE_OK;
 }

@@ -147,7 +147,7 @@ def test_csr_row_order_is_the_canonical_emission_order(gpu, case, layout):
     csr, st = L.generate_trunc_log_matrix(dgraph(off, edg), int(rec["seed"]), walk_len=rec["walk_len"],
                                           sample=bool(rec["sample"]), sample_ratio=rec["sample_ratio"],
                                           table_layout=layout, ctx=gpu)
-    assert st["table_layout"] == layout
+    assert st["table_layout"] == layout and st["row_order"] == 1
     rp, col, _ = csr.to_numpy()
     keys, _, wgh, _ = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
                                threads=1)
@@ -161,6 +161,33 @@ def test_csr_row_order_is_the_canonical_emission_order(gpu, case, layout):
     np.testing.assert_array_equal(rp, erp)
     np.testing.assert_array_equal(col, cols)
     csr.free()
+
+
+def test_crowded_table_rows_fall_back_to_column_order(gpu):
+    """A table capped near full (load 0.97) has clusters longer than the
+    canonical pass stages: the rows are then sorted by column (row_order 2),
+    equal the oracle's row by row, and a second run gives the same CSR."""
+    rec = _rec("cl2000_t10_s0")
+    off, edg = H.golden_graph(rec["graph"])
+    keys, _, wgh, _ = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
+                               threads=1)
+    cap = (int(len(keys) / 0.97) + 255) // 256 * 256
+    g = dgraph(off, edg)
+    kw = dict(walk_len=rec["walk_len"], sample_ratio=rec["sample_ratio"], max_table_bytes=cap * 8, table_layout=2,
+              ctx=gpu)
+    a, st = L.generate_trunc_log_matrix(g, int(rec["seed"]), **kw)
+    assert st["table_capacity"] == cap and st["distinct_keys"] == len(keys)
+    assert st["row_order"] == 2
+    _compare_csr(a, O.trunc_log_csr(off, keys, wgh))
+    rp, col, val = a.to_numpy()
+    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int64), np.diff(rp))
+    assert np.all((np.diff(col) > 0) | (np.diff(rows) != 0))  # each row by column
+    b, st2 = L.generate_trunc_log_matrix(g, int(rec["seed"]), **kw)
+    assert st2["row_order"] == 2
+    for x, y in zip((rp, col, val), b.to_numpy()):
+        np.testing.assert_array_equal(x, y)
+    a.free()
+    b.free()
 
 
 def _c1_graph():
