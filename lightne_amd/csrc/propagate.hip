@@ -154,6 +154,11 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
     hipLaunchKernelGGL(k_y_from_conv, dim3(grid_blocks((int64_t)pe, 256)), dim3(256), 0, s, a, conv, yout, (int64_t)pe);
     LN_LAUNCH(ctx);
   }
+  // FP32 iterates: every other step leaves its c_i T_i to the next one, which
+  // reads T_i as its Lx1 anyway (one read and one write of conv fewer per
+  // pair of steps, the same sums in the same order).  BF16 iterates keep the
+  // update per step: the next step reads T_i rounded to BF16.
+  int64_t pend = -1;
   for (int64_t i = 2; i < K; ++i) {
     // the BF16 form's second FP32 step: T_2 from the FP32 T_1 and a
     const bool f32_step = bf && i == 2;
@@ -175,6 +180,16 @@ int propagate_forward(lightne_ctx* ctx, const lightne_graph* g, const float* a, 
     t.c1 = c[i];
     t.lx1 = lx1;
     t.lx0 = lx0;
+    if (!bf) {
+      if (pend >= 0) {
+        t.conv_mode = 2;
+        t.c_prev = c[pend];
+        pend = -1;
+      } else if (i + 1 < K) {
+        t.conv_mode = 1;
+        pend = i;
+      }
+    }
     if (i + 1 < K) {
       t.lx_out = lx2;
       LN_CHECK(spmm_run(ctx, P, t, xti, EPI_CHEB, true));

@@ -45,6 +45,12 @@ struct SpmmArgs {
   const void* X = nullptr;  // gathered panel
   void* out = nullptr;
   float alpha = 1.f, c0 = 0.f, c1 = 0.f;
+  // EPI_CHEB / EPI_CHEB_LAST accumulator update: 0 adds c1 Lx2 to conv; 1
+  // leaves conv alone (the next step adds this step's term from its Lx1); 2
+  // adds c_prev Lx1, then c1 Lx2 (the previous step's deferred term first, so
+  // the sum is the one the step-by-step update gives)
+  int conv_mode = 0;
+  float c_prev = 0.f;
   const float* a_in = nullptr;
   const void* lx1 = nullptr;
   const void* lx0 = nullptr;
@@ -161,15 +167,17 @@ __device__ __forceinline__ void epilogue(const SpmmArgs& a, int64_t r, int c, fl
     float l1[4], l0[4], cv[4] = {0.f, 0.f, 0.f, 0.f};
     load4<XT>(reinterpret_cast<const XT*>(a.lx1) + o, l1);
     load4<XT>(reinterpret_cast<const XT*>(a.lx0) + o, l0);
-    if (a.conv) load4<float>(a.conv + o, cv);
+    const bool upd = a.conv && a.conv_mode != 1;
+    if (upd) load4<float>(a.conv + o, cv);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       v[i] = (v[i] - l0[i]) + (-2.0f) * l1[i];
+      if (a.conv_mode == 2) cv[i] = a.c_prev * l1[i] + cv[i];
       cv[i] = a.c1 * v[i] + cv[i];
     }
     if (EPI == EPI_CHEB) {
       store4<XT>(reinterpret_cast<XT*>(a.lx_out) + o, v);
-      if (a.conv) store4<float>(a.conv + o, cv);  // conv == NULL: plain recurrence
+      if (upd) store4<float>(a.conv + o, cv);  // conv == NULL: plain recurrence
     } else {
       float av[4];
       load4<float>(a.a_in + o, av);

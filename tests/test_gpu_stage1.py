@@ -233,7 +233,8 @@ def test_sparse_graph_csr_sorted_beside_the_table(gpu):
 
 
 def test_c2_counts_bit_exact(gpu):
-    """Config 2: Chung-Lu n=1.14M, m=2.99M, T=10, M = 20 m (10 per directed edge)."""
+    """Config 2: Chung-Lu n=1.14M, m=2.99M, T=10, M = 20 m (10 per directed edge):
+    per-key counts bit-exact and the CSR equal to the oracle's row by row."""
     hg = L.chung_lu(1_140_000, 2.0, 5.25, 2500.0, 2)
     ratio = H.c1_sample_ratio(hg.n, hg.m, 10, 10.0)
     g = hg.to_device()
@@ -243,6 +244,23 @@ def test_c2_counts_bit_exact(gpu):
     np.testing.assert_array_equal(t.counts.astype(np.uint64), cnt)
     assert t.stats["samples"] == st["samples"] and t.stats["walk_steps"] == st["walk_steps"]
     print(f"C2: samples={st['samples']} keys={len(keys)}")
+    # the C2 sparsifier against the oracle's trunc-log CSR, row by row as sets
+    # (rows sorted by (row, col) on the device: 9.7e7 entries)
+    import torch
+
+    csr, s2 = L.generate_trunc_log_matrix(g, 20240601, walk_len=10, sample_ratio=ratio, ctx=gpu)
+    erp, ecol, evl, info = O.trunc_log_csr(hg.offsets, keys, wgh)
+    assert s2["factor"] == info["factor"] and s2["nnz"] == info["nnz"] == csr.nnz
+    rp, col, val = csr.to_numpy()
+    np.testing.assert_array_equal(rp, erp)
+    rows = torch.repeat_interleave(torch.arange(hg.n, device="cuda"), torch.from_numpy(np.diff(rp)).cuda())
+    order = torch.sort(rows * hg.n + torch.from_numpy(col).cuda().long(), stable=True).indices
+    dcol = torch.from_numpy(col).cuda()[order]
+    dval = torch.from_numpy(val.view(np.int32)).cuda()[order].long()
+    assert torch.equal(dcol, torch.from_numpy(ecol).cuda())
+    ulps = (dval - torch.from_numpy(evl.view(np.int32)).cuda().long()).abs().max().item()
+    print(f"C2 CSR: nnz={csr.nnz} row order {s2['row_order']}, max value difference {ulps} ulp")
+    assert ulps <= 1
 
 
 def test_table_full_is_an_error_and_context_survives(gpu):
