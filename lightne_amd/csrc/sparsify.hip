@@ -2,7 +2,8 @@
 // device.
 //
 //  k_edge_src      : source vertex of every directed edge (binary search in
-//                    the offsets), so the sampler can run one lane per edge.
+//                    the offsets staged per tile of edges), so the sampler
+//                    can run one lane per edge.
 //  k_sample<MODE, PACKED> : one lane per directed edge (u, v); its
 //                    pbbs::random chain is seeded by fork((u << 32) | v) and
 //                    drives exp_rv path samples exactly as PathEmbed.h:128-171;
@@ -174,18 +175,59 @@ __global__ void k_off32(const int64_t* __restrict__ off, int64_t count, uint32_t
     o32[i] = (uint32_t)off[i];
 }
 
-__global__ void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m, uint32_t* __restrict__ src) {
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
-    // largest u with off[u] <= j
-    int64_t lo = 0, hi = n - 1;
-    while (lo < hi) {
-      int64_t mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= j)
-        lo = mid;
-      else
-        hi = mid - 1;
+// largest u in [lo, hi] with off[u] <= j, given off[lo] <= j
+__device__ __forceinline__ int64_t edge_row(const int64_t* __restrict__ off, int64_t lo, int64_t hi, int64_t j) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= j)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Each block takes ES_TILE consecutive edges: two lanes find the rows of its
+// first and last edge in the offsets, and when those rows number at most
+// ES_ROWS their offsets are staged in LDS and every edge searches there
+// (a tile spans about ES_TILE / mean degree rows).  A tile over a longer run
+// of rows searches the offsets in memory between the two rows.
+constexpr int ES_THREADS = 256, ES_PER = 16, ES_TILE = ES_THREADS * ES_PER, ES_ROWS = 2048;
+
+__global__ __launch_bounds__(ES_THREADS) void k_edge_src(const int64_t* __restrict__ off, int64_t n, int64_t m,
+                                                         uint32_t* __restrict__ src) {
+  __shared__ int64_t s_off[ES_ROWS];
+  __shared__ int64_t s_u[2];
+  for (int64_t j0 = (int64_t)blockIdx.x * ES_TILE; j0 < m; j0 += (int64_t)gridDim.x * ES_TILE) {
+    const int64_t j1 = j0 + ES_TILE < m ? j0 + ES_TILE : m;
+    if (threadIdx.x < 2) s_u[threadIdx.x] = edge_row(off, 0, n - 1, threadIdx.x ? j1 - 1 : j0);
+    __syncthreads();
+    const int64_t u0 = s_u[0], u1 = s_u[1], rows = u1 - u0 + 1;
+    if (rows <= ES_ROWS) {
+      for (int64_t i = threadIdx.x; i < rows; i += ES_THREADS) s_off[i] = off[u0 + i];
+      __syncthreads();
+#pragma unroll 4
+      for (int e = 0; e < ES_PER; ++e) {
+        const int64_t j = j0 + e * ES_THREADS + threadIdx.x;
+        if (j < j1) {
+          int lo = 0, hi = (int)rows - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= j)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          src[j] = (uint32_t)(u0 + lo);
+        }
+      }
+    } else {
+      for (int e = 0; e < ES_PER; ++e) {
+        const int64_t j = j0 + e * ES_THREADS + threadIdx.x;
+        if (j < j1) src[j] = (uint32_t)edge_row(off, u0, u1, j);
+      }
     }
-    src[j] = (uint32_t)lo;
+    __syncthreads();  // the next tile rewrites s_u and s_off
   }
 }
 
@@ -1071,8 +1113,8 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   if (wtab) LN_HIP(ctx, hipMemsetAsync(wtab, 0, cap * sizeof(double), s));
   LN_HIP(ctx, hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
   if (g->m > 0) {
-    hipLaunchKernelGGL(k_edge_src, dim3(grid_blocks(g->m, 256, (int64_t)1 << 22)), dim3(256), 0, s, g->offsets,
-                       g->n, g->m, src);
+    hipLaunchKernelGGL(k_edge_src, dim3(grid_blocks(g->m, ES_TILE, (int64_t)1 << 16)), dim3(ES_THREADS), 0, s,
+                       g->offsets, g->n, g->m, src);
     LN_LAUNCH(ctx);
     if (!off64) {
       hipLaunchKernelGGL(k_off32, dim3(grid_blocks(g->n + 1, 256)), dim3(256), 0, s, g->offsets, g->n + 1, off32);
