@@ -413,6 +413,28 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Partition of the table shared by the canonical-order pass and the trunc-log
+// passes: trunc_blocks(cap) blocks of trunc_per_block slots each (a multiple
+// of TR_WAVES * TR_ITEMS).  Wave w of a block scans its own contiguous quarter
+// of the block's slots (wave_span), so every thread's TR_ITEMS slots start
+// 32-byte aligned in a table of 8-byte slots, and a wave numbers the occupied
+// slots of its quarter in slot order by itself (the value cache's index).
+// ---------------------------------------------------------------------------
+constexpr int TR_THREADS = 256;
+constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in flight
+constexpr int TR_WAVES = TR_THREADS / 64;
+
+// enough blocks that the last round of resident blocks leaves little of the
+// device idle on large tables (about seven 256-thread blocks fit a CU)
+inline int trunc_blocks(uint64_t cap) { return cap >= (1ull << 30) ? 16384 : 4096; }
+
+inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
+  constexpr uint64_t U = (uint64_t)TR_WAVES * TR_ITEMS;
+  const uint64_t pb = (cap + nblk - 1) / nblk;
+  return (pb + U - 1) / U * U;
+}
+
+// ---------------------------------------------------------------------------
 // Canonical slot order.  Which slots of a linear-probing table are occupied
 // does not depend on the order of the inserts, and neither does which keys a
 // cluster (a maximal run of occupied slots) holds: exactly those whose home
@@ -425,14 +447,20 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
 // of the entries within a CSR row and of every sum over a row) is the same on
 // every run.
 //
-// A block step owns the clusters headed in CN_WIN slots and stages those
-// slots, the one before and CN_LA after them in LDS, with an occupancy bitmap
-// (one ballot per 64 slots).  Every staged occupied slot finds its cluster's
-// head and end from the bitmap (the nearest empty slots before and after it)
-// and, when the head is owned, moves to the head's slot plus its rank (the
-// number of keys of its cluster below its own).  A cluster running past the
-// staged slots raises *err and keeps its order (the caller then orders the
-// CSR rows by column instead).
+// The slots of every trunc-log block are cut into windows of CN_WIN owned
+// slots (the block's last one shorter), taken by the grid in slot order.  A
+// window stages its owned slots, the one before and CN_LA after them
+// in LDS, with an occupancy bitmap (one ballot per 64 slots).  Every staged
+// occupied slot finds its cluster's head and end from the bitmap (the nearest
+// empty slots before and after it) and, when the head is owned, moves to the
+// head's slot plus its rank (the number of keys of its cluster below its
+// own).  A cluster running past the staged slots raises *err and keeps its
+// order (the caller then orders the CSR rows by column instead).
+//
+// The pass also records per window what the trunc-log passes need first:
+// the occupied owned slots in each wave quarter of its trunc-log block (a
+// slot's occupancy does not change when its cluster is sorted), summed per
+// block by k_canon_fold.
 // ---------------------------------------------------------------------------
 constexpr int CN_THREADS = 256;
 constexpr int CN_WIN = 2048;                      // owned slots per block step
@@ -440,13 +468,24 @@ constexpr int CN_LA = 512;                        // staged slots past the owned
 constexpr int CN_STAGE = 1 + CN_WIN + CN_LA;      // the slot before, the owned ones, the lookahead
 constexpr int CN_BM = (CN_STAGE + 63) / 64;       // occupancy bitmap words
 static_assert(CN_THREADS % 64 == 0, "one bitmap word per wave and 64 staged slots");
+static_assert(CN_WIN % 64 == 0, "owned slots in whole bitmap words");
+
+// occupied owned slots of a window per wave quarter of its trunc-log block
+struct CanonWin {
+  uint32_t occ[TR_WAVES];
+};
 
 struct CanonArgs {
   Slot* wide;
   unsigned long long* pk;
+  const unsigned long long* ovf;  // side table of the packed layout
   double* wtab;
   uint64_t cap;
+  uint64_t per_block;             // slots per trunc-log block
+  uint64_t wpb;                   // windows per trunc-log block
+  uint64_t nwin;                  // windows: trunc-log blocks x wpb
   uint32_t* err;
+  CanonWin* win;                  // [nwin]
 };
 
 // a staged slot holds the word (packed) or the key (wide)
@@ -458,94 +497,166 @@ template <bool PK>
 __device__ __forceinline__ uint64_t cn_key(uint64_t w) {
   return PK ? (w & (PK_OCC - 1)) : w;
 }
-
 __device__ __forceinline__ uint64_t cn_pos(uint64_t i, uint64_t j, uint64_t cap) {
   const uint64_t p = i + j;
   return p >= cap ? p - cap : p;
 }
 
 template <bool PK, bool WT>
-__global__ __launch_bounds__(CN_THREADS) void k_table_canon(CanonArgs a) {
+__global__ __launch_bounds__(CN_THREADS, 7) void k_table_canon(CanonArgs a) {
   __shared__ uint64_t s_w[CN_STAGE];
   __shared__ uint32_t s_c[PK ? 1 : CN_STAGE];
   __shared__ double s_x[WT ? CN_STAGE : 1];
   __shared__ uint64_t s_bm[CN_BM];  // bit k % 64 of word k / 64: staged slot k occupied
+  __shared__ uint32_t s_occ[TR_WAVES];
   const uint64_t cap = a.cap;
   const int tid = threadIdx.x;
-  for (uint64_t w0 = (uint64_t)blockIdx.x * CN_WIN; w0 < cap; w0 += (uint64_t)gridDim.x * CN_WIN) {
-    // staged index k is slot st0 + k (modulo cap): the slot before w0, the
-    // owned ones, the lookahead
-    const uint64_t st0 = w0 ? w0 - 1 : cap - 1;
-    for (int k0 = 0; k0 < CN_STAGE; k0 += CN_THREADS) {  // block-uniform: every lane ballots
-      const int k = k0 + tid;
-      bool occ = false;
-      if (k < CN_STAGE) {
-        const uint64_t p = cn_pos(st0, (uint64_t)k % cap, cap);
-        uint64_t w;
+  const uint64_t q = a.per_block / TR_WAVES;  // slots per wave quarter
+  if (tid < TR_WAVES) s_occ[tid] = 0;
+  for (uint64_t t = blockIdx.x; t < a.nwin; t += gridDim.x) {
+    // window t: the (t % wpb)-th of trunc-log block t / wpb
+    const uint64_t b0 = (t / a.wpb) * a.per_block;
+    const uint64_t b1 = min(cap, b0 + a.per_block);
+    const uint64_t w0 = b0 + (t % a.wpb) * CN_WIN;
+    // owned: slots w0 .. w0 + own - 1 (none past the table's end)
+    const int own = w0 < b1 ? (int)min((uint64_t)CN_WIN, b1 - w0) : 0;
+    if (own > 0) {  // block-uniform
+      // staged index k is slot st0 + k (modulo cap): the slot before w0, the
+      // owned ones, the lookahead
+      const uint64_t st0 = w0 ? w0 - 1 : cap - 1;
+      for (int k0 = 0; k0 < CN_STAGE; k0 += CN_THREADS) {  // block-uniform: every lane ballots
+        const int k = k0 + tid;
+        bool occ = false;
+        if (k < CN_STAGE) {
+          const uint64_t p = cn_pos(st0, (uint64_t)k % cap, cap);
+          uint64_t w;
+          if (PK) {
+            w = a.pk[p];
+          } else {
+            const Slot sl = a.wide[p];
+            w = sl.key;
+            s_c[k] = sl.cnt;
+          }
+          s_w[k] = w;
+          occ = cn_occ<PK>(w);
+          if (WT) s_x[k] = a.wtab[p];
+        }
+        const uint64_t b = __ballot(occ);
+        if ((tid & 63) == 0 && k < CN_STAGE) s_bm[k >> 6] = b;
+      }
+      __syncthreads();
+      // occupied owned slots per wave quarter: word i covers staged 1 + 64 i
+      // .. 64 + 64 i, slots w0 + 64 i .. (a quarter holds at least TR_ITEMS
+      // slots)
+      for (int i = tid; 64 * i < own; i += CN_THREADS) {
+        uint64_t bits = (s_bm[i] >> 1) | (s_bm[i + 1] << 63);
+        const int len = min(64, own - 64 * i);
+        if (len < 64) bits &= (1ull << len) - 1;
+        uint64_t off = w0 + 64 * (uint64_t)i - b0;  // offset of bit j in the trunc-log block
+        int j = 0;
+        while (j < len) {
+          int wv = 0;  // the quarter of offset off (< per_block = TR_WAVES q)
+          while (wv + 1 < TR_WAVES && off >= (uint64_t)(wv + 1) * q) ++wv;
+          const uint64_t left = (uint64_t)(wv + 1) * q - off;  // >= 1 slot left in the quarter
+          const int jn = left < (uint64_t)(len - j) ? j + (int)left : len;
+          const uint64_t m = (jn - j == 64) ? ~0ull : ((1ull << (jn - j)) - 1) << j;
+          const uint32_t c = (uint32_t)__popcll(bits & m);
+          if (c) atomicAdd(&s_occ[wv], c);
+          off += (uint64_t)(jn - j);
+          j = jn;
+        }
+      }
+      for (int k = 1 + tid; k < CN_STAGE; k += CN_THREADS) {
+        if (!((s_bm[k >> 6] >> (k & 63)) & 1ull)) continue;
+        const uint64_t w = s_w[k];
+        // the cluster's head: after the nearest empty slot before k (h = 0:
+        // every staged slot before k is occupied)
+        int h = 0;
+        for (int wi = k >> 6; wi >= 0; --wi) {
+          uint64_t z = ~s_bm[wi];
+          if (wi == (k >> 6)) z &= (1ull << (k & 63)) - 1;  // the slots before k
+          if (z) {
+            h = 64 * wi + 63 - __clzll((long long)z) + 1;
+            break;
+          }
+        }
+        // owned: headed in [w0, w0 + own) (h = 0: headed before w0; staged
+        // slots past the table's end wrap to its start, which window 0 owns)
+        if (h == 0 || h > own) continue;
+        // the cluster's end: its first empty slot after k (staged bits past
+        // CN_STAGE read as empty)
+        int e = CN_STAGE;
+        for (int wi = k >> 6; wi < CN_BM; ++wi) {
+          uint64_t z = ~s_bm[wi];
+          if (wi == (k >> 6)) z &= ~((2ull << (k & 63)) - 1);  // the slots after k
+          if (z) {
+            e = 64 * wi + __ffsll((long long)z) - 1;
+            break;
+          }
+        }
+        if (e >= CN_STAGE) {  // runs past the staged slots: left as it is
+          atomicOr(a.err, 1u);
+          continue;
+        }
+        const uint64_t key = cn_key<PK>(w);
+        int rank = 0;
+        for (int r = h; r < e; ++r) rank += cn_key<PK>(s_w[r]) < key ? 1 : 0;
+        if (h + rank == k) continue;
+        const uint64_t qd = cn_pos(st0, (uint64_t)(h + rank), cap);
         if (PK) {
-          w = a.pk[p];
+          a.pk[qd] = w;
         } else {
-          const Slot sl = a.wide[p];
-          w = sl.key;
-          s_c[k] = sl.cnt;
+          Slot sl;
+          sl.key = w;
+          sl.cnt = s_c[k];
+          sl.pad = 0;
+          a.wide[qd] = sl;
         }
-        s_w[k] = w;
-        occ = cn_occ<PK>(w);
-        if (WT) s_x[k] = a.wtab[p];
+        if (WT) a.wtab[qd] = s_x[k];
       }
-      const uint64_t b = __ballot(occ);
-      if ((tid & 63) == 0 && k < CN_STAGE) s_bm[k >> 6] = b;
     }
     __syncthreads();
-    for (int k = 1 + tid; k < CN_STAGE; k += CN_THREADS) {
-      if (!((s_bm[k >> 6] >> (k & 63)) & 1ull)) continue;
-      const uint64_t w = s_w[k];
-      // the cluster's head: after the nearest empty slot before k (h = 0:
-      // every staged slot before k is occupied)
-      int h = 0;
-      for (int wi = k >> 6; wi >= 0; --wi) {
-        uint64_t z = ~s_bm[wi];
-        if (wi == (k >> 6)) z &= (1ull << (k & 63)) - 1;  // the slots before k
-        if (z) {
-          h = 64 * wi + 63 - __clzll((long long)z) + 1;
-          break;
-        }
+    if (tid == 0) {  // the window's record; the counters start again at zero
+      CanonWin r;
+#pragma unroll
+      for (int i = 0; i < TR_WAVES; ++i) {
+        r.occ[i] = s_occ[i];
+        s_occ[i] = 0;
       }
-      // owned: headed in [w0, w0 + CN_WIN) (h = 0: headed before w0; past the
-      // table's end: the wrapped start, owned by the first step)
-      if (h == 0 || h > CN_WIN || w0 + (uint64_t)(h - 1) >= cap) continue;
-      // the cluster's end: its first empty slot after k (staged bits past
-      // CN_STAGE read as empty)
-      int e = CN_STAGE;
-      for (int wi = k >> 6; wi < CN_BM; ++wi) {
-        uint64_t z = ~s_bm[wi];
-        if (wi == (k >> 6)) z &= ~((2ull << (k & 63)) - 1);  // the slots after k
-        if (z) {
-          e = 64 * wi + __ffsll((long long)z) - 1;
-          break;
-        }
-      }
-      if (e >= CN_STAGE) {  // runs past the staged slots: left as it is
-        atomicOr(a.err, 1u);
-        continue;
-      }
-      const uint64_t key = cn_key<PK>(w);
-      int rank = 0;
-      for (int r = h; r < e; ++r) rank += cn_key<PK>(s_w[r]) < key ? 1 : 0;
-      if (h + rank == k) continue;
-      const uint64_t q = cn_pos(st0, (uint64_t)(h + rank), cap);
-      if (PK) {
-        a.pk[q] = w;
-      } else {
-        Slot sl;
-        sl.key = w;
-        sl.cnt = s_c[k];
-        sl.pad = 0;
-        a.wide[q] = sl;
-      }
-      if (WT) a.wtab[q] = s_x[k];
+      a.win[t] = r;
     }
-    __syncthreads();
+    // the next window's staging barrier orders the reset before its updates
+  }
+}
+
+// per trunc-log block: the sums of its windows' records (the occupied slots
+// of each wave quarter, and of the block)
+__global__ __launch_bounds__(256) void k_canon_fold(const CanonWin* __restrict__ win, uint64_t wpb,
+                                                    uint32_t* __restrict__ wocc, unsigned long long* __restrict__ bocc) {
+  __shared__ uint32_t s_r[TR_WAVES][256 / 64];
+  uint32_t v[TR_WAVES] = {};
+  const CanonWin* w = win + (uint64_t)blockIdx.x * wpb;
+  for (uint64_t i = threadIdx.x; i < wpb; i += 256) {
+    const CanonWin r = w[i];
+#pragma unroll
+    for (int j = 0; j < TR_WAVES; ++j) v[j] += r.occ[j];
+  }
+#pragma unroll
+  for (int j = 0; j < TR_WAVES; ++j) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v[j] += __shfl_xor(v[j], d, 64);
+    if ((threadIdx.x & 63) == 0) s_r[j][threadIdx.x >> 6] = v[j];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long b = 0;
+    for (int j = 0; j < TR_WAVES; ++j) {
+      uint32_t t = 0;
+      for (int i = 0; i < 256 / 64; ++i) t += s_r[j][i];
+      wocc[(uint64_t)blockIdx.x * TR_WAVES + j] = t;
+      b += t;
+    }
+    bocc[blockIdx.x] = b;
   }
 }
 
@@ -650,7 +761,6 @@ __device__ __forceinline__ double trunc_adj_w(const TruncArgs& a, uint32_t u, ui
 }
 
 
-constexpr int TR_THREADS = 256;
 
 __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
 #pragma unroll
@@ -665,19 +775,6 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
 }
 
 constexpr int HIST_BUCKETS = 4096;
-constexpr int TR_ITEMS = 4;  // consecutive slots per thread per step: loads in flight
-constexpr int TR_WAVES = TR_THREADS / 64;
-
-// slots per block of the trunc-log passes: a multiple of TR_WAVES * TR_ITEMS.
-// Wave w of a block scans its own contiguous quarter of the block's slots
-// (wave_span), so every thread's TR_ITEMS slots start 32-byte aligned in a
-// table of 8-byte slots, and a wave numbers the occupied slots of its
-// quarter in slot order by itself (the value cache's index).
-inline uint64_t trunc_per_block(uint64_t cap, int nblk) {
-  constexpr uint64_t U = (uint64_t)TR_WAVES * TR_ITEMS;
-  const uint64_t pb = (cap + nblk - 1) / nblk;
-  return (pb + U - 1) / U * U;
-}
 
 // the calling wave's quarter [w0, w1) of its block's slots; q (a multiple of
 // TR_ITEMS) is the same for every wave, so loops over [0, q) take the same
@@ -740,26 +837,6 @@ __device__ __forceinline__ bool raw_occ(const RawSlot& r) {
     return (r.a & PK_OCC) != 0;
   else
     return r.a != EMPTY_KEY;
-}
-
-// occupied slots of every wave's quarter of the trunc-log partition:
-// wocc[block * TR_WAVES + wave] (their scan numbers the occupied slots in
-// slot order for the value cache)
-template <bool PK>
-__global__ __launch_bounds__(TR_THREADS) void k_occ_counts(TabView t, uint64_t per_block,
-                                                           uint32_t* __restrict__ wocc) {
-  const WaveSpan ws = wave_span(t.cap, per_block);
-  const int lane = threadIdx.x & 63;
-  uint32_t c = 0;
-  for (uint64_t t0 = 0; t0 < ws.q; t0 += 64 * TR_ITEMS) {
-    RawSlot rs[TR_ITEMS];
-    load_group<PK>(t, ws.w0 + t0 + (uint64_t)lane * TR_ITEMS, ws.w1, rs);
-#pragma unroll
-    for (int j = 0; j < TR_ITEMS; ++j) c += raw_occ<PK>(rs[j]) ? 1u : 0u;
-  }
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (lane == 0) wocc[(uint64_t)blockIdx.x * TR_WAVES + (threadIdx.x >> 6)] = c;
 }
 
 // pass 1: per-block (contiguous slot range) emitted entries and kept keys;
@@ -1183,22 +1260,36 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   return LIGHTNE_OK;
 }
 
-// puts every cluster of the table in canonical order (k_table_canon);
-// *canonical = false when a cluster was too long to reorder (the table is a
-// valid one either way)
+// puts every cluster of the table in canonical order (k_table_canon), over
+// the trunc-log partition; *canonical = false when a cluster was too long to
+// reorder (the table is a valid one either way).  Also leaves the wave
+// quarters' occupied slots in the "tr_wocc" buffer and the blocks' in
+// "canon_bocc" (canon_distinct).
 int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical, double* seconds) {
   *canonical = false;
+  const int nblk = trunc_blocks(tv.cap);
   auto* err = (uint32_t*)ctx_buf(ctx, "canon_err", sizeof(uint32_t));
-  if (!err) return LIGHTNE_ERR_OOM;
+  auto* wocc = (uint32_t*)ctx_buf(ctx, "tr_wocc", (size_t)nblk * TR_WAVES * sizeof(uint32_t));
+  auto* bocc = (unsigned long long*)ctx_buf(ctx, "canon_bocc", (size_t)nblk * sizeof(unsigned long long));
+  const uint64_t per_block = trunc_per_block(tv.cap, nblk);
+  const uint64_t wpb = (per_block + CN_WIN - 1) / CN_WIN;
+  const uint64_t nwin = (uint64_t)nblk * wpb;
+  auto* win = (CanonWin*)ctx_buf(ctx, "canon_win", nwin * sizeof(CanonWin));
+  if (!err || !wocc || !bocc || !win) return LIGHTNE_ERR_OOM;
   LN_HIP(ctx, hipMemsetAsync(err, 0, sizeof(uint32_t), ctx->stream));
   LN_HIP(ctx, hipEventRecord(ctx->ev[6], ctx->stream));
   CanonArgs a;
   a.wide = const_cast<Slot*>(tv.wide);
   a.pk = const_cast<unsigned long long*>(tv.pk);
+  a.ovf = tv.ovf;
   a.wtab = const_cast<double*>(tv.wtab);
   a.cap = tv.cap;
+  a.per_block = per_block;
+  a.wpb = wpb;
+  a.nwin = nwin;
   a.err = err;
-  const unsigned nb = (unsigned)std::min<uint64_t>((tv.cap + CN_WIN - 1) / CN_WIN, 16384);
+  a.win = win;
+  const unsigned nb = (unsigned)std::min<uint64_t>(nwin, 16384);
   const hipStream_t s = ctx->stream;
   if (tv.pk) {
     if (tv.wtab)
@@ -1212,12 +1303,29 @@ int table_canon(lightne_ctx* ctx, const TabView& tv, bool* canonical, double* se
       hipLaunchKernelGGL((k_table_canon<false, false>), dim3(nb), dim3(CN_THREADS), 0, s, a);
   }
   LN_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_canon_fold, dim3(nblk), dim3(256), 0, s, win, wpb, wocc, bocc);
+  LN_LAUNCH(ctx);
   LN_HIP(ctx, hipEventRecord(ctx->ev[7], s));
   uint32_t h = 0;
   LN_HIP(ctx, hipMemcpyAsync(&h, err, sizeof(h), hipMemcpyDeviceToHost, s));
   LN_HIP(ctx, hipStreamSynchronize(s));
   *canonical = h == 0;
   *seconds = ev_seconds(ctx->ev[6], ctx->ev[7]);
+  return LIGHTNE_OK;
+}
+
+// distinct keys (occupied slots) from the canonical-order pass just run
+int canon_distinct(lightne_ctx* ctx, const TabView& tv, uint64_t* distinct) {
+  const int nblk = trunc_blocks(tv.cap);
+  auto* bocc = (unsigned long long*)ctx_buf(ctx, "canon_bocc", (size_t)nblk * sizeof(unsigned long long));
+  if (!bocc) return LIGHTNE_ERR_OOM;
+  std::vector<unsigned long long> hb((size_t)nblk);
+  LN_HIP(ctx, hipMemcpyAsync(hb.data(), bocc, hb.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  LN_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  uint64_t d = 0;
+  for (int b = 0; b < nblk; ++b) d += hb[(size_t)b];
+  *distinct = d;
   return LIGHTNE_OK;
 }
 
@@ -1319,14 +1427,14 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   hipStream_t s = ctx->stream;
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int colbits = std::max(1, bits_for((uint64_t)g->n));
-  // blocks of the trunc-log passes (contiguous slot ranges): enough that the
-  // last round of resident blocks leaves little of the device idle on large
-  // tables (about seven 256-thread blocks fit a CU)
-  const int nblk = tv.cap >= (1ull << 30) ? 16384 : 4096;
+  // blocks of the trunc-log passes (contiguous slot ranges, the partition
+  // the canonical-order pass took)
+  const int nblk = trunc_blocks(tv.cap);
   auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
   auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
   auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
-  // occupied slots per wave quarter and their scan (the value cache's index)
+  // occupied slots per wave quarter (left by the canonical-order pass) and
+  // their scan (the value cache's index)
   auto* wocc = (uint32_t*)ctx_buf(ctx, "tr_wocc", (size_t)nblk * TR_WAVES * sizeof(uint32_t));
   auto* occ_off = (uint64_t*)ctx_buf(ctx, "tr_occ", ((size_t)nblk * TR_WAVES + 1) * sizeof(uint64_t));
   if (!blk_emit || !blk_kept || !blk_off || !wocc || !occ_off) return LIGHTNE_ERR_OOM;
@@ -1376,14 +1484,22 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     lw_in_ws = false;
     return LIGHTNE_OK;
   };
-  // totals (PathEmbed.h:175-200), then per-block counts of the kept entries
-  // over contiguous slot ranges and their scan; repeated after a workspace
-  // regrowth re-ran the (deterministic) sampler
+  // totals (PathEmbed.h:175-200; the distinct keys from the canonical-order
+  // pass, -sample 1 weights by their own reduction), then per-block counts of
+  // the kept entries over contiguous slot ranges and their scan; repeated
+  // after a workspace regrowth re-ran the (deterministic) sampler
   auto count_pass = [&](bool hist) -> int {
     LN_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     uint64_t distinct, csum;
-    double wsum;
-    LN_CHECK(table_totals(ctx, tv, &distinct, &csum, &wsum));
+    double wsum = 0.0;
+    if (p->sample) {
+      LN_CHECK(table_totals(ctx, tv, &distinct, &csum, &wsum));
+    } else {
+      // every sample added one to its key's count (the sampler fails
+      // otherwise), so the counts sum to the samples drawn
+      LN_CHECK(canon_distinct(ctx, tv, &distinct));
+      csum = st.samples;
+    }
     st.distinct_keys = distinct;
     // PathEmbed.h:187 (sum of slot values) and :199-200 (factor)
     st.M_exact = p->sample ? wsum : (double)csum;
@@ -1444,11 +1560,6 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     if (lw) {
       // rank of each wave quarter's first occupied slot
-      if (tv.pk)
-        hipLaunchKernelGGL(k_occ_counts<true>, dim3(nblk), dim3(TR_THREADS), 0, s, tv, per_block, wocc);
-      else
-        hipLaunchKernelGGL(k_occ_counts<false>, dim3(nblk), dim3(TR_THREADS), 0, s, tv, per_block, wocc);
-      LN_LAUNCH(ctx);
       LN_CHECK(scan_u32_to_u64(ctx, wocc, occ_off, (int64_t)nblk * TR_WAVES));
       ta.occ_off = occ_off;
     }
