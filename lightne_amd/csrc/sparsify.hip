@@ -774,7 +774,7 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* s_w) {
   return t;
 }
 
-constexpr int HIST_BUCKETS = 4096;
+constexpr int HIST_BUCKETS = 1024;  // row buckets of a wave's histogram
 
 // the calling wave's quarter [w0, w1) of its block's slots; q (a multiple of
 // TR_ITEMS) is the same for every wave, so loops over [0, q) take the same
@@ -839,22 +839,24 @@ __device__ __forceinline__ bool raw_occ(const RawSlot& r) {
     return r.a != EMPTY_KEY;
 }
 
-// pass 1: per-block (contiguous slot range) emitted entries and kept keys;
-// with HIST also the block's row histogram (HIST_BUCKETS ranges of 2^shift
-// rows), stored bucket-major: bh[bucket * gridDim.x + block]
+// pass 1: per wave quarter (contiguous slot range) the emitted entries,
+// wav_emit[block * TR_WAVES + wave], and per block the kept keys; with HIST
+// also every wave's row histogram (HIST_BUCKETS ranges of 2^shift rows),
+// stored bucket-major: bh[bucket * (gridDim.x * TR_WAVES) + block * TR_WAVES + wave]
 template <bool HIST, bool PK>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_t per_block,
-                                                            uint32_t* __restrict__ blk_emit,
+                                                            uint32_t* __restrict__ wav_emit,
                                                             uint32_t* __restrict__ blk_kept, int shift, int nbuck,
                                                             uint32_t* __restrict__ bh) {
   __shared__ uint32_t s_w[TR_WAVES];
-  __shared__ uint32_t s_h[HIST ? HIST_BUCKETS : 1];
+  __shared__ uint32_t s_h[HIST ? TR_WAVES * HIST_BUCKETS : 1];
   if (HIST) {
-    for (int b = threadIdx.x; b < HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
+    for (int b = threadIdx.x; b < TR_WAVES * HIST_BUCKETS; b += TR_THREADS) s_h[b] = 0;
     __syncthreads();
   }
   const WaveSpan ws = wave_span(a.t.cap, per_block);
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t* const wh = s_h + (HIST ? wid * HIST_BUCKETS : 0);  // the wave's histogram
   // occupied rank of the wave's next occupied slot (value cache index)
   uint64_t orun = a.lw_out ? a.occ_off[(uint64_t)blockIdx.x * TR_WAVES + (threadIdx.x >> 6)] : 0;
   uint32_t e = 0, kk = 0;
@@ -897,8 +899,8 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
       e += 1 + (ev ? 1 : 0);
       ++kk;
       if (HIST) {
-        atomicAdd(&s_h[u >> shift], 1u);
-        if (ev) atomicAdd(&s_h[v >> shift], 1u);
+        atomicAdd(&wh[u >> shift], 1u);
+        if (ev) atomicAdd(&wh[v >> shift], 1u);
       }
     }
     // the value cache: the wave's occupied slots in slot order take
@@ -915,25 +917,29 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_count(TruncArgs a, uint64_
       orun += wtot;
     }
   }
-  const uint32_t te = block_sum_u32(e, s_w);
-  const uint32_t tk = block_sum_u32(kk, s_w);
-  if (threadIdx.x == 0) {
-    blk_emit[blockIdx.x] = te;
-    blk_kept[blockIdx.x] = tk;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) e += __shfl_xor(e, d, 64);
+  if (lane == 0) wav_emit[(uint64_t)blockIdx.x * TR_WAVES + wid] = e;
+  const uint32_t tk = block_sum_u32(kk, s_w);  // its barriers also order the histograms' updates
+  if (threadIdx.x == 0) blk_kept[blockIdx.x] = tk;
+  if (HIST) {
+    const uint64_t nw = (uint64_t)gridDim.x * TR_WAVES;
+    for (int i = threadIdx.x; i < nbuck * TR_WAVES; i += TR_THREADS) {
+      const int b = i / TR_WAVES, w = i % TR_WAVES;
+      bh[(uint64_t)b * nw + (uint64_t)blockIdx.x * TR_WAVES + w] = s_h[w * HIST_BUCKETS + b];
+    }
   }
-  if (HIST)
-    for (int b = threadIdx.x; b < nbuck; b += TR_THREADS) bh[(uint64_t)b * gridDim.x + blockIdx.x] = s_h[b];
 }
 
-// row histogram totals: bucket b (one block per bucket) sums its nblk block
-// counts, read along the bucket's contiguous row of bh
-__global__ __launch_bounds__(256) void k_bh_totals(const uint32_t* __restrict__ bh, int nbuck, int nblk,
+// row histogram totals: bucket b (one block per bucket) sums its nw wave
+// quarters' counts, read along the bucket's contiguous row of bh
+__global__ __launch_bounds__(256) void k_bh_totals(const uint32_t* __restrict__ bh, int nbuck, int nw,
                                                    unsigned long long* __restrict__ hist) {
   __shared__ unsigned long long s_t[256 / 64];
   const int b = blockIdx.x;
   if (b >= nbuck) return;
   unsigned long long t = 0;
-  for (int k = threadIdx.x; k < nblk; k += 256) t += bh[(uint64_t)b * nblk + k];
+  for (int k = threadIdx.x; k < nw; k += 256) t += bh[(uint64_t)b * nw + k];
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) t += __shfl_xor(t, d, 64);
   if ((threadIdx.x & 63) == 0) s_t[threadIdx.x >> 6] = t;
@@ -941,14 +947,15 @@ __global__ __launch_bounds__(256) void k_bh_totals(const uint32_t* __restrict__ 
   if (threadIdx.x == 0) hist[b] = s_t[0] + s_t[1] + s_t[2] + s_t[3];
 }
 
-// per-block entries of the buckets [lo, hi) (a row range)
-__global__ void k_range_counts(const uint32_t* __restrict__ bh, int lo, int hi, int nblk,
-                               uint32_t* __restrict__ blk_cnt) {
+// per wave quarter: entries of the buckets [lo, hi) (a row range); nw
+// quarters
+__global__ void k_range_counts(const uint32_t* __restrict__ bh, int lo, int hi, int nw,
+                               uint32_t* __restrict__ wav_cnt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nblk) return;
+  if (k >= nw) return;
   uint32_t t = 0;
-  for (int b = lo; b < hi; ++b) t += bh[(uint64_t)b * nblk + k];
-  blk_cnt[k] = t;
+  for (int b = lo; b < hi; ++b) t += bh[(uint64_t)b * nw + k];
+  wav_cnt[k] = t;
 }
 
 __global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* __restrict__ deg) {
@@ -957,17 +964,16 @@ __global__ void k_degrees(const int64_t* __restrict__ off, int64_t n, uint32_t* 
 }
 
 // pass 2: same partition; the entries of rows [r0, r1) (all rows: 0, ~0) at
-// the block's offset (the scan of pass 1's per-block counts, or of a row
-// range's), as sort keys ((row - r0) << colbits | col) and float values, in
-// an order fixed by the partition (the sort that follows orders them
-// completely: no two entries share a key).  No contended atomics.
+// the wave quarter's offset (the scan of pass 1's per-quarter counts, or of
+// a row range's), as sort keys ((row - r0) << colbits | col) and float
+// values, in slot order (a key emits its row-u entry, then its row-v one).
+// Every wave writes its own run: no block barrier, no contended atomics.
 template <bool PK>
 __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t per_block, uint32_t r0, uint32_t r1,
-                                                           const uint64_t* __restrict__ blk_off) {
-  __shared__ uint32_t s_w[TR_WAVES];
+                                                           const uint64_t* __restrict__ wav_off) {
   const WaveSpan ws = wave_span(a.t.cap, per_block);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint64_t run = blk_off[blockIdx.x];
+  uint64_t run = wav_off[(uint64_t)blockIdx.x * TR_WAVES + wid];
   uint64_t orun = a.lw_in ? a.occ_off[(uint64_t)blockIdx.x * TR_WAVES + wid] : 0;
   for (uint64_t t0 = 0; t0 < ws.q; t0 += 64 * TR_ITEMS) {
     const uint64_t ib = ws.w0 + t0 + (uint64_t)lane * TR_ITEMS;
@@ -1041,22 +1047,9 @@ __global__ __launch_bounds__(TR_THREADS) void k_trunc_emit(TruncArgs a, uint64_t
       }
       c += (eu[j] ? 1 : 0) + (ev[j] ? 1 : 0);
     }
-    // block-wide exclusive scan of the threads' counts
-    int incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int t = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += t;
-    }
-    __syncthreads();
-    if (lane == 63) s_w[wid] = (uint32_t)incl;
-    __syncthreads();
-    uint32_t wbase = 0, tot = 0;
-    for (int w = 0; w < TR_WAVES; ++w) {
-      if (w < wid) wbase += s_w[w];
-      tot += s_w[w];
-    }
-    uint64_t pos = run + wbase + (uint32_t)(incl - c);
+    // the wave's exclusive scan of its lanes' counts
+    uint32_t tot;
+    uint64_t pos = run + wave_excl_scan((uint32_t)c, &tot);
 #pragma unroll
     for (int j = 0; j < TR_ITEMS; ++j) {
       if (!eu[j] && !ev[j]) continue;
@@ -1430,24 +1423,29 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
   // blocks of the trunc-log passes (contiguous slot ranges, the partition
   // the canonical-order pass took)
   const int nblk = trunc_blocks(tv.cap);
-  auto* blk_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", nblk * sizeof(uint32_t));
+  // the emission's units are the blocks' wave quarters: entries per quarter
+  // and their scan (each quarter's output offset); kept keys per block
+  const int nw = nblk * TR_WAVES;
+  auto* wav_emit = (uint32_t*)ctx_buf(ctx, "tr_emit", (size_t)nw * sizeof(uint32_t));
   auto* blk_kept = (uint32_t*)ctx_buf(ctx, "tr_kept", nblk * sizeof(uint32_t));
-  auto* blk_off = (uint64_t*)ctx_buf(ctx, "tr_off", (nblk + 1) * sizeof(uint64_t));
+  auto* wav_off = (uint64_t*)ctx_buf(ctx, "tr_off", ((size_t)nw + 1) * sizeof(uint64_t));
   // occupied slots per wave quarter (left by the canonical-order pass) and
   // their scan (the value cache's index)
   auto* wocc = (uint32_t*)ctx_buf(ctx, "tr_wocc", (size_t)nblk * TR_WAVES * sizeof(uint32_t));
   auto* occ_off = (uint64_t*)ctx_buf(ctx, "tr_occ", ((size_t)nblk * TR_WAVES + 1) * sizeof(uint64_t));
-  if (!blk_emit || !blk_kept || !blk_off || !wocc || !occ_off) return LIGHTNE_ERR_OOM;
+  if (!wav_emit || !blk_kept || !wav_off || !wocc || !occ_off) return LIGHTNE_ERR_OOM;
   TruncArgs ta;
   int64_t nnz = 0;
   auto* deg = (uint32_t*)ctx_buf(ctx, "s1_deg", (size_t)g->n * 4);
   if (!deg) return LIGHTNE_ERR_OOM;
   hipLaunchKernelGGL(k_degrees, dim3(grid_blocks(g->n, 256)), dim3(256), 0, s, g->offsets, g->n, deg);
   LN_LAUNCH(ctx);
-  // row buckets for a build by row ranges: HIST_BUCKETS ranges of 2^shift rows
-  const int shift = std::max(0, bits_for((uint64_t)g->n) - 12);
+  // row buckets for a build by row ranges: at most HIST_BUCKETS ranges of
+  // 2^shift rows
+  const int shift = std::max(0, bits_for((uint64_t)g->n) - 10);
   const int nbuck = (int)(((uint64_t)g->n + ((1ull << shift) - 1)) >> shift);
-  uint32_t* bh = nullptr;  // per-block row histograms, when taken
+  static_assert(HIST_BUCKETS == 1 << 10, "n < 2^bits_for(n) rows in 2^10 buckets");
+  uint32_t* bh = nullptr;  // per-quarter row histograms, when taken
   // Value cache: the count pass leaves the CSR value of every occupied slot
   // (one float per distinct key, in slot order) in an array right behind the
   // table in the workspace (context-owned builds only), so the emission
@@ -1564,28 +1562,28 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       ta.occ_off = occ_off;
     }
     if (hist) {
-      bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nblk * 4);
+      bh = (uint32_t*)ctx_buf(ctx, "tr_bh", (size_t)nbuck * nw * 4);
       if (!bh) return LIGHTNE_ERR_OOM;
       if (tv.pk)
-        hipLaunchKernelGGL((k_trunc_count<true, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+        hipLaunchKernelGGL((k_trunc_count<true, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, wav_emit,
                            blk_kept, shift, nbuck, bh);
       else
-        hipLaunchKernelGGL((k_trunc_count<true, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+        hipLaunchKernelGGL((k_trunc_count<true, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, wav_emit,
                            blk_kept, shift, nbuck, bh);
     } else {
       if (tv.pk)
-        hipLaunchKernelGGL((k_trunc_count<false, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+        hipLaunchKernelGGL((k_trunc_count<false, true>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, wav_emit,
                            blk_kept, 0, 0, (uint32_t*)nullptr);
       else
-        hipLaunchKernelGGL((k_trunc_count<false, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, blk_emit,
+        hipLaunchKernelGGL((k_trunc_count<false, false>), dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, wav_emit,
                            blk_kept, 0, 0, (uint32_t*)nullptr);
     }
     LN_LAUNCH(ctx);
-    LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+    LN_CHECK(scan_u32_to_u64(ctx, wav_emit, wav_off, nw));
     std::vector<uint32_t> hk(nblk);
     uint64_t htot = 0, otot = distinct;
     LN_HIP(ctx, hipMemcpyAsync(hk.data(), blk_kept, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    LN_HIP(ctx, hipMemcpyAsync(&htot, blk_off + nblk, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    LN_HIP(ctx, hipMemcpyAsync(&htot, wav_off + nw, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     if (lw)
       LN_HIP(ctx, hipMemcpyAsync(&otot, occ_off + (size_t)nblk * TR_WAVES, sizeof(uint64_t), hipMemcpyDeviceToHost,
                                  s));
@@ -1683,10 +1681,10 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
     const uint64_t per_block = trunc_per_block(tv.cap, nblk);
     if (tv.pk)
       hipLaunchKernelGGL(k_trunc_emit<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
-                         (const uint64_t*)blk_off);
+                         (const uint64_t*)wav_off);
     else
       hipLaunchKernelGGL(k_trunc_emit<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, 0u, 0xffffffffu,
-                         (const uint64_t*)blk_off);
+                         (const uint64_t*)wav_off);
     LN_LAUNCH(ctx);
     // rows sorted by column: a key's table slot depends on the order concurrent
     // inserts claimed slots, so the column order is what makes the matrix (and
@@ -1699,12 +1697,12 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
                        g->n, colbits, rp, col, vres == val ? nullptr : (const float*)vres, val, (int64_t)0);
     LN_LAUNCH(ctx);
   } else {
-    // row histogram of the emitted entries: per block (taken by the count
-    // pass), summed per bucket
+    // row histogram of the emitted entries: per wave quarter (taken by the
+    // count pass), summed per bucket
     if (!bh) LN_CHECK(count_pass(true));
     auto* hist = (unsigned long long*)ctx_buf(ctx, "tr_hist", HIST_BUCKETS * 8);
     if (!hist) return LIGHTNE_ERR_OOM;
-    hipLaunchKernelGGL(k_bh_totals, dim3(nbuck), dim3(256), 0, s, (const uint32_t*)bh, nbuck, nblk,
+    hipLaunchKernelGGL(k_bh_totals, dim3(nbuck), dim3(256), 0, s, (const uint32_t*)bh, nbuck, nw,
                        hist);
     LN_LAUNCH(ctx);
     std::vector<unsigned long long> hh(HIST_BUCKETS);
@@ -1975,22 +1973,22 @@ static int sparsify_sized(lightne_ctx* ctx, const lightne_graph* g, const lightn
       float* vres = pval;
       if (cnt > 0) {
         const uint64_t per_block = trunc_per_block(tv.cap, nblk);
-        hipLaunchKernelGGL(k_range_counts, dim3((nblk + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh,
-                           rng[r].first, rng[r].second, nblk, blk_emit);
+        hipLaunchKernelGGL(k_range_counts, dim3((nw + 255) / 256), dim3(256), 0, s, (const uint32_t*)bh,
+                           rng[r].first, rng[r].second, nw, wav_emit);
         LN_LAUNCH(ctx);
-        LN_CHECK(scan_u32_to_u64(ctx, blk_emit, blk_off, nblk));
+        LN_CHECK(scan_u32_to_u64(ctx, wav_emit, wav_off, nw));
         uint64_t got = 0;
-        LN_HIP(ctx, hipMemcpyAsync(&got, blk_off + nblk, 8, hipMemcpyDeviceToHost, s));
+        LN_HIP(ctx, hipMemcpyAsync(&got, wav_off + nw, 8, hipMemcpyDeviceToHost, s));
         LN_HIP(ctx, hipStreamSynchronize(s));
         if ((int64_t)got != cnt)
           return ctx_fail(ctx, LIGHTNE_ERR_HIP, "row range %llu..%llu holds %llu entries, the histogram %lld",
                           (unsigned long long)r0, (unsigned long long)r1, (unsigned long long)got, (long long)cnt);
         if (tv.pk)
           hipLaunchKernelGGL(k_trunc_emit<true>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
-                             (uint32_t)r1, (const uint64_t*)blk_off);
+                             (uint32_t)r1, (const uint64_t*)wav_off);
         else
           hipLaunchKernelGGL(k_trunc_emit<false>, dim3(nblk), dim3(TR_THREADS), 0, s, ta, per_block, (uint32_t)r0,
-                             (uint32_t)r1, (const uint64_t*)blk_off);
+                             (uint32_t)r1, (const uint64_t*)wav_off);
         LN_LAUNCH(ctx);
         LN_CHECK(radix_sort_pairs(ctx, pkey, pval, cnt, bits_for(r1 - r0) + (canon ? 0 : colbits), ktmp, vtmp,
                                   canon ? colbits : 0, &kres, &vres));

@@ -82,14 +82,13 @@ def test_csr_matches_oracle(gpu, rec):
     csr.free()
 
 
-def _canonical_row_order(keys, kept, cap, nblk=4096, per_thread=4, threads=256):
+def _canonical_row_order(keys, kept, cap):
     """Host model of the CSR's within-row order: linear probing of `keys` in a
     table of `cap` slots, each cluster (maximal run of occupied slots) sorted
-    by key, then the
-    trunc-log emission's order over the slots (nblk blocks of per_block
-    slots, each block's waves scanning their own quarter in steps of
-    64 x per_thread slots), the kept keys' entries (u, v) then (v, u), and a
-    stable sort by row.  Returns the entries' (row, col) in CSR order."""
+    by key, then the trunc-log emission's order, which is slot order (every
+    wave quarter of the partition writes its own run, the quarters in slot
+    order), the kept keys' entries (u, v) then (v, u), and a stable sort by
+    row.  Returns the entries' (row, col) in CSR order."""
     h = np.array([(O.hash64_2(int(k)) * cap) >> 64 for k in keys], np.int64)
     slot = np.full(cap, -1, np.int64)
     for i, hi in enumerate(h):
@@ -110,15 +109,7 @@ def _canonical_row_order(keys, kept, cap, nblk=4096, per_thread=4, threads=256):
             ids = slot[run]
             slot[run] = ids[np.argsort(keys[ids])]
         run = []
-    pb = -(-cap // nblk)
-    unit = 4 * per_thread
-    pb = -(-pb // unit) * unit
-    q = pb // 4
     pos = np.flatnonzero(occ)
-    b, o = pos // pb, pos % pb
-    w, oq = o // q, o % q
-    t, lane, j = oq // (64 * per_thread), (oq % (64 * per_thread)) // per_thread, oq % per_thread
-    pos = pos[np.lexsort((j, lane, w, t, b))]
     rows, cols = [], []
     for p in pos:
         i = slot[p]
