@@ -135,10 +135,13 @@ typedef struct {
                                each distinct key's CSR value where memory
                                allows), 1 always (the workspace grows for
                                it), -1 never; the CSR is the same either way */
-  int32_t wide_offsets;     /* 0: the walks read 32-bit row pointers when the
-                               graph has fewer than 2^32 directed edges and
-                               the graph's 64-bit offsets otherwise; 1: the
-                               64-bit offsets always (the same samples)     */
+  int32_t wide_offsets;     /* 0: the walks read each neighbour with its row
+                               (entries built per call) when the graph has
+                               fewer than 2^32 directed edges, else the
+                               graph's 64-bit offsets; 1: neighbours and the
+                               64-bit offsets always; 2: neighbours and 32-bit
+                               row pointers where they fit (the same samples
+                               every way)                                    */
 } lightne_sparsify_params;
 
 typedef struct {
@@ -173,7 +176,11 @@ typedef struct {
                               the key set (canonical table order), 2 by
                               column (a table cluster was too long to
                               reorder); the same on every run either way   */
-  uint32_t reserved_;
+  uint32_t walk_rows;      /* 1: the walks read each neighbour with its row
+                              (one 16-byte entry per directed edge, for the
+                              sampler's run), 0: neighbours and row pointers
+                              apart (64-bit offsets, or no memory for the
+                              entries); the same samples either way        */
 } lightne_sparsify_stats;
 
 /* Host-only evaluation of the sampling scalars (PathEmbed.h:89,99,108-117). */

@@ -67,7 +67,7 @@ class SparsifyStats(C.Structure):
         ("M_exact", C.c_double), ("factor", C.c_double), ("kept_upper", C.c_uint64), ("nnz", C.c_uint64),
         ("t_sample", C.c_double), ("t_table", C.c_double), ("t_csr", C.c_double),
         ("table_layout", C.c_uint32), ("csr_parts", C.c_uint32), ("value_cache", C.c_uint32),
-        ("sampler_runs", C.c_uint32), ("t_canon", C.c_double), ("row_order", C.c_uint32), ("reserved_", C.c_uint32),
+        ("sampler_runs", C.c_uint32), ("t_canon", C.c_double), ("row_order", C.c_uint32), ("walk_rows", C.c_uint32),
     ]
 
 

@@ -321,9 +321,11 @@ def _sparsify_params(seed, walk_len, upper, sample, table_size, negative, sample
     ``value_cache`` (pipeline): 0 auto, 1 always, -1 never cache the count
     pass's CSR value of every distinct key for the emission (same CSR either
     way).
-    ``wide_offsets``: 0 auto (32-bit row pointers for the walks when the graph
-    has fewer than 2^32 directed edges), 1 the graph's 64-bit offsets always
-    (same samples)."""
+    ``wide_offsets``: 0 auto (the walks read each neighbour with its row,
+    entries built per call, when the graph has fewer than 2^32 directed edges;
+    else the graph's 64-bit offsets), 1 neighbours and the 64-bit offsets
+    always, 2 neighbours and 32-bit row pointers where they fit (same samples
+    every way)."""
     return SparsifyParams(walk_len=walk_len, negative=negative, sample=int(bool(sample)), upper=int(bool(upper)),
                           sample_ratio=sample_ratio, mem_ratio=mem_ratio, tablesz=table_size,
                           seed=seed & 0xFFFFFFFFFFFFFFFF, max_table_bytes=max_table_bytes,

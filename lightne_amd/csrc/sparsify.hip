@@ -254,12 +254,24 @@ __device__ __forceinline__ void block_add3(unsigned long long* dst, uint64_t a, 
   }
 }
 
+// A directed edge's target with that vertex's row: a walk step reads one of
+// these (one 16-byte load, one line) instead of a neighbour and then, at the
+// next step, the neighbour's row pointers (a second random line).  Built per
+// call where the workspace has room and every degree fits 32 bits (2m < 2^32).
+struct NbrRow {
+  uint32_t v;    // the edge's target
+  uint32_t deg;  // its degree
+  uint64_t off;  // its first edge
+};
+static_assert(sizeof(NbrRow) == 16, "one 16-byte load per walk step");
+
 struct SampleArgs {
   // row pointers: 32 bits when 2m < 2^32 (half the bytes of the walk's hot
   // array), else the graph's own 64-bit offsets
   const uint32_t* off32;
   const int64_t* off64;
   const uint32_t* adj;
+  const NbrRow* nr;  // neighbour rows (NR instantiations), else null
   uint64_t ith;  // 1: compressed-graph neighbour lookup
   const uint32_t* src;
   int64_t m;
@@ -299,6 +311,36 @@ __device__ __forceinline__ uint32_t walk(const OffT* __restrict__ off, const uin
     uint64_t i = r % d;
     i -= ith & (uint64_t)((i & 63) != 0);
     x = adj[(uint64_t)o + i];
+    s = hash64_2(r);
+  }
+  return x;
+}
+
+__global__ void k_nbr_rows(const int64_t* __restrict__ off, const uint32_t* __restrict__ adj, int64_t m,
+                           NbrRow* __restrict__ nr) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = adj[j];
+    const int64_t o = off[v];
+    nr[j] = NbrRow{v, (uint32_t)(off[v + 1] - o), (uint64_t)o};
+  }
+}
+
+// random_walk over the neighbour rows: the same draws and picks as walk(),
+// the row of each vertex reached coming with it
+__device__ __forceinline__ uint32_t walk_nr(const NbrRow* __restrict__ nr, uint32_t x, uint64_t len, uint64_t& s,
+                                            bool& bad, uint64_t ith, uint64_t o, uint64_t d) {
+  for (uint64_t t = 0; t < len; ++t) {
+    if (d == 0) {
+      bad = true;
+      break;
+    }
+    const uint64_t r = hash64_2(s);
+    uint64_t i = r % d;
+    i -= ith & (uint64_t)((i & 63) != 0);
+    const NbrRow e = nr[o + i];
+    x = e.v;
+    o = e.off;
+    d = e.deg;
     s = hash64_2(r);
   }
   return x;
@@ -351,7 +393,7 @@ __device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint3
   return 0;
 }
 
-template <int MODE, bool PACKED, bool OFF64>
+template <int MODE, bool PACKED, bool OFF64, bool NR>
 __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   const int64_t j = a.j0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const auto* off = [&] {
@@ -364,7 +406,9 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
   bool overflow = false, bad = false, side_full = false;
   if (j < a.m) {
     const uint32_t u = a.src[j];
-    const uint32_t v = a.adj[j];
+    NbrRow ej{};
+    if (NR) ej = a.nr[j];
+    const uint32_t v = NR ? ej.v : a.adj[j];
     const uint64_t uv = ((uint64_t)u << 32) | (uint64_t)v;
     uint64_t s = hash64_2(hash64_2(uv + a.seed));  // seed.fork(uv)
     uint64_t e = a.exp_rv;
@@ -376,8 +420,9 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
     n_attempts = e;
     double p = 1.0, ipw = 1.0;
     // the rows of u and v, where every walk of this edge's samples starts
-    const auto ou = off[u], ov = off[v];
-    const uint64_t du_ = (uint64_t)(off[u + 1] - ou), dv_ = (uint64_t)(off[v + 1] - ov);
+    const auto ou = off[u];
+    const auto ov = NR ? (decltype(ou))ej.off : off[v];
+    const uint64_t du_ = (uint64_t)(off[u + 1] - ou), dv_ = NR ? (uint64_t)ej.deg : (uint64_t)(off[v + 1] - ov);
     if (MODE == 1) {
       const double du = (double)du_;
       const double dv = (double)dv_;
@@ -397,8 +442,14 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       rr = hash64_2(s);
       const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
       s = hash64_2(rr);
-      const uint32_t x = walk(off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith, ou, du_);
-      const uint32_t z = walk(off, a.adj, v, r - kk, s, bad, a.ith, ov, dv_);
+      uint32_t x, z;
+      if (NR) {
+        x = walk_nr(a.nr, u, (uint64_t)kk - 1, s, bad, a.ith, (uint64_t)ou, du_);
+        z = walk_nr(a.nr, v, r - kk, s, bad, a.ith, (uint64_t)ov, dv_);
+      } else {
+        x = walk(off, a.adj, u, (uint64_t)kk - 1, s, bad, a.ith, ou, du_);
+        z = walk(off, a.adj, v, r - kk, s, bad, a.ith, ov, dv_);
+      }
       n_steps += r - 1;
       const int rc = table_add<MODE, PACKED>(a, min(x, z), max(x, z), ipw);
       if (rc == 0) overflow = true;
@@ -1118,8 +1169,8 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "walk_len and negative must be >= 1");
   if (p->table_layout < 0 || p->table_layout > 2 || p->csr_parts < 0)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "table_layout must be 0, 1 or 2 and csr_parts >= 0");
-  if (p->wide_offsets != 0 && p->wide_offsets != 1)
-    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "wide_offsets must be 0 or 1");
+  if (p->wide_offsets < 0 || p->wide_offsets > 2)
+    return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "wide_offsets must be 0, 1 or 2");
   const bool fits_packed = (uint64_t)g->n <= (1ull << PK_VB);
   if (p->table_layout == 2 && !fits_packed)
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "the packed table layout needs n <= 2^26");
@@ -1168,6 +1219,21 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
   const bool off64 = p->wide_offsets == 1 || (uint64_t)g->m >= ((uint64_t)1 << 32);
   auto* off32 = off64 ? nullptr : (uint32_t*)ctx_buf(ctx, "sample_off32", (size_t)(g->n + 1) * 4);
   if (!stats || (packed && !ovf) || (!off64 && !off32)) return LIGHTNE_ERR_OOM;
+  // neighbour rows for the walks (every degree fits 32 bits when 2m < 2^32):
+  // behind the table and the sources where the workspace already has the
+  // room (it does not grow for them), else in memory of their own for the
+  // sampler's run, else the walks read row pointers
+  NbrRow* nr = nullptr;
+  DevMem nr_mem;
+  if (p->wide_offsets == 0 && !off64 && g->m > 0) {
+    const auto& wsb = ctx->bufs["ws"];
+    const size_t nr_at = wc.end(), nr_bytes = (size_t)g->m * sizeof(NbrRow);
+    if (wsb.p && wsb.bytes >= nr_at + nr_bytes)
+      nr = (NbrRow*)((char*)wsb.p + nr_at);
+    else if (ctx_malloc(ctx, &nr_mem.p, nr_bytes) == hipSuccess)
+      nr = (NbrRow*)nr_mem.p;
+  }
+  st->walk_rows = nr ? 1u : 0u;
 
   hipStream_t s = ctx->stream;
   LN_HIP(ctx, hipEventRecord(ctx->ev[0], s));
@@ -1194,6 +1260,7 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     a.off32 = off32;
     a.off64 = g->offsets;
     a.adj = g->edges;
+    a.nr = nr;
     a.ith = g->ith_rule == LIGHTNE_ITH_BYTEPD ? 1u : 0u;
     a.src = src;
     a.m = g->m;
@@ -1209,22 +1276,33 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     a.cap = cap;
     a.stats = stats;
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    if (nr) {
+      hipLaunchKernelGGL(k_nbr_rows, dim3(grid_blocks(g->m, 256, (int64_t)1 << 20)), dim3(256), 0, s, g->offsets,
+                         g->edges, g->m, nr);
+      LN_LAUNCH(ctx);
+    }
     // one lane per directed edge; a launch holds at most 2^31 lanes (one grid
     // dimension holds fewer than 2^32 work-items)
     constexpr int64_t LANES = (int64_t)1 << 31;
     for (int64_t j0 = 0; j0 < g->m; j0 += LANES) {
       a.j0 = j0;
       const unsigned nb = (unsigned)((std::min(LANES, g->m - j0) + 255) / 256);
-#define LN_SAMPLE(M, P, O) hipLaunchKernelGGL((k_sample<M, P, O>), dim3(nb), dim3(256), 0, s, a)
-      switch ((p->sample ? 4 : 0) + (packed ? 2 : 0) + (off64 ? 1 : 0)) {
-        case 0: LN_SAMPLE(0, false, false); break;
-        case 1: LN_SAMPLE(0, false, true); break;
-        case 2: LN_SAMPLE(0, true, false); break;
-        case 3: LN_SAMPLE(0, true, true); break;
-        case 4: LN_SAMPLE(1, false, false); break;
-        case 5: LN_SAMPLE(1, false, true); break;
-        case 6: LN_SAMPLE(1, true, false); break;
-        default: LN_SAMPLE(1, true, true); break;
+#define LN_SAMPLE(M, P, O, R) hipLaunchKernelGGL((k_sample<M, P, O, R>), dim3(nb), dim3(256), 0, s, a)
+      // neighbour rows only with 32-bit offsets (off64 = 0)
+      switch ((nr ? 8 : 0) + (p->sample ? 4 : 0) + (packed ? 2 : 0) + (off64 ? 1 : 0)) {
+        case 0: LN_SAMPLE(0, false, false, false); break;
+        case 1: LN_SAMPLE(0, false, true, false); break;
+        case 2: LN_SAMPLE(0, true, false, false); break;
+        case 3: LN_SAMPLE(0, true, true, false); break;
+        case 4: LN_SAMPLE(1, false, false, false); break;
+        case 5: LN_SAMPLE(1, false, true, false); break;
+        case 6: LN_SAMPLE(1, true, false, false); break;
+        case 7: LN_SAMPLE(1, true, true, false); break;
+        case 8: LN_SAMPLE(0, false, false, true); break;
+        case 10: LN_SAMPLE(0, true, false, true); break;
+        case 12: LN_SAMPLE(1, false, false, true); break;
+        case 14: LN_SAMPLE(1, true, false, true); break;
+        default: return ctx_fail(ctx, LIGHTNE_ERR_HIP, "sampler: neighbour rows need 32-bit offsets");
       }
 #undef LN_SAMPLE
       LN_LAUNCH(ctx);

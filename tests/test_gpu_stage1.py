@@ -50,6 +50,43 @@ def test_counts_match_golden_and_oracle(gpu, rec):
         np.testing.assert_array_equal(t.keys, gd["keys"])
 
 
+@pytest.mark.parametrize("case", ["toy_t3_s1", "cl2000_t10_s0", "cl2000_t5_s1", "blog_t10_s0", "ccl2000_t10_s0",
+                                  "ccl2000_t5_s1"])
+def test_walk_layouts_draw_the_same_samples(gpu, case):
+    """The walks over neighbour rows (each directed edge's target with its row,
+    the default), over 32-bit row pointers and over the graph's 64-bit offsets
+    draw the same samples: equal counts, tables and CSR, bit for bit with
+    -sample 0 (with -sample 1 the FP64 weights are sums in atomic order)."""
+    rec = next(r for r in H.manifest() if r["case"] == case)
+    off, edg, g, ith = case_graph(rec, gpu)
+    kw = dict(walk_len=rec["walk_len"], sample=bool(rec["sample"]), sample_ratio=rec["sample_ratio"], ctx=gpu)
+    runs = {}
+    for w in (0, 2, 1):
+        t = L.sparsify_counts(g, int(rec["seed"]), rec["walk_len"], bool(rec["sample"]),
+                              sample_ratio=rec["sample_ratio"], wide_offsets=w, ctx=gpu)
+        csr, st = L.generate_trunc_log_matrix(g, int(rec["seed"]), wide_offsets=w, **kw)
+        runs[w] = (t, csr.to_numpy(), st)
+        csr.free()
+        assert st["walk_rows"] == (1 if w == 0 else 0) and t.stats["walk_rows"] == st["walk_rows"]
+    t0, c0, s0 = runs[0]
+    for w in (2, 1):
+        t, c, st = runs[w]
+        np.testing.assert_array_equal(t.keys, t0.keys)
+        np.testing.assert_array_equal(t.counts, t0.counts)
+        for k in ("samples", "walk_steps", "attempts", "distinct_keys"):
+            assert st[k] == s0[k], k
+        if not rec["sample"]:
+            np.testing.assert_array_equal(t.weights, t0.weights)
+            for x, y in zip(c, c0):
+                np.testing.assert_array_equal(x, y)
+        else:
+            np.testing.assert_allclose(t.weights, t0.weights, rtol=1e-12)
+    keys, cnt, wgh, ost = O.sample(off, edg, int(rec["seed"]), rec["walk_len"], rec["sample_ratio"], rec["sample"],
+                                   threads=1, ith=ith)
+    np.testing.assert_array_equal(t0.keys, keys)
+    np.testing.assert_array_equal(t0.counts.astype(np.uint64), cnt)
+
+
 def _compare_csr(csr, ref, ulps=1):
     """The device CSR against a reference, row by row as sets of (column,
     value): the same columns, values within `ulps`."""

@@ -50,7 +50,7 @@ def test_wide_offsets_walks_match_oracle(gpu, rec):
 def test_wide_offsets_flag_checked(gpu):
     g = L.chung_lu(500, 2.1, 6.0, 40.0, 1).to_device()
     with pytest.raises(L.LightNEError) as e:
-        L.sparsify_counts(g, 1, 5, sample_ratio=0.1, wide_offsets=2, ctx=gpu)
+        L.sparsify_counts(g, 1, 5, sample_ratio=0.1, wide_offsets=3, ctx=gpu)
     assert e.value.code == _lib.ERR_BAD_ARG
 
 
