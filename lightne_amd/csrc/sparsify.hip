@@ -316,12 +316,13 @@ __device__ __forceinline__ uint32_t walk(const OffT* __restrict__ off, const uin
   return x;
 }
 
-__global__ void k_nbr_rows(const int64_t* __restrict__ off, const uint32_t* __restrict__ adj, int64_t m,
+// (from the 32-bit row pointers: the gather's array is half the bytes)
+__global__ void k_nbr_rows(const uint32_t* __restrict__ off32, const uint32_t* __restrict__ adj, int64_t m,
                            NbrRow* __restrict__ nr) {
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t v = adj[j];
-    const int64_t o = off[v];
-    nr[j] = NbrRow{v, (uint32_t)(off[v + 1] - o), (uint64_t)o};
+    const uint32_t o = off32[v];
+    nr[j] = NbrRow{v, off32[v + 1] - o, (uint64_t)o};
   }
 }
 
@@ -1277,8 +1278,8 @@ int run_sampler(lightne_ctx* ctx, const lightne_graph* g, const lightne_sparsify
     a.stats = stats;
     LN_HIP(ctx, hipEventRecord(ctx->ev[1], s));
     if (nr) {
-      hipLaunchKernelGGL(k_nbr_rows, dim3(grid_blocks(g->m, 256, (int64_t)1 << 20)), dim3(256), 0, s, g->offsets,
-                         g->edges, g->m, nr);
+      hipLaunchKernelGGL(k_nbr_rows, dim3(grid_blocks(g->m, 256, (int64_t)1 << 20)), dim3(256), 0, s,
+                         (const uint32_t*)off32, g->edges, g->m, nr);
       LN_LAUNCH(ctx);
     }
     // one lane per directed edge; a launch holds at most 2^31 lanes (one grid
