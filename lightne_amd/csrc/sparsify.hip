@@ -4,7 +4,9 @@
 //  k_edge_src      : source vertex of every directed edge (binary search in
 //                    the offsets staged per tile of edges), so the sampler
 //                    can run one lane per edge.
-//  k_sample<MODE, PACKED> : one lane per directed edge (u, v); its
+//  k_nbr_rows      : every directed edge's target with its degree and first
+//                    edge, so a walk step is one random load.
+//  k_sample<MODE, PACKED, OFF64, NR> : one lane per directed edge (u, v); its
 //                    pbbs::random chain is seeded by fork((u << 32) | v) and
 //                    drives exp_rv path samples exactly as PathEmbed.h:128-171;
 //                    every sample is aggregated in an open-addressing table
@@ -14,23 +16,28 @@
 //                    an occupied bit and a 10-bit count whose wraps are added to
 //                    a small side table, so counts stay exact), else 16 bytes
 //                    {key, count}.
-//  k_table_reduce  : distinct keys and total weight (PathEmbed.h:175-188),
-//                    deterministic two-level reduction.
+//  k_table_canon + k_canon_fold : every table cluster sorted by key, so the
+//                    slot order depends on the key set alone; the occupied
+//                    slots per wave quarter of the trunc-log partition, whose
+//                    sum is the distinct keys (PathEmbed.h:175-186).
+//  k_table_reduce  : -sample 1: distinct keys and total weight
+//                    (PathEmbed.h:175-188), deterministic two-level reduction.
 //  k_trunc_count / k_trunc_emit : trunc-log (PathEmbed.h:199-218) fused with
 //                    the symmetric expansion (PathEmbed.h:278-311): kept keys emit
-//                    (row, col) sort keys and float values; per-block counts, a
-//                    scan, then writes in slot order (no contended atomics).
-//                    Where memory allows, the count pass also caches every
-//                    slot's value, so the emission reads no degrees.
-//  radix sort + k_csr_finish : stable LSD radix sort on (row, col), then
+//                    (row, col) sort keys and float values; per wave quarter
+//                    counts, a scan, then each wave writes its run in slot order
+//                    (no contended atomics, no block barriers).  Where memory
+//                    allows, the count pass also caches every distinct key's
+//                    value, so the emission reads no degrees.
+//  radix sort + k_csr_finish : stable LSD radix sort on the row bits (on
+//                    (row, col) when a cluster was too long to sort), then
 //                    row_ptr by boundary detection (no atomics, deterministic).
 //  by row ranges : when the sort buffers for every entry do not fit beside
 //                    the table and the CSR, the rows are cut into ranges by
-//                    the count pass's per-block row histograms; each range is
-//                    emitted from the table (a scan of its per-block counts,
-//                    ordered writes), sorted and written into its slice of the
-//                    CSR.  The (row, col)
-//                    keys are unique, so the result is the same as one global
+//                    the count pass's per-quarter row histograms; each range
+//                    is emitted from the table (a scan of its per-quarter
+//                    counts, ordered writes), sorted and written into its
+//                    slice of the CSR.  The result is the same as one global
 //                    sort.
 #include "lightne_internal.h"
 
