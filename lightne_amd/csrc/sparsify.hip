@@ -19,7 +19,7 @@
 //  k_table_canon + k_canon_fold : every table cluster sorted by key, so the
 //                    slot order depends on the key set alone; the occupied
 //                    slots per wave quarter of the trunc-log partition, whose
-//                    sum is the distinct keys (PathEmbed.h:175-186).
+//                    sum is the distinct keys (PathEmbed.h:181-186).
 //  k_table_reduce  : -sample 1: distinct keys and total weight
 //                    (PathEmbed.h:175-188), deterministic two-level reduction.
 //  k_trunc_count / k_trunc_emit : trunc-log (PathEmbed.h:199-218) fused with
