@@ -354,10 +354,13 @@ __device__ __forceinline__ uint32_t walk_nr(const NbrRow* __restrict__ nr, uint3
   return x;
 }
 
-// par_table::insert_add (par_table.h:86-109): count (and weight) of `key`;
-// 0 when the table is full, -1 when the packed layout's side table is full
+// par_table::insert_add (par_table.h:86-109): adds c samples (and with
+// MODE 1 the weight w) to `key`'s count; 0 when the table is full, -1 when
+// the packed layout's side table is full.  Packed counts keep their low 10
+// bits in the slot (an add of c * PK_ONE wraps them modulo 1024 without
+// touching the key bits) and 1024 per wrap in the side table.
 template <int MODE, bool PACKED>
-__device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint32_t hi, double w) {
+__device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint32_t hi, double w, uint64_t c = 1) {
   const uint64_t key = ((uint64_t)lo << 32) + (uint64_t)hi;
   uint64_t h = slot_of(key, a.cap);
   if (PACKED) {
@@ -365,18 +368,21 @@ __device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint3
     for (uint64_t p = 0; p < a.cap; ++p) {
       unsigned long long cur = __hip_atomic_load(&a.pk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (cur == 0ull) {
-        cur = atomicCAS(&a.pk[h], 0ull, (unsigned long long)(word | PK_ONE));
+        cur = atomicCAS(&a.pk[h], 0ull, (unsigned long long)(word | (c & 1023) * PK_ONE | (c >> 10 ? PK_OVF : 0)));
         if (cur == 0ull) {
+          int rc = 1;
+          if ((c >> 10) && !ovf_add(a.ovf, word, c & ~1023ull)) rc = -1;
           if (MODE == 1) atomicAdd(&a.wtab[h], w);
-          return 1;
+          return rc;
         }
       }
       if ((cur & PK_WORD) == word) {
-        const unsigned long long old = atomicAdd(&a.pk[h], (unsigned long long)PK_ONE);
+        const unsigned long long old = atomicAdd(&a.pk[h], (unsigned long long)(c * PK_ONE));
         int rc = 1;
-        if ((old >> 54) == 1023ull) {
+        const uint64_t wraps = ((old >> 54) + c) >> 10;
+        if (wraps) {
           atomicOr(&a.pk[h], (unsigned long long)PK_OVF);
-          if (!ovf_add(a.ovf, word, 1024)) rc = -1;
+          if (!ovf_add(a.ovf, word, 1024 * wraps)) rc = -1;
         }
         if (MODE == 1) atomicAdd(&a.wtab[h], w);
         return rc;
@@ -392,7 +398,7 @@ __device__ __forceinline__ int table_add(const SampleArgs& a, uint32_t lo, uint3
       if (cur == EMPTY_KEY) cur = key;
     }
     if (cur == key) {
-      atomicAdd(&a.tab[h].cnt, 1u);
+      atomicAdd(&a.tab[h].cnt, (unsigned int)c);
       if (MODE == 1) atomicAdd(&a.wtab[h], w);
       return 1;
     }
@@ -438,6 +444,9 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       p = p < 1.0 ? p : 1.0;
       ipw = 1.0 / p;
     }
+    // -sample 0: the samples of one walk length (r = 1) all hit the edge's own
+    // key; their count goes into the table in one add
+    uint64_t c_uv = 0;
     for (uint64_t k = 0; k < e; ++k) {
       if (MODE == 1) {
         const uint64_t q = hash64_2(s);
@@ -450,6 +459,11 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       rr = hash64_2(s);
       const uint32_t kk = (uint32_t)(rr % r + 1);  // path_sample, PathEmbed.h:54
       s = hash64_2(rr);
+      if (MODE == 0 && r == 1) {  // walks of no step: the edge itself, added once below
+        ++c_uv;
+        ++n_samples;
+        continue;
+      }
       uint32_t x, z;
       if (NR) {
         x = walk_nr(a.nr, u, (uint64_t)kk - 1, s, bad, a.ith, (uint64_t)ou, du_);
@@ -463,6 +477,11 @@ __global__ __launch_bounds__(256) void k_sample(SampleArgs a) {
       if (rc == 0) overflow = true;
       if (rc < 0) side_full = true;
       ++n_samples;
+    }
+    if (MODE == 0 && c_uv) {
+      const int rc = table_add<MODE, PACKED>(a, min(u, v), max(u, v), 1.0, c_uv);
+      if (rc == 0) overflow = true;
+      if (rc < 0) side_full = true;
     }
   }
   block_add3(a.stats, n_samples, n_steps, n_attempts);
@@ -561,8 +580,16 @@ __device__ __forceinline__ uint64_t cn_pos(uint64_t i, uint64_t j, uint64_t cap)
   return p >= cap ? p - cap : p;
 }
 
+// registers for seven waves per SIMD where the LDS allows seven blocks of
+// this size per CU (the packed layout without weights; the other layouts
+// stage more per slot and fit fewer)
 template <bool PK, bool WT>
-__global__ __launch_bounds__(CN_THREADS, 7) void k_table_canon(CanonArgs a) {
+constexpr int canon_min_waves() {
+  return PK && !WT ? 7 : 1;
+}
+
+template <bool PK, bool WT>
+__global__ __launch_bounds__(CN_THREADS, (canon_min_waves<PK, WT>())) void k_table_canon(CanonArgs a) {
   __shared__ uint64_t s_w[CN_STAGE];
   __shared__ uint32_t s_c[PK ? 1 : CN_STAGE];
   __shared__ double s_x[WT ? CN_STAGE : 1];

@@ -334,14 +334,18 @@ def test_table_layouts_agree(gpu, case):
         np.testing.assert_allclose(tp.weights, tw.weights, rtol=1e-12)
 
 
-def test_packed_counts_wrap_exactly(gpu):
+@pytest.mark.parametrize("walk_len,ratio", [(10, 1100.0), (1, 3000.0)])
+def test_packed_counts_wrap_exactly(gpu, walk_len, ratio):
     """Counts far above the packed slot's 10 bits (several wraps into the side
-    table) stay exact: packed == wide == oracle."""
+    table) stay exact: packed == wide == oracle.  With T = 1 every sample is
+    its edge, so each directed edge adds its whole count (thousands) to the
+    key in one add: a new slot that starts wrapped, and an add that wraps
+    several times."""
     off, edg = H.golden_graph("toy")
     g = dgraph(off, edg)
-    tp = L.sparsify_counts(g, 11, 10, False, sample_ratio=1100.0, table_layout=2, ctx=gpu)
-    tw = L.sparsify_counts(g, 11, 10, False, sample_ratio=1100.0, table_layout=1, ctx=gpu)
-    keys, cnt, _, st = O.sample(off, edg, 11, 10, 1100.0, 0, threads=8)
+    tp = L.sparsify_counts(g, 11, walk_len, False, sample_ratio=ratio, table_layout=2, ctx=gpu)
+    tw = L.sparsify_counts(g, 11, walk_len, False, sample_ratio=ratio, table_layout=1, ctx=gpu)
+    keys, cnt, _, st = O.sample(off, edg, 11, walk_len, ratio, 0, threads=8)
     assert int(cnt.max()) > 3 * 1024
     np.testing.assert_array_equal(tp.keys, keys)
     np.testing.assert_array_equal(tp.counts.astype(np.uint64), cnt)
