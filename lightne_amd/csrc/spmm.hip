@@ -11,6 +11,8 @@
 //
 // Rows are split by length (power-law graphs), in one launch:
 //   * short rows (nnz <= SHORT): one group per row, no cross-lane reduction;
+//     a wave's NG rows come from the plan's list of short rows grouped by
+//     length, so they take about the same number of steps;
 //   * long rows: chunks of CHUNK nnz, one wave per chunk, the NG groups of the
 //     wave stride over the chunk and are reduced by in-wave shuffles; rows with
 //     several chunks write partial sums that k_spmm_fixup adds in chunk order
@@ -133,7 +135,7 @@ __device__ __forceinline__ void gather_row(const SpmmArgs& a, int64_t j0, int64_
 }
 
 // One launch for both row classes.  Waves [0, W_short) take NG consecutive
-// rows each, one group per row, and skip rows longer than short_nnz; the
+// entries of the short-row list each, one group per row; the
 // remaining waves take one chunk of a long row each: the NG groups stride over
 // the chunk and are summed into group 0 by in-wave shuffles (group order, so
 // the result is deterministic), which applies the epilogue or writes the
@@ -149,10 +151,10 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int64_t w_short, int64
   for (int i = 0; i < 16; ++i) acc[i] = 0.f;
   if (wave < w_short) {
     if (g >= NG) return;
-    const int64_t r = wave * NG + g;
-    if (r >= a.n) return;
+    const int64_t i = wave * NG + g;
+    if (i >= a.nshort) return;
+    const int64_t r = a.short_rows[i];
     const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
-    if (j1 - j0 > a.short_nnz) return;
     gather_row<XT, IMPL>(a, j0, j1, 1, sub, acc);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -211,6 +213,8 @@ __global__ __launch_bounds__(256) void k_spmm_fixup(SpmmArgs a, int64_t wave0) {
 template <typename XT, int EPI, bool IMPL>
 int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   a.short_nnz = P.short_nnz;
+  a.nshort = P.nshort;
+  a.short_rows = P.short_rows;
   a.nchunks = P.nchunks;
   a.chunk_start = P.chunk_start;
   a.chunk_row = P.chunk_row;
@@ -224,7 +228,7 @@ int launch(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
     a.partial = (float*)ctx_buf(ctx, "spmm_partial", (size_t)P.nparts * a.k * sizeof(float));
     if (!a.partial) return LIGHTNE_ERR_OOM;
   }
-  const int64_t waves_short = (a.n + NG - 1) / NG;
+  const int64_t waves_short = (a.nshort + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
   for (int64_t w0 = 0; w0 < waves; w0 += MAX_WAVES) {
     hipLaunchKernelGGL((k_spmm<XT, EPI, IMPL>), dim3((unsigned)((std::min(MAX_WAVES, waves - w0) + 3) / 4)),
@@ -278,10 +282,10 @@ __global__ __launch_bounds__(256) void k_spmm64(SpmmArgs a, int64_t w_short, int
   auto* part = reinterpret_cast<double*>(a.partial);
   if (wave < w_short) {
     if (g >= NG) return;
-    const int64_t r = wave * NG + g;
-    if (r >= a.n) return;
+    const int64_t i = wave * NG + g;
+    if (i >= a.nshort) return;
+    const int64_t r = a.short_rows[i];
     const int64_t j0 = a.rp[r], j1 = a.rp[r + 1];
-    if (j1 - j0 > a.short_nnz) return;
     gather_row64(a, j0, j1, 1, sub, acc);
     store_row64(out + r * a.k, a.k, G, sub, acc);
     return;
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(256) void k_spmm64_fixup(SpmmArgs a, int64_t wave0)
 
 void spmm_plan_free(SpmmPlan& P) {
   for (void* p : {(void*)P.chunk_start, (void*)P.chunk_row, (void*)P.chunk_slot, (void*)P.multi_row,
-                  (void*)P.multi_first})
+                  (void*)P.multi_first, (void*)P.short_rows})
     if (p) (void)hipFree(p);
   P = SpmmPlan{};
 }
@@ -364,6 +368,52 @@ __global__ void k_plan_fill(const int64_t* __restrict__ rp, int64_t n, int chunk
   if (nc > 1) {
     multi_row[moff[r]] = (int32_t)r;
     multi_first[moff[r]] = (int64_t)poff[r];
+  }
+}
+
+// short rows by length: per block of 256 rows its short rows of every length,
+// length-major: cnt[len * gridDim.x + block]
+constexpr int PL_BINS = SHORT_NNZ + 1;  // lengths 0 .. SHORT_NNZ
+
+__global__ __launch_bounds__(256) void k_plan_short_count(const int64_t* __restrict__ rp, int64_t n,
+                                                          uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t s_c[PL_BINS];
+  for (int b = threadIdx.x; b < PL_BINS; b += 256) s_c[b] = 0;
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r < n) {
+    const int64_t len = rp[r + 1] - rp[r];
+    if (len <= SHORT_NNZ) atomicAdd(&s_c[len], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < PL_BINS; b += 256) cnt[(uint64_t)b * gridDim.x + blockIdx.x] = s_c[b];
+}
+
+// every short row to its place in the list: lengths in increasing order,
+// blocks in order within a length, rows of a block in index order (wave
+// ranks by ballot, waves in order)
+__global__ __launch_bounds__(256) void k_plan_short_fill(const int64_t* __restrict__ rp, int64_t n,
+                                                         const uint64_t* __restrict__ off, int32_t* __restrict__ rows) {
+  __shared__ uint32_t s_w[256 / 64][PL_BINS];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int bin = -1;
+  if (r < n) {
+    const int64_t len = rp[r + 1] - rp[r];
+    if (len <= SHORT_NNZ) bin = (int)len;
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t rank = 0;
+  for (int b = 0; b < PL_BINS; ++b) {
+    const uint64_t m = __ballot(bin == b);
+    if (bin == b) rank = (uint32_t)__popcll(m & below);
+    if (lane == 0) s_w[wid][b] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  if (bin >= 0) {
+    uint32_t before = 0;
+    for (int w = 0; w < wid; ++w) before += s_w[w][bin];
+    rows[off[(uint64_t)bin * gridDim.x + blockIdx.x] + before + rank] = (int32_t)r;
   }
 }
 
@@ -417,6 +467,20 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
                      (const uint64_t*)moff, (const uint64_t*)poff, P.chunk_start, P.chunk_row, P.chunk_slot,
                      P.multi_row, P.multi_first);
   LN_LAUNCH(ctx);
+  // the short rows grouped by length
+  auto* scnt = (uint32_t*)ctx_buf(ctx, "plan_scnt", (size_t)PL_BINS * nb * 4);
+  auto* soff = (uint64_t*)ctx_buf(ctx, "plan_soff", ((size_t)PL_BINS * nb + 1) * 8);
+  if (!scnt || !soff) return LIGHTNE_ERR_OOM;
+  hipLaunchKernelGGL(k_plan_short_count, dim3(nb), dim3(256), 0, s, rp_dev, n, scnt);
+  LN_LAUNCH(ctx);
+  LN_CHECK(scan_u32_to_u64(ctx, scnt, soff, (int64_t)PL_BINS * nb));
+  uint64_t ns = 0;
+  LN_HIP(ctx, hipMemcpyAsync(&ns, soff + (size_t)PL_BINS * nb, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipStreamSynchronize(s));
+  P.nshort = (int64_t)ns;
+  LN_CHECK(al((void**)&P.short_rows, P.nshort * sizeof(int32_t)));
+  hipLaunchKernelGGL(k_plan_short_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, (const uint64_t*)soff, P.short_rows);
+  LN_LAUNCH(ctx);
   return LIGHTNE_OK;
 }
 
@@ -463,6 +527,8 @@ int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
   if ((a.ldx && a.ldx != a.k) || (a.ldo && a.ldo != a.k))
     return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: the FP64 form needs contiguous panels");
   a.short_nnz = P.short_nnz;
+  a.nshort = P.nshort;
+  a.short_rows = P.short_rows;
   a.nchunks = P.nchunks;
   a.chunk_start = P.chunk_start;
   a.chunk_row = P.chunk_row;
@@ -476,7 +542,7 @@ int spmm_run64(lightne_ctx* ctx, const SpmmPlan& P, SpmmArgs a) {
     a.partial = (float*)ctx_buf(ctx, "spmm_partial64", (size_t)P.nparts * a.k * sizeof(double));
     if (!a.partial) return LIGHTNE_ERR_OOM;
   }
-  const int64_t waves_short = (a.n + NG - 1) / NG;
+  const int64_t waves_short = (a.nshort + NG - 1) / NG;
   const int64_t waves = waves_short + P.nchunks;
   for (int64_t w0 = 0; w0 < waves; w0 += MAX_WAVES) {
     hipLaunchKernelGGL(k_spmm64, dim3((unsigned)((std::min(MAX_WAVES, waves - w0) + 3) / 4)), dim3(256), 0,

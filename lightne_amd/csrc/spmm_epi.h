@@ -22,6 +22,10 @@ enum SpmmImpl {
 struct SpmmPlan {
   int64_t n = 0;
   int short_nnz = 0, chunk_nnz = 0;
+  // the rows of at most short_nnz entries, grouped by length (the NG rows a
+  // wave takes together then take about the same time)
+  int64_t nshort = 0;
+  int32_t* short_rows = nullptr;
   int64_t nchunks = 0, nmulti = 0, nparts = 0;
   int64_t* chunk_start = nullptr;  // 2 * nchunks: (start, end) pairs
   int32_t* chunk_row = nullptr;
@@ -59,6 +63,8 @@ struct SpmmArgs {
   float* y_out = nullptr;
   // plan (filled by spmm_run)
   int short_nnz = 0, G = 0;
+  int64_t nshort = 0;
+  const int32_t* short_rows = nullptr;
   int64_t nchunks = 0;
   const int64_t* chunk_start = nullptr;
   const int32_t* chunk_row = nullptr;
