@@ -327,9 +327,7 @@ __global__ __launch_bounds__(256) void k_spmm64_fixup(SpmmArgs a, int64_t wave0)
 }  // namespace
 
 void spmm_plan_free(SpmmPlan& P) {
-  for (void* p : {(void*)P.chunk_start, (void*)P.chunk_row, (void*)P.chunk_slot, (void*)P.multi_row,
-                  (void*)P.multi_first, (void*)P.short_rows})
-    if (p) (void)hipFree(p);
+  if (P.mem) (void)hipFree(P.mem);
   P = SpmmPlan{};
 }
 
@@ -439,46 +437,52 @@ int spmm_plan_build(lightne_ctx* ctx, const int64_t* rp_dev, int64_t n, SpmmPlan
   auto* poff = (uint64_t*)ctx_buf(ctx, "plan_poff", (n + 1) * 8);
   if (!nch || !mul || !par || !coff || !moff || !poff) return LIGHTNE_ERR_OOM;
   const unsigned nb = (unsigned)((n + 255) / 256);
+  auto* scnt = (uint32_t*)ctx_buf(ctx, "plan_scnt", (size_t)PL_BINS * nb * 4);
+  auto* soff = (uint64_t*)ctx_buf(ctx, "plan_soff", ((size_t)PL_BINS * nb + 1) * 8);
+  if (!scnt || !soff) return LIGHTNE_ERR_OOM;
   hipLaunchKernelGGL(k_plan_count, dim3(nb), dim3(256), 0, s, rp_dev, n, SHORT_NNZ, chunk_nnz, nch, mul, par);
+  LN_LAUNCH(ctx);
+  hipLaunchKernelGGL(k_plan_short_count, dim3(nb), dim3(256), 0, s, rp_dev, n, scnt);
   LN_LAUNCH(ctx);
   LN_CHECK(scan_u32_to_u64(ctx, nch, coff, n));
   LN_CHECK(scan_u32_to_u64(ctx, mul, moff, n));
   LN_CHECK(scan_u32_to_u64(ctx, par, poff, n));
-  uint64_t tot[3];
+  LN_CHECK(scan_u32_to_u64(ctx, scnt, soff, (int64_t)PL_BINS * nb));
+  uint64_t tot[4];
   LN_HIP(ctx, hipMemcpyAsync(&tot[0], coff + n, 8, hipMemcpyDeviceToHost, s));
   LN_HIP(ctx, hipMemcpyAsync(&tot[1], moff + n, 8, hipMemcpyDeviceToHost, s));
   LN_HIP(ctx, hipMemcpyAsync(&tot[2], poff + n, 8, hipMemcpyDeviceToHost, s));
+  LN_HIP(ctx, hipMemcpyAsync(&tot[3], soff + (size_t)PL_BINS * nb, 8, hipMemcpyDeviceToHost, s));
   LN_HIP(ctx, hipStreamSynchronize(s));
   P.nchunks = (int64_t)tot[0];
   P.nmulti = (int64_t)tot[1];
   P.nparts = (int64_t)tot[2];
+  P.nshort = (int64_t)tot[3];
   if (P.nparts >= ((int64_t)1 << 31)) return ctx_fail(ctx, LIGHTNE_ERR_BAD_ARG, "spmm: too many partial chunks");
-  auto al = [&](void** dst, size_t bytes) -> int {
-    if (ctx_malloc(ctx, dst, bytes ? bytes : 16) != hipSuccess)
-      return ctx_fail(ctx, LIGHTNE_ERR_OOM, "spmm plan allocation failed");
-    return LIGHTNE_OK;
-  };
-  LN_CHECK(al((void**)&P.chunk_start, (2 * P.nchunks + 2) * sizeof(int64_t)));
-  LN_CHECK(al((void**)&P.chunk_row, P.nchunks * sizeof(int32_t)));
-  LN_CHECK(al((void**)&P.chunk_slot, P.nchunks * sizeof(int32_t)));
-  LN_CHECK(al((void**)&P.multi_row, P.nmulti * sizeof(int32_t)));
-  LN_CHECK(al((void**)&P.multi_first, (P.nmulti + 1) * sizeof(int64_t)));
+  // the plan's arrays in one allocation (freed with the plan)
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t b_cs = al((2 * P.nchunks + 2) * sizeof(int64_t)), b_cr = al(P.nchunks * sizeof(int32_t)),
+               b_mr = al(P.nmulti * sizeof(int32_t)), b_mf = al((P.nmulti + 1) * sizeof(int64_t)),
+               b_sr = al(P.nshort * sizeof(int32_t));
+  if (ctx_malloc(ctx, &P.mem, b_cs + 2 * b_cr + b_mr + b_mf + b_sr) != hipSuccess)
+    return ctx_fail(ctx, LIGHTNE_ERR_OOM, "spmm plan allocation failed");
+  char* q = (char*)P.mem;
+  P.chunk_start = (int64_t*)q;
+  q += b_cs;
+  P.chunk_row = (int32_t*)q;
+  q += b_cr;
+  P.chunk_slot = (int32_t*)q;
+  q += b_cr;
+  P.multi_row = (int32_t*)q;
+  q += b_mr;
+  P.multi_first = (int64_t*)q;
+  q += b_mf;
+  P.short_rows = (int32_t*)q;
   hipLaunchKernelGGL(k_plan_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, chunk_nnz, (const uint64_t*)coff,
                      (const uint64_t*)moff, (const uint64_t*)poff, P.chunk_start, P.chunk_row, P.chunk_slot,
                      P.multi_row, P.multi_first);
   LN_LAUNCH(ctx);
   // the short rows grouped by length
-  auto* scnt = (uint32_t*)ctx_buf(ctx, "plan_scnt", (size_t)PL_BINS * nb * 4);
-  auto* soff = (uint64_t*)ctx_buf(ctx, "plan_soff", ((size_t)PL_BINS * nb + 1) * 8);
-  if (!scnt || !soff) return LIGHTNE_ERR_OOM;
-  hipLaunchKernelGGL(k_plan_short_count, dim3(nb), dim3(256), 0, s, rp_dev, n, scnt);
-  LN_LAUNCH(ctx);
-  LN_CHECK(scan_u32_to_u64(ctx, scnt, soff, (int64_t)PL_BINS * nb));
-  uint64_t ns = 0;
-  LN_HIP(ctx, hipMemcpyAsync(&ns, soff + (size_t)PL_BINS * nb, 8, hipMemcpyDeviceToHost, s));
-  LN_HIP(ctx, hipStreamSynchronize(s));
-  P.nshort = (int64_t)ns;
-  LN_CHECK(al((void**)&P.short_rows, P.nshort * sizeof(int32_t)));
   hipLaunchKernelGGL(k_plan_short_fill, dim3(nb), dim3(256), 0, s, rp_dev, n, (const uint64_t*)soff, P.short_rows);
   LN_LAUNCH(ctx);
   return LIGHTNE_OK;

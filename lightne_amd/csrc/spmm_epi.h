@@ -20,6 +20,7 @@ enum SpmmImpl {
 };
 
 struct SpmmPlan {
+  void* mem = nullptr;  // one allocation holding every array below
   int64_t n = 0;
   int short_nnz = 0, chunk_nnz = 0;
   // the rows of at most short_nnz entries, grouped by length (the NG rows a
